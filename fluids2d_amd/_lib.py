@@ -1,0 +1,140 @@
+"""ctypes binding of libfluids2d_amd.so (the C ABI of include/fluids2d_amd.h).
+
+The library is the only compute path of the package: if it is missing or the
+device is unavailable every entry point raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libfluids2d_amd.so")
+
+F64, F32 = 0, 1
+METHODS = {"weno": 0, "upwind": 1, "centered": 2, "cweno": 3}
+INTEGRATORS = {"rk3": 0, "ef": 1, "enrk3": 2}
+NSTAGES = {"rk3": 3, "ef": 1, "enrk3": 3}
+POISSON_F64, POISSON_F32_REFINE, POISSON_F32 = 0, 1, 2
+
+EXPORTS = [
+    "f2d_abi_version", "f2d_last_error", "f2d_device_synchronize",
+    "f2d_set_masks", "f2d_set_order", "f2d_slipcoef", "f2d_gindex",
+    "f2d_compflux", "f2d_vortexforce", "f2d_innerproduct",
+    "f2d_transport_stage", "f2d_rk_addto", "f2d_c2v", "f2d_perpgrad",
+    "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar",
+    "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
+    "f2d_poisson_solve_c2v", "f2d_poisson_stats",
+]
+
+
+class F2DError(RuntimeError):
+    pass
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("ny", ctypes.c_int64), ("nx", ctypes.c_int64),
+                ("halowidth", ctypes.c_int32), ("xperiodic", ctypes.c_int32),
+                ("yperiodic", ctypes.c_int32), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double)]
+
+
+class PoissonOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("nu", ctypes.c_int32),
+                ("coarse_sweeps", ctypes.c_int32),
+                ("refine_inner", ctypes.c_int32),
+                ("tail_points", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises F2DError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise F2DError(f"{LIBPATH} is not built: run `python "
+                       f"fluids2d_amd/csrc/build.py` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIBPATH)
+    vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    gp = ctypes.POINTER(Grid)
+    sigs = {
+        "f2d_abi_version": (i32, []),
+        "f2d_last_error": (ctypes.c_char_p, []),
+        "f2d_device_synchronize": (i32, []),
+        "f2d_set_masks": (i32, [vp, i64, i64, vp, vp, vp, vp]),
+        "f2d_set_order": (i32, [vp, i64, i64, i32, vp, vp]),
+        "f2d_slipcoef": (i32, [vp, i64, i64, i32, i32, i32, vp, vp]),
+        "f2d_gindex": (i32, [vp, i64, vp, ctypes.POINTER(i64), vp]),
+        "f2d_compflux": (i32, [i32, vp, vp, vp, vp, i64, i64, i32, vp]),
+        "f2d_vortexforce": (i32, [i32, vp, vp, vp, vp, i64, i64, i64, i32, i32, vp]),
+        "f2d_innerproduct": (i32, [i32, vp, vp, vp, vp, i64, i64, i32, vp]),
+        "f2d_transport_stage": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
+                                      vp, vp, vp, i32, i32, vp, vp]),
+        "f2d_rk_addto": (i32, [i32, vp, vp, vp, vp, i64, i32, i32, vp, vp]),
+        "f2d_c2v": (i32, [i32, gp, vp, vp, vp, i32, vp]),
+        "f2d_perpgrad": (i32, [i32, gp, vp, vp, vp, vp, vp, i32, vp, vp]),
+        "f2d_perpgrad_ex": (i32, [i32, gp, vp, vp, vp, vp, vp, d, d, i32, vp, vp,
+                                  d, d, vp]),
+        "f2d_fill_halo": (i32, [i32, gp, vp, vp]),
+        "f2d_cfl_dt": (i32, [i32, vp, vp, i64, d, d, vp, vp]),
+        "f2d_set_scalar": (i32, [vp, d, vp]),
+        "f2d_poisson_create": (i32, [vp, gp, d, ctypes.POINTER(PoissonOpts),
+                                     ctypes.POINTER(vp)]),
+        "f2d_poisson_destroy": (i32, [vp]),
+        "f2d_poisson_solve": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
+                                    ctypes.POINTER(d), vp]),
+        "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
+                                        ctypes.POINTER(d), vp]),
+        "f2d_poisson_stats": (vp, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().f2d_last_error().decode(errors="replace")
+        raise F2DError(f"fluids2d_amd: {msg} (code {rc})")
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def dtype_code(dt):
+    import torch
+    if dt == torch.float64:
+        return F64
+    if dt == torch.float32:
+        return F32
+    raise F2DError(f"unsupported dtype {dt}")
+
+
+def make_grid(param):
+    g = Grid()
+    g.ny, g.nx = param.ny, param.nx
+    g.halowidth = param.halowidth
+    g.xperiodic = int(bool(param.xperiodic))
+    g.yperiodic = int(bool(getattr(param, "yperiodic", False)))
+    g.dx = param.Lx / param.nx
+    g.dy = param.Ly / param.ny
+    return g

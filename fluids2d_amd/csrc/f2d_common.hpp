@@ -1,0 +1,340 @@
+// f2d_common.hpp -- shared device/host helpers of the fluids2d_amd library.
+//
+// All kernels are compiled with -ffp-contract=off: the FP64 path reproduces
+// the reference's (numba, no fastmath) IEEE double arithmetic operation by
+// operation, so FMA contraction must not merge a multiply and an add.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/fluids2d_amd.h"
+
+namespace f2d {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+  std::string msg;
+};
+
+#define F2D_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      throw ::f2d::Error{std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define F2D_REQUIRE(cond, msg)                                  \
+  do {                                                          \
+    if (!(cond)) throw ::f2d::Error{std::string(msg)};          \
+  } while (0)
+
+// Wrap an entry point body: exceptions become a status code + message.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error(std::string("kernel launch: ") + hipGetErrorString(e));
+      return -2;
+    }
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return -3;
+  }
+}
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int grid1d(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  return (int)(b < 1 ? 1 : b);
+}
+
+// ---------------------------------------------------------------------------
+// pointwise reconstructions (weno.py:21-43, 75-134, 167-197, 255-300),
+// evaluation order as in the Python source; T = double reproduces it exactly.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T ce2(T qm, T qp) {
+  return (qm + qp) * T(0.5);
+}
+template <class T>
+__device__ __forceinline__ T ce4(T qmm, T qm, T qp, T qpp) {
+  return ((-qmm + T(7) * (qm + qp)) - qpp) / T(12);
+}
+template <class T>
+__device__ __forceinline__ T ce6(T qmmm, T qmm, T qm, T qp, T qpp, T qppp) {
+  return (((qmmm - T(8) * (qmm + qpp)) + T(37) * (qm + qp)) + qppp) / T(60);
+}
+template <class T>
+__device__ __forceinline__ T up3(T qm, T q0, T qp) {
+  return ((T(5) * q0 + T(2) * qp) - qm) / T(6);
+}
+template <class T>
+__device__ __forceinline__ T up5(T qmmm, T qmm, T qm, T qp, T qpp) {
+  return ((((T(2) * qmmm - T(13) * qmm) + T(47) * qm) + T(27) * qp) -
+          T(3) * qpp) /
+         T(60);
+}
+
+// weno.py:75-100
+template <class T>
+__device__ __forceinline__ T weno3z(T qm, T q0, T qp) {
+  const T eps = T(1e-14);
+  T qi1 = T(-1. / 2.) * qm + T(3. / 2.) * q0;
+  T qi2 = T(1. / 2.) * (q0 + qp);
+  T d1 = q0 - qm, d2 = qp - q0;
+  T beta1 = d1 * d1, beta2 = d2 * d2;
+  T tau = fabs(beta2 - beta1);
+  T w1 = T(1. / 3.) * (T(1) + tau / (beta1 + eps));
+  T w2 = T(2. / 3.) * (T(1) + tau / (beta2 + eps));
+  return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+}
+
+// weno.py:103-134
+template <class T>
+__device__ __forceinline__ T cweno3z(T U, T qmm, T qm, T qp, T qpp) {
+  const T eps = T(1e-14);
+  T qi1 = T(-1. / 2.) * qmm + T(3. / 2.) * qm;
+  T qi2 = T(1. / 2.) * (qm + qp);
+  T qi3 = T(-1. / 2.) * qpp + T(3. / 2.) * qp;
+  T a = (U > T(0)) ? (qm - qmm) : (qp - qpp);
+  T b = qp - qm;
+  T beta1 = a * a, beta2 = b * b;
+  T tau = fabs(beta2 - beta1);
+  T w1 = T(1. / 3.) * (T(1) + tau / (beta1 + eps));
+  T w2 = T(2. / 3.) * (T(1) + tau / (beta2 + eps));
+  return ((w1 * (qi1 + qi3)) * T(0.5) + w2 * qi2) / (w1 + w2);
+}
+
+// weno.py:167-197
+template <class T>
+__device__ __forceinline__ T weno5z(T qmm, T qm, T q0, T qp, T qpp) {
+  const T eps = T(1e-16);
+  T qi1 = (T(1. / 3.) * qmm - T(7. / 6.) * qm) + T(11. / 6.) * q0;
+  T qi2 = (T(-1. / 6.) * qm + T(5. / 6.) * q0) + T(1. / 3.) * qp;
+  T qi3 = (T(1. / 3.) * q0 + T(5. / 6.) * qp) - T(1. / 6.) * qpp;
+  const T k1 = T(13. / 12.), k2 = T(0.25);
+  T a1 = (qmm - T(2) * qm) + q0, b1 = (qmm - T(4) * qm) + T(3) * q0;
+  T a2 = (qm - T(2) * q0) + qp, b2 = qm - qp;
+  T a3 = (q0 - T(2) * qp) + qpp, b3 = (T(3) * q0 - T(4) * qp) + qpp;
+  T beta1 = k1 * (a1 * a1) + k2 * (b1 * b1);
+  T beta2 = k1 * (a2 * a2) + k2 * (b2 * b2);
+  T beta3 = k1 * (a3 * a3) + k2 * (b3 * b3);
+  T tau5 = fabs(beta1 - beta3);
+  T w1 = T(0.1) * (T(1) + tau5 / (beta1 + eps));
+  T w2 = T(0.6) * (T(1) + tau5 / (beta2 + eps));
+  T w3 = T(0.3) * (T(1) + tau5 / (beta3 + eps));
+  return ((w1 * qi1 + w2 * qi2) + w3 * qi3) / ((w1 + w2) + w3);
+}
+
+// weno.py:255-300 (cweno5z_v0)
+template <class T>
+__device__ __forceinline__ T cweno5z(T qmmm, T qmm, T qm, T qp, T qpp,
+                                     T qppp) {
+  const T eps = T(1e-16);
+  T qi1 = (T(1. / 3.) * qmmm - T(7. / 6.) * qmm) + T(11. / 6.) * qm;
+  T qi2 = (T(-1. / 6.) * qmm + T(5. / 6.) * qm) + T(1. / 3.) * qp;
+  T qi3 = (T(1. / 3.) * qm + T(5. / 6.) * qp) - T(1. / 6.) * qpp;
+  T qi4 = (T(1. / 3.) * qp + T(5. / 6.) * qm) - T(1. / 6.) * qmm;
+  T qi5 = (T(-1. / 6.) * qpp + T(5. / 6.) * qp) + T(1. / 3.) * qm;
+  T qi6 = (T(1. / 3.) * qppp - T(7. / 6.) * qpp) + T(11. / 6.) * qp;
+  const T k1 = T(13. / 12.), k2 = T(0.25);
+  T a1 = (qmmm - T(2) * qmm) + qm, b1 = (qmmm - T(4) * qmm) + T(3) * qm;
+  T a2 = (qmm - T(2) * qm) + qp, b2 = qmm - qp;
+  T a3 = (qm - T(2) * qp) + qpp, b3 = (T(3) * qm - T(4) * qp) + qpp;
+  T beta1 = k1 * (a1 * a1) + k2 * (b1 * b1);
+  T beta2 = k1 * (a2 * a2) + k2 * (b2 * b2);
+  T beta3 = k1 * (a3 * a3) + k2 * (b3 * b3);
+  T tau5 = fabs(beta1 - beta3);
+  T w1 = T(0.1) * (T(1) + tau5 / (beta1 + eps));
+  T w2 = T(0.6) * (T(1) + tau5 / (beta2 + eps));
+  T w3 = T(0.3) * (T(1) + tau5 / (beta3 + eps));
+  return ((w1 * (qi1 + qi6) + w2 * (qi2 + qi5)) + w3 * (qi3 + qi4)) /
+         (T(2) * ((w1 + w2) + w3));
+}
+
+// Upwind selection, weno.py:303-343.  The WENO branch selects the stencil
+// operands first and evaluates weno5z once, so a wavefront whose lanes see
+// both signs of U does not run both evaluations.
+template <class T>
+__device__ __forceinline__ T f1(int method, T U, T qm, T qp) {
+  if (method >= F2D_CENTERED) return ce2(qm, qp);
+  return U > T(0) ? qm : qp;
+}
+
+template <class T>
+__device__ __forceinline__ T f3(int method, T U, T qmm, T qm, T qp, T qpp) {
+  const bool pos = U > T(0);
+  switch (method) {
+    case F2D_WENO:
+      return pos ? weno3z(qmm, qm, qp) : weno3z(qpp, qp, qm);
+    case F2D_UPWIND:
+      return pos ? up3(qmm, qm, qp) : up3(qpp, qp, qm);
+    case F2D_CENTERED:
+      return ce4(qmm, qm, qp, qpp);
+    default:
+      return cweno3z(U, qmm, qm, qp, qpp);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T f5(int method, T U, T qmmm, T qmm, T qm, T qp,
+                                T qpp, T qppp) {
+  const bool pos = U > T(0);
+  switch (method) {
+    case F2D_WENO: {
+      T a = pos ? qmmm : qppp, b = pos ? qmm : qpp, c = pos ? qm : qp;
+      T d = pos ? qp : qm, e = pos ? qpp : qmm;
+      return weno5z(a, b, c, d, e);
+    }
+    case F2D_UPWIND:
+      return pos ? up5(qmmm, qmm, qm, qp, qpp) : up5(qppp, qpp, qp, qm, qmm);
+    case F2D_CENTERED:
+      return ce6(qmmm, qmm, qm, qp, qpp, qppp);
+    default:
+      return cweno5z(qmmm, qmm, qm, qp, qpp, qppp);
+  }
+}
+
+// Specialised WENO selection used by the fused kernels (method known at
+// compile time through the template argument).
+template <class T, int M>
+__device__ __forceinline__ T flux_by_order(int o, T U, T q_3, T q_2, T q_1,
+                                           T q0, T q1, T q2) {
+  // stencil around the face between q_1 and q0 (compflux, weno.py:352-363)
+  if (o > 4) return f5<T>(M, U, q_3, q_2, q_1, q0, q1, q2);
+  if (o > 2) return f3<T>(M, U, q_2, q_1, q0, q1);
+  if (o > 0) return f1<T>(M, U, q_1, q0);
+  return T(0);
+}
+
+// ---------------------------------------------------------------------------
+// RK stage coefficients evaluated from dt exactly as the reference's Python
+// expressions (integrators.py:82-124).  Returns the number of coefficients.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int rk_coefs(int integrator, int stage, double dt,
+                                        double c[3]) {
+  c[0] = c[1] = c[2] = 0.0;
+  if (integrator == F2D_RK3) {
+    if (stage == 0) { c[0] = dt; return 1; }
+    if (stage == 1) { c[0] = (-3.0 * dt) / 4.0; c[1] = dt / 4.0; return 2; }
+    c[0] = (-dt) / 12.0; c[1] = (-dt) / 12.0; c[2] = (2.0 * dt) / 3.0;
+    return 3;
+  }
+  if (integrator == F2D_EF) { c[0] = dt; return 1; }
+  // F2D_ENRK3
+  if (stage == 0) { c[0] = dt / 3.0; return 1; }
+  if (stage == 1) {
+    c[0] = ((-dt) / 3.0) - ((5.0 * dt) / 48.0);
+    c[1] = (15.0 * dt) / 16.0;
+    return 2;
+  }
+  c[0] = ((5.0 * dt) / 48.0) + (dt / 10.0);
+  c[1] = ((-7.0) * dt) / 16.0;
+  c[2] = (2.0 * dt) / 5.0;
+  return 3;
+}
+
+// y + ((0 + c0 x0) + c1 x1) + c2 x2 with Python's left-to-right sum
+// (integrators.py:174); the leading 0 + v is v.
+template <class T>
+__device__ __forceinline__ T rk_combine(int nc, const double c[3], T x0, T x1,
+                                        T x2) {
+  T s = T(c[0]) * x0;
+  if (nc > 1) s = s + T(c[1]) * x1;
+  if (nc > 2) s = s + T(c[2]) * x2;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic grid reductions: every block writes its partial, the last
+// block to arrive (device-scope counter) reduces the partials in a fixed
+// order and runs a finaliser.  Counters reset themselves.
+// ---------------------------------------------------------------------------
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <int BS>
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void store_partial(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_partial(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in the last block (all threads of it).  `nvals` partial
+// values per block are stored at partials[k * nblocks + block].
+template <int BS>
+__device__ __forceinline__ bool last_block_arrive(unsigned* counter,
+                                                  int* sh_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    *sh_flag = (prev == nb - 1);
+    if (prev == nb - 1)
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  bool last = *sh_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// Sum (or max) of nb partials in a fixed order by one block.
+template <int BS>
+__device__ __forceinline__ double reduce_partials(const double* part, int nb,
+                                                  double* sh, bool is_max) {
+  double acc = is_max ? 0.0 : 0.0;
+  for (int i = threadIdx.x; i < nb; i += BS) {
+    double v = load_partial(part + i);
+    acc = is_max ? fmax(acc, v) : acc + v;
+  }
+  return is_max ? block_max<BS>(acc, sh) : block_sum<BS>(acc, sh);
+}
+
+inline int block_count(dim3 g) { return (int)(g.x * g.y * g.z); }
+
+}  // namespace f2d

@@ -1,0 +1,734 @@
+// f2d_stencil.hip -- WENO flux loops, the fused transport stage and the
+// C-grid point operators of the eulerpsi/advection/qg paths.
+//
+//   flat loops          weno.py:346-405, 412-436
+//   fused stage         operators.py:21-29,104-107 + meshes.py:106-111 +
+//                       integrators.py:92-124,154-196
+//   c2v / perpgrad      operators.py:126-133, 144-149
+//   halo fill           meshes.py:135-143 (+ y-periodic extension)
+//   CFL time step       model.py:71-87
+#include "f2d_common.hpp"
+
+namespace f2d {
+
+// numba wraps negative indices; an index past the end (never reached with
+// orders from set_order) is wrapped too so that no access leaves the array.
+__device__ __forceinline__ int64_t wr(int64_t i, int64_t n) {
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// ---------------------------------------------------------------------------
+// flat-index loops, one thread per point
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_compflux(T* __restrict__ flx, const T* __restrict__ U,
+                           const T* __restrict__ q, const int8_t* __restrict__ o,
+                           int64_t n, int64_t s, int method) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int oi = o[i];
+  T u = U[i], r;
+  if (oi > 4)
+    r = f5<T>(method, u, q[wr(i - 3 * s, n)], q[wr(i - 2 * s, n)],
+              q[wr(i - s, n)], q[i], q[wr(i + s, n)], q[wr(i + 2 * s, n)]) *
+        u;
+  else if (oi > 2)
+    r = f3<T>(method, u, q[wr(i - 2 * s, n)], q[wr(i - s, n)], q[i],
+              q[wr(i + s, n)]) *
+        u;
+  else if (oi > 0)
+    r = f1<T>(method, u, q[wr(i - s, n)], q[i]) * u;
+  else
+    r = T(0);
+  flx[i] = r;
+}
+
+template <class T>
+__global__ void k_vortexforce(T* __restrict__ du, const T* __restrict__ V,
+                              const T* __restrict__ q,
+                              const int8_t* __restrict__ o, int64_t n,
+                              int64_t s, int64_t s2, int sign, int method) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int oi = o[i];
+  if (oi <= 0) {
+    du[i] = T(0);
+    return;
+  }
+  T Vm = T(0.25) * (((V[i] + V[wr(i + s, n)]) + V[wr(i - s2, n)]) +
+                    V[wr(i + s - s2, n)]);
+  T f;
+  if (oi > 4)
+    f = f5<T>(method, Vm, q[wr(i - 2 * s, n)], q[wr(i - s, n)], q[i],
+              q[wr(i + s, n)], q[wr(i + 2 * s, n)], q[wr(i + 3 * s, n)]);
+  else if (oi > 2)
+    f = f3<T>(method, Vm, q[wr(i - s, n)], q[i], q[wr(i + s, n)],
+              q[wr(i + 2 * s, n)]);
+  else
+    f = f1<T>(method, Vm, q[i], q[wr(i + s, n)]);
+  du[i] = (T(sign) * f) * Vm;
+}
+
+template <class T>
+__global__ void k_innerproduct(T* __restrict__ ke, const T* __restrict__ U,
+                               const T* __restrict__ q,
+                               const int8_t* __restrict__ o, int64_t n,
+                               int64_t s, int method) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int oi = o[i];
+  if (oi <= 0) return;
+  T Um = T(0.5) * (U[i] + U[wr(i + s, n)]);
+  T f;
+  if (oi > 4)
+    f = f5<T>(method, Um, q[wr(i - 2 * s, n)], q[wr(i - s, n)], q[i],
+              q[wr(i + s, n)], q[wr(i + 2 * s, n)], q[wr(i + 3 * s, n)]);
+  else if (oi > 2)
+    f = f3<T>(method, Um, q[wr(i - s, n)], q[i], q[wr(i + s, n)],
+              q[wr(i + 2 * s, n)]);
+  else
+    f = f1<T>(method, Um, q[i], q[wr(i + s, n)]);
+  ke[i] = ke[i] + f * Um;
+}
+
+// ---------------------------------------------------------------------------
+// Fused transport stage.
+//
+// Interior cells [3, 3+ny) x [3, 3+nx) are covered by TY x TX tiles: the tile
+// of q with its 3-cell halo is staged in LDS once, every x- and y-face flux
+// of the tile is computed once into LDS, then each cell forms
+//   ds = ((-(Fx[i+1]-Fx[i])) - (Fy[j+1]-Fy[j])) * msk          (div)
+// and q_out = q_in + S(ds_prev, ds)                           (addto).
+// Cells of the halo ring are handled by extra blocks: a periodic halo cell
+// takes the tendency of its wrapped interior source (what fill() copies), a
+// closed-direction halo cell (dry: msk = 0) gets ds = 0.
+// ---------------------------------------------------------------------------
+template <class T>
+struct TransportArgs {
+  const T* q;
+  T* qo;
+  const T* Ux;
+  const T* Uy;
+  const int8_t* ocx;
+  const int8_t* ocy;
+  const int8_t* msk;
+  const T* ds0;
+  const T* ds1;
+  T* dso;
+  const double* dt;
+  int integ, stage;
+  int ny, nx, P;  // interior sizes, padded row length
+  int xper, yper;
+  int tiles_x, ntiles;
+  int64_t nhalo;
+};
+
+template <class T, int M>
+__device__ __forceinline__ T xface_global(const TransportArgs<T>& a, int j,
+                                          int i) {
+  const int64_t k = (int64_t)j * a.P + i;
+  const int o = a.ocx[k];
+  if (o <= 0) return T(0);
+  const T u = a.Ux[k];
+  const T* q = a.q + k;
+  return flux_by_order<T, M>(o, u, q[-3], q[-2], q[-1], q[0], q[1], q[2]) * u;
+}
+
+template <class T, int M>
+__device__ __forceinline__ T yface_global(const TransportArgs<T>& a, int j,
+                                          int i) {
+  const int64_t k = (int64_t)j * a.P + i;
+  const int o = a.ocy[k];
+  if (o <= 0) return T(0);
+  const T u = a.Uy[k];
+  const T* q = a.q + k;
+  const int64_t P = a.P;
+  return flux_by_order<T, M>(o, u, q[-3 * P], q[-2 * P], q[-P], q[0], q[P],
+                             q[2 * P]) *
+         u;
+}
+
+template <class T>
+__device__ __forceinline__ void stage_update(const TransportArgs<T>& a,
+                                             int64_t k, T ds, T qin) {
+  if (a.dso) a.dso[k] = ds;
+  if (a.integ < 0) return;
+  double c[3];
+  const int nc = rk_coefs(a.integ, a.stage, *a.dt, c);
+  T x0 = (nc == 1) ? ds : a.ds0[k];
+  T x1 = (nc == 2) ? ds : (nc > 2 ? a.ds1[k] : T(0));
+  T x2 = (nc == 3) ? ds : T(0);
+  a.qo[k] = qin + rk_combine<T>(nc, c, x0, x1, x2);
+}
+
+template <class T, int M, int TY, int TX>
+__global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
+  constexpr int LW = TX + 6;
+  __shared__ T sq[(TY + 6) * LW];
+  __shared__ T sfx[TY * (TX + 1)];
+  __shared__ T sfy[(TY + 1) * TX];
+  const int t = threadIdx.x;
+  const int P = a.P;
+
+  if ((int)blockIdx.x < a.ntiles) {
+    const int by = blockIdx.x / a.tiles_x, bx = blockIdx.x - by * a.tiles_x;
+    const int j0 = 3 + by * TY, i0 = 3 + bx * TX;
+    const int NYP = a.ny + 6, NXP = a.nx + 6;
+    for (int k = t; k < (TY + 6) * LW; k += 256) {
+      int r = k / LW, c = k - r * LW;
+      int gj = j0 - 3 + r, gi = i0 - 3 + c;
+      sq[k] = (gj < NYP && gi < NXP) ? a.q[(int64_t)gj * P + gi] : T(0);
+    }
+    __syncthreads();
+    // x faces: rows j0..j0+TY-1, faces i0..i0+TX
+    for (int f = t; f < TY * (TX + 1); f += 256) {
+      int r = f / (TX + 1), c = f - r * (TX + 1);
+      int gj = j0 + r, gi = i0 + c;
+      T v = T(0);
+      if (gj < 3 + a.ny && gi <= 3 + a.nx) {
+        int64_t k = (int64_t)gj * P + gi;
+        int o = a.ocx[k];
+        if (o > 0) {
+          T u = a.Ux[k];
+          const T* s = sq + (r + 3) * LW + c;  // s[0] = q[gi-3]
+          v = flux_by_order<T, M>(o, u, s[0], s[1], s[2], s[3], s[4], s[5]) * u;
+        }
+      }
+      sfx[f] = v;
+    }
+    // y faces: faces j0..j0+TY, columns i0..i0+TX-1
+    for (int f = t; f < (TY + 1) * TX; f += 256) {
+      int r = f / TX, c = f - r * TX;
+      int gj = j0 + r, gi = i0 + c;
+      T v = T(0);
+      if (gj <= 3 + a.ny && gi < 3 + a.nx) {
+        int64_t k = (int64_t)gj * P + gi;
+        int o = a.ocy[k];
+        if (o > 0) {
+          T u = a.Uy[k];
+          const T* s = sq + r * LW + (c + 3);  // s[0] = q[gj-3]
+          v = flux_by_order<T, M>(o, u, s[0], s[LW], s[2 * LW], s[3 * LW],
+                                  s[4 * LW], s[5 * LW]) *
+              u;
+        }
+      }
+      sfy[f] = v;
+    }
+    __syncthreads();
+    for (int e = t; e < TY * TX; e += 256) {
+      int r = e / TX, c = e - r * TX;
+      int gj = j0 + r, gi = i0 + c;
+      if (gj >= 3 + a.ny || gi >= 3 + a.nx) continue;
+      int64_t k = (int64_t)gj * P + gi;
+      T dfx = sfx[r * (TX + 1) + c + 1] - sfx[r * (TX + 1) + c];
+      T dfy = sfy[(r + 1) * TX + c] - sfy[r * TX + c];
+      T ds = ((-dfx) - dfy) * T(a.msk[k]);
+      stage_update<T>(a, k, ds, sq[(r + 3) * LW + c + 3]);
+    }
+    return;
+  }
+
+  // halo ring
+  int64_t h = (int64_t)(blockIdx.x - a.ntiles) * 256 + t;
+  if (h >= a.nhalo) return;
+  const int NXP = a.nx + 6;
+  int j, i;
+  if (h < 6 * (int64_t)NXP) {
+    int row = (int)(h / NXP);
+    i = (int)(h - (int64_t)row * NXP);
+    j = row < 3 ? row : a.ny + row;  // rows 0..2, ny+3..ny+5
+  } else {
+    int64_t u = h - 6 * (int64_t)NXP;
+    j = 3 + (int)(u / 6);
+    int c = (int)(u % 6);
+    i = c < 3 ? c : a.nx + c;
+  }
+  int js = j, is = i;
+  if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
+  if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
+  T ds = T(0);
+  if (js >= 3 && js < 3 + a.ny && is >= 3 && is < 3 + a.nx) {
+    int64_t ks = (int64_t)js * P + is;
+    T dfx = xface_global<T, M>(a, js, is + 1) - xface_global<T, M>(a, js, is);
+    T dfy = yface_global<T, M>(a, js + 1, is) - yface_global<T, M>(a, js, is);
+    ds = ((-dfx) - dfy) * T(a.msk[ks]);
+  }
+  int64_t k = (int64_t)j * P + i;
+  stage_update<T>(a, k, ds, a.q[k]);
+}
+
+template <class T, int M>
+void launch_transport(const TransportArgs<T>& a0, hipStream_t s) {
+  constexpr int TY = 16, TX = 64;
+  TransportArgs<T> a = a0;
+  a.tiles_x = (a.nx + TX - 1) / TX;
+  int tiles_y = (a.ny + TY - 1) / TY;
+  a.ntiles = a.tiles_x * tiles_y;
+  a.nhalo = 6 * (int64_t)(a.nx + 6) + 6 * (int64_t)a.ny;
+  int nb = a.ntiles + (int)((a.nhalo + 255) / 256);
+  k_transport<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
+}
+
+template <class T>
+__global__ void k_rk_addto(T* __restrict__ y, const T* __restrict__ x0,
+                           const T* __restrict__ x1, const T* __restrict__ x2,
+                           int64_t n, int integ, int stage,
+                           const double* __restrict__ dt) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double c[3];
+  int nc = rk_coefs(integ, stage, *dt, c);
+  T a0 = x0[k];
+  T a1 = nc > 1 ? x1[k] : T(0);
+  T a2 = nc > 2 ? x2[k] : T(0);
+  y[k] = y[k] + rk_combine<T>(nc, c, a0, a1, a2);
+}
+
+// ---------------------------------------------------------------------------
+// centres -> vertices, operators.py:126-133
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_c2v(const T* __restrict__ c, T* __restrict__ v,
+                      const int8_t* __restrict__ mv, int NY, int NX,
+                      int addto) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)NY * NX) return;
+  int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+  T val;
+  if (j > 0 && i > 0) {
+    T s = T(0.25) * (((c[k - NX - 1] + c[k - 1]) + c[k - NX]) + c[k]);
+    val = addto ? v[k] + s : s;
+  } else {
+    val = v[k];
+  }
+  v[k] = mv[k] ? val : T(0);
+}
+
+// ---------------------------------------------------------------------------
+// perpgrad + fill (+ CFL max), operators.py:144-149, equations.py:79-84.
+// Every entry of Ux, Uy is written: the rows/columns numpy's slices do not
+// reach keep their old value (times the contravariant factor); with fill,
+// periodic halo entries take the value of their wrapped source.
+// ---------------------------------------------------------------------------
+template <class T>
+struct PerpArgs {
+  const T* psi;
+  T* Ux;
+  T* Uy;
+  const int8_t* mx;
+  const int8_t* my;
+  int ny, nx;  // interior
+  int xper, yper, fill;
+  double sx, sy;
+  double* maxabs;    // nullable, [2]
+  double* dt;        // nullable: dt = min(cfl/(mx+my+1e-99), dtmax)
+  double cfl, dtmax;
+  double* partials;  // 2 * nblocks
+  unsigned* counter;
+};
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  return (a > b || a != a) ? a : b;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
+  __shared__ double sh[256];
+  __shared__ int flag;
+  const int NY = a.ny + 6, NX = a.nx + 6;
+  int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double mxv = 0.0, myv = 0.0;
+  if (k < (int64_t)NY * NX) {
+    int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+    int js = j, is = i;
+    if (a.fill) {
+      if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
+      if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
+    }
+    int64_t ks = (int64_t)js * NX + is;
+    T ux = (js < NY - 1)
+               ? (-(a.psi[ks + NX] - a.psi[ks])) * T(a.mx[ks])
+               : a.Ux[ks];
+    T uy = (is < NX - 1) ? (a.psi[ks + 1] - a.psi[ks]) * T(a.my[ks])
+                         : a.Uy[ks];
+    ux = ux * T(a.sx);
+    uy = uy * T(a.sy);
+    a.Ux[k] = ux;
+    a.Uy[k] = uy;
+    mxv = fabs((double)ux);
+    myv = fabs((double)uy);
+  }
+  if (!a.maxabs && !a.dt) return;
+  // block max over the padded arrays, then the last block finalises
+  {
+    const int t = threadIdx.x;
+    sh[t] = mxv;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+      __syncthreads();
+    }
+    double bx = sh[0];
+    __syncthreads();
+    sh[t] = myv;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+      __syncthreads();
+    }
+    double byv = sh[0];
+    __syncthreads();
+    const int nb = gridDim.x;
+    if (t == 0) {
+      store_partial(a.partials + blockIdx.x, bx);
+      store_partial(a.partials + nb + blockIdx.x, byv);
+    }
+    if (!last_block_arrive<256>(a.counter, &flag)) return;
+    double ax = 0.0, ay = 0.0;
+    for (int b = t; b < nb; b += 256) {
+      ax = nanmax(ax, load_partial(a.partials + b));
+      ay = nanmax(ay, load_partial(a.partials + nb + b));
+    }
+    sh[t] = ax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+      __syncthreads();
+    }
+    ax = sh[0];
+    __syncthreads();
+    sh[t] = ay;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+      __syncthreads();
+    }
+    ay = sh[0];
+    if (t == 0) {
+      if (a.maxabs) {
+        a.maxabs[0] = ax;
+        a.maxabs[1] = ay;
+      }
+      if (a.dt) {
+        double v = a.cfl / ((ax + ay) + 1e-99);
+        a.dt[0] = (a.dtmax < v) ? a.dtmax : v;  // Python min(v, dtmax)
+      }
+    }
+  }
+}
+
+// fill_halo_array (meshes.py:135-143) + y extension: halo entries take
+// their wrapped source; entries of closed directions are left alone.
+template <class T>
+__global__ void k_fill(T* __restrict__ a, int ny, int nx, int xper, int yper) {
+  const int NX = nx + 6, NY = ny + 6;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)NY * NX) return;
+  int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+  int js = j, is = i;
+  if (xper) is = (i < 3) ? i + nx : (i >= nx + 3 ? i - nx : i);
+  if (yper) js = (j < 3) ? j + ny : (j >= ny + 3 ? j - ny : j);
+  if (js == j && is == i) return;
+  a[k] = a[(int64_t)js * NX + is];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cfl(const T* __restrict__ Ux,
+                                             const T* __restrict__ Uy,
+                                             int64_t n, double cfl,
+                                             double dtmax, double* dt,
+                                             double* partials,
+                                             unsigned* counter) {
+  __shared__ double sh[256];
+  __shared__ int flag;
+  const int t = threadIdx.x;
+  double mx = 0.0, my = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + t; k < n;
+       k += (int64_t)gridDim.x * 256) {
+    mx = nanmax(mx, fabs((double)Ux[k]));
+    my = nanmax(my, fabs((double)Uy[k]));
+  }
+  sh[t] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  mx = sh[0];
+  __syncthreads();
+  sh[t] = my;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  my = sh[0];
+  __syncthreads();
+  const int nb = gridDim.x;
+  if (t == 0) {
+    store_partial(partials + blockIdx.x, mx);
+    store_partial(partials + nb + blockIdx.x, my);
+  }
+  if (!last_block_arrive<256>(counter, &flag)) return;
+  if (t == 0) {
+    double ax = 0.0, ay = 0.0;
+    for (int b = 0; b < nb; b++) {
+      ax = nanmax(ax, load_partial(partials + b));
+      ay = nanmax(ay, load_partial(partials + nb + b));
+    }
+    double v = cfl / ((ax + ay) + 1e-99);
+    dt[0] = (dtmax < v) ? dtmax : v;
+  }
+}
+
+__global__ void k_set_scalar(double* d, double v) { *d = v; }
+
+// Scratch for the grid reductions of this file (per device, set-up once).
+struct ReduceScratch {
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  int cap = 0;
+};
+static ReduceScratch g_red[64];
+
+static ReduceScratch& red_scratch(int nblocks) {
+  int dev = 0;
+  F2D_HIP(hipGetDevice(&dev));
+  F2D_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+  ReduceScratch& r = g_red[dev];
+  if (r.cap < nblocks) {
+    F2D_REQUIRE(r.cap == 0 || r.partials == nullptr || true, "");
+    if (r.partials) F2D_HIP(hipFree(r.partials));
+    int cap = nblocks < 4096 ? 4096 : nblocks;
+    F2D_HIP(hipMalloc((void**)&r.partials, sizeof(double) * 2 * cap));
+    if (!r.counter) {
+      F2D_HIP(hipMalloc((void**)&r.counter, sizeof(unsigned) * 4));
+      F2D_HIP(hipMemset(r.counter, 0, sizeof(unsigned) * 4));
+    }
+    r.cap = cap;
+  }
+  return r;
+}
+
+}  // namespace f2d
+
+using namespace f2d;
+
+#define F2D_DTYPE_DISPATCH(dtype, ...)              \
+  do {                                              \
+    if ((dtype) == F2D_F64) {                       \
+      using T = double;                             \
+      __VA_ARGS__;                                  \
+    } else if ((dtype) == F2D_F32) {                \
+      using T = float;                              \
+      __VA_ARGS__;                                  \
+    } else {                                        \
+      throw ::f2d::Error{"unknown dtype"};          \
+    }                                               \
+  } while (0)
+
+static void check_grid(const f2d_grid* g) {
+  F2D_REQUIRE(g != nullptr, "grid descriptor is NULL");
+  F2D_REQUIRE(g->halowidth == 3, "halowidth must be 3");
+  F2D_REQUIRE(g->ny > 0 && g->nx > 0, "grid is empty");
+  F2D_REQUIRE(g->ny < (1 << 24) && g->nx < (1 << 24), "grid too large");
+}
+
+static void check_method(int m) {
+  F2D_REQUIRE(m >= F2D_WENO && m <= F2D_CWENO, "unknown flux method");
+}
+
+extern "C" {
+
+int f2d_compflux(int dtype, void* flx, const void* U, const void* q,
+                 const int8_t* o, int64_t n, int64_t s, int method,
+                 void* stream) {
+  return guarded([&] {
+    check_method(method);
+    F2D_REQUIRE(n > 0 && s != 0, "f2d_compflux: bad size/shift");
+    F2D_DTYPE_DISPATCH(dtype, k_compflux<T><<<grid1d(n, 256), 256, 0,
+                                              S(stream)>>>(
+                                  (T*)flx, (const T*)U, (const T*)q, o, n, s,
+                                  method));
+  });
+}
+
+int f2d_vortexforce(int dtype, void* du, const void* V, const void* q,
+                    const int8_t* o, int64_t n, int64_t s, int64_t s2,
+                    int sign, int method, void* stream) {
+  return guarded([&] {
+    check_method(method);
+    F2D_REQUIRE(n > 0 && s != 0, "f2d_vortexforce: bad size/shift");
+    F2D_DTYPE_DISPATCH(dtype, k_vortexforce<T><<<grid1d(n, 256), 256, 0,
+                                                 S(stream)>>>(
+                                  (T*)du, (const T*)V, (const T*)q, o, n, s,
+                                  s2, sign, method));
+  });
+}
+
+int f2d_innerproduct(int dtype, void* ke, const void* U, const void* q,
+                     const int8_t* o, int64_t n, int64_t s, int method,
+                     void* stream) {
+  return guarded([&] {
+    check_method(method);
+    F2D_REQUIRE(n > 0 && s != 0, "f2d_innerproduct: bad size/shift");
+    F2D_DTYPE_DISPATCH(dtype, k_innerproduct<T><<<grid1d(n, 256), 256, 0,
+                                                  S(stream)>>>(
+                                  (T*)ke, (const T*)U, (const T*)q, o, n, s,
+                                  method));
+  });
+}
+
+int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
+                        const void* q_in, void* q_out, const void* Ux,
+                        const void* Uy, const int8_t* ocx, const int8_t* ocy,
+                        const int8_t* msk, const void* ds_prev0,
+                        const void* ds_prev1, void* ds_out, int integrator,
+                        int stage, const double* dt, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    check_method(method);
+    if (integrator >= 0) {
+      F2D_REQUIRE(integrator <= F2D_ENRK3, "unknown integrator");
+      int nst = integrator == F2D_EF ? 1 : 3;
+      F2D_REQUIRE(stage >= 0 && stage < nst, "bad stage");
+      F2D_REQUIRE(q_out && q_out != q_in, "q_out must be a distinct array");
+      F2D_REQUIRE(dt, "dt is NULL");
+      double c[3];
+      int nc = rk_coefs(integrator, stage, 0.0, c);
+      F2D_REQUIRE(nc < 2 || ds_prev0, "ds_prev0 is required at this stage");
+      F2D_REQUIRE(nc < 3 || ds_prev1, "ds_prev1 is required at this stage");
+    } else {
+      F2D_REQUIRE(ds_out, "rhs-only call needs ds_out");
+    }
+    F2D_DTYPE_DISPATCH(dtype, {
+      TransportArgs<T> a{};
+      a.q = (const T*)q_in;
+      a.qo = (T*)q_out;
+      a.Ux = (const T*)Ux;
+      a.Uy = (const T*)Uy;
+      a.ocx = ocx;
+      a.ocy = ocy;
+      a.msk = msk;
+      a.ds0 = (const T*)ds_prev0;
+      a.ds1 = (const T*)ds_prev1;
+      a.dso = (T*)ds_out;
+      a.dt = dt;
+      a.integ = integrator;
+      a.stage = stage;
+      a.ny = (int)g->ny;
+      a.nx = (int)g->nx;
+      a.P = (int)g->nx + 6;
+      a.xper = g->xperiodic;
+      a.yper = g->yperiodic;
+      switch (method) {
+        case F2D_WENO: launch_transport<T, F2D_WENO>(a, S(stream)); break;
+        case F2D_UPWIND: launch_transport<T, F2D_UPWIND>(a, S(stream)); break;
+        case F2D_CENTERED: launch_transport<T, F2D_CENTERED>(a, S(stream)); break;
+        default: launch_transport<T, F2D_CWENO>(a, S(stream)); break;
+      }
+    });
+  });
+}
+
+int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,
+                 const void* x2, int64_t n, int integrator, int stage,
+                 const double* dt, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(integrator >= 0 && integrator <= F2D_ENRK3, "bad integrator");
+    F2D_DTYPE_DISPATCH(dtype, k_rk_addto<T><<<grid1d(n, 256), 256, 0,
+                                              S(stream)>>>(
+                                  (T*)y, (const T*)x0, (const T*)x1,
+                                  (const T*)x2, n, integrator, stage, dt));
+  });
+}
+
+int f2d_c2v(int dtype, const f2d_grid* g, const void* c, void* v,
+            const int8_t* mskv, int addto, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    int NY = (int)g->ny + 6, NX = (int)g->nx + 6;
+    F2D_DTYPE_DISPATCH(dtype, k_c2v<T><<<grid1d((int64_t)NY * NX, 256), 256,
+                                         0, S(stream)>>>(
+                                  (const T*)c, (T*)v, mskv, NY, NX, addto));
+  });
+}
+
+int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                    void* Uy, const int8_t* mskx, const int8_t* msky,
+                    double sx, double sy, int fill, double* maxabs,
+                    double* dt, double cfl, double dtmax, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    int NY = (int)g->ny + 6, NX = (int)g->nx + 6;
+    int nb = grid1d((int64_t)NY * NX, 256);
+    F2D_DTYPE_DISPATCH(dtype, {
+      PerpArgs<T> a{};
+      a.psi = (const T*)psi;
+      a.Ux = (T*)Ux;
+      a.Uy = (T*)Uy;
+      a.mx = mskx;
+      a.my = msky;
+      a.ny = (int)g->ny;
+      a.nx = (int)g->nx;
+      a.xper = g->xperiodic;
+      a.yper = g->yperiodic;
+      a.fill = fill;
+      a.sx = sx;
+      a.sy = sy;
+      a.maxabs = maxabs;
+      a.dt = dt;
+      a.cfl = cfl;
+      a.dtmax = dtmax;
+      if (maxabs || dt) {
+        ReduceScratch& r = red_scratch(nb);
+        a.partials = r.partials;
+        a.counter = r.counter;
+      }
+      k_perpgrad<T><<<nb, 256, 0, S(stream)>>>(a);
+    });
+  });
+}
+
+int f2d_perpgrad(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                 void* Uy, const int8_t* mskx, const int8_t* msky,
+                 int contravariant, double* maxabs, void* stream) {
+  double sx = 1.0, sy = 1.0;
+  if (contravariant) {
+    sx = 1.0 / (g->dy * g->dy);
+    sy = 1.0 / (g->dx * g->dx);
+  }
+  return f2d_perpgrad_ex(dtype, g, psi, Ux, Uy, mskx, msky, sx, sy, 1, maxabs,
+                         nullptr, 0.0, 0.0, stream);
+}
+
+int f2d_fill_halo(int dtype, const f2d_grid* g, void* a, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    if (!g->xperiodic && !g->yperiodic) return;
+    int64_t n = (g->ny + 6) * (g->nx + 6);
+    F2D_DTYPE_DISPATCH(dtype, k_fill<T><<<grid1d(n, 256), 256, 0,
+                                          S(stream)>>>(
+                                  (T*)a, (int)g->ny, (int)g->nx,
+                                  g->xperiodic, g->yperiodic));
+  });
+}
+
+int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
+               double cfl, double dtmax, double* dt, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(n > 0, "f2d_cfl_dt: empty");
+    int nb = grid1d(n, 256);
+    if (nb > 1024) nb = 1024;
+    ReduceScratch& r = red_scratch(nb);
+    F2D_DTYPE_DISPATCH(dtype, k_cfl<T><<<nb, 256, 0, S(stream)>>>(
+                                  (const T*)Ux, (const T*)Uy, n, cfl, dtmax,
+                                  dt, r.partials, r.counter));
+  });
+}
+
+int f2d_set_scalar(double* dst, double value, void* stream) {
+  return guarded([&] { k_set_scalar<<<1, 1, 0, S(stream)>>>(dst, value); });
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""Poisson2D: masked Poisson/Helmholtz solve on the vertex grid
+(elliptic.py:71-203), by the library's multigrid-preconditioned CG.
+
+The reference factorises the masked 5-point operator with SuperLU once per
+``mesh.finalize()`` and solves with it at every stage.  Here the operator is
+matrix-free; ``create`` builds the multigrid hierarchy from the mask and every
+solve is warm-started from the previous solution held in ``x``.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+DEFAULT_RTOL = {"fp64": 1e-13, "fp32": 1e-9}
+DEFAULT_MAXIT = {"fp64": 100, "fp32": 4}
+
+
+class PoissonNotConverged(RuntimeError):
+    pass
+
+
+class Poisson2D:
+    def __init__(self, mesh, location, maindiag=0):
+        assert location in ["c", "v"]
+        if location == "c":
+            raise NotImplementedError("the Neumann (cell) operator is not on the GPU path")
+        self.mesh = mesh
+        self.location = location
+        self.maindiag = float(maindiag)
+        p = mesh.param
+        self.precision = getattr(p, "precision", "fp64")
+        self.msk = get_msk_from_mesh(mesh, location)
+        self.G, self.N = self._gindex()
+        opts = _lib.PoissonOpts()
+        opts.mode = _lib.POISSON_F64 if self.precision == "fp64" else _lib.POISSON_F32_REFINE
+        opts.nu = int(getattr(p, "poisson_nu", 2) or 2)
+        rt = getattr(p, "poisson_rtol", None)
+        mi = getattr(p, "poisson_maxit", None)
+        self.rtol = DEFAULT_RTOL[self.precision] if rt is None else float(rt)
+        self.maxit = DEFAULT_MAXIT[self.precision] if mi is None else int(mi)
+        self.dtype = torch.float64 if self.precision == "fp64" else torch.float32
+        unk = np.ascontiguousarray(self.msk, dtype=np.int8)
+        h = ctypes.c_void_p()
+        _lib.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), mesh.grid_ref(),
+                  self.maindiag, ctypes.byref(opts), ctypes.byref(h))
+        self._h = h
+        self.last_iters = 0
+        self.last_relres = 0.0
+
+    def _gindex(self):
+        """get_Gindex (elliptic.py:198-203), computed on the device."""
+        d = torch.from_numpy(np.ascontiguousarray(self.msk, dtype=np.int8)).to(self.mesh.device)
+        G = torch.zeros(d.shape, dtype=torch.int32, device=d.device)
+        n = ctypes.c_int64()
+        _lib.call("f2d_gindex", _lib.ptr(d), d.numel(), _lib.ptr(G), ctypes.byref(n),
+                  _lib.stream_ptr())
+        return G.cpu().numpy(), int(n.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().f2d_poisson_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def handle(self):
+        return self._h
+
+    def solve(self, b, x, check=True):
+        """Poisson2D.solve(b, x) (elliptic.py:80-87): x is written at the
+        unknowns, then its halo is filled.  numpy arrays are solved through a
+        device copy; device tensors in place.  Raises PoissonNotConverged if
+        the relative residual exceeds the tolerance after maxit iterations."""
+        host = not isinstance(x, torch.Tensor)
+        if host:
+            db = torch.from_numpy(np.ascontiguousarray(b, dtype=np.float64)).to(
+                self.mesh.device).to(self.dtype)
+            dx = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(
+                self.mesh.device).to(self.dtype)
+        else:
+            db, dx = b, x
+            assert db.dtype == self.dtype and dx.dtype == self.dtype
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        _lib.call("f2d_poisson_solve", self._h, _lib.ptr(db), _lib.ptr(dx), self.rtol,
+                  self.maxit, ctypes.byref(it), ctypes.byref(rr), _lib.stream_ptr())
+        self.last_iters, self.last_relres = it.value, rr.value
+        if check and not (rr.value <= self.rtol):
+            raise PoissonNotConverged(
+                f"Poisson solve reached relres {rr.value:.3e} > {self.rtol:.1e} "
+                f"after {it.value} iterations")
+        if host:
+            x[...] = dx.double().cpu().numpy()
+        return None
+
+    def solve_c2v(self, omega, x, stats=True):
+        """Fused eulerpsi/qg diagnostic: b = centerstovertices(omega) formed on
+        the fly, then solve (equations.py:80-82, operators.py:186-191)."""
+        if stats:
+            it, rr = ctypes.c_int(), ctypes.c_double()
+            _lib.call("f2d_poisson_solve_c2v", self._h, _lib.ptr(omega), _lib.ptr(x),
+                      self.rtol, self.maxit, ctypes.byref(it), ctypes.byref(rr),
+                      _lib.stream_ptr())
+            self.last_iters, self.last_relres = it.value, rr.value
+            if not (rr.value <= self.rtol):
+                raise PoissonNotConverged(
+                    f"Poisson solve reached relres {rr.value:.3e} > {self.rtol:.1e} "
+                    f"after {it.value} iterations")
+        else:
+            _lib.call("f2d_poisson_solve_c2v", self._h, _lib.ptr(omega), _lib.ptr(x),
+                      self.rtol, self.maxit, None, None, _lib.stream_ptr())
+
+
+def get_msk_from_mesh(mesh, location):
+    """elliptic.py:102-111, plus the y-periodic extension."""
+    msk = mesh.msk if location == "c" else mesh.mskv
+    n = mesh.param.halowidth
+    m = msk * 1
+    if mesh.param.xperiodic:
+        m[:, :n] = 0
+        m[:, -n:] = 0
+    if getattr(mesh.param, "yperiodic", False):
+        m[:n, :] = 0
+        m[-n:, :] = 0
+    return m

@@ -1,0 +1,134 @@
+"""Device models: rhs/diag of each model on the GPU (equations.py:73-238).
+
+Each model class offers
+  * ``rhs(s, ds)`` / ``diag(s)`` -- the reference's closures, operator by
+    operator, on device states (used when user code calls them);
+  * ``stage(...)`` -- one fused RK stage used by the integrator's hot loop:
+    the tendency, the halo fill and the `addto` update in one kernel, then the
+    diagnostic fields (Poisson solve with the right-hand side formed inside
+    the solver, perp-gradient + fill + next CFL step in one kernel).
+"""
+from . import operators as op
+from .elliptic import PoissonNotConverged  # noqa: F401  (re-export)
+
+
+class _Base:
+    prognostic = ()
+    vector_prognostic = ()
+
+    def __init__(self, param, mesh):
+        self.param = param
+        self.mesh = mesh
+
+    def materialize(self, s):
+        """Fill the diagnostic fields the fused path does not store, before a
+        device -> host copy."""
+
+    def cfl_velocity(self, s):
+        return s.U
+
+
+class EulerPsi(_Base):
+    """Euler in vorticity-streamfunction form (equations.py:269-282)."""
+    prognostic = ("omega",)
+
+    def rhs(self, s, ds):
+        m, p = self.mesh, self.param
+        op.compflux(s.flx.x, s.U.x, s.omega, m.d_oc.x, m.xshift, p.compflux)
+        op.compflux(s.flx.y, s.U.y, s.omega, m.d_oc.y, m.yshift, p.compflux)
+        op.transport_rhs(m, p.compflux, s.omega, s.U, ds.omega)
+
+    def diag(self, s):
+        m = self.mesh
+        op.centerstovertices(m, s.omega, s.vomega)
+        m.poisson_vertices.solve(s.vomega, s.psi)
+        op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True)
+
+    def diag_fused(self, s, dt_next=None, stats=True):
+        m, p = self.mesh, self.param
+        m.poisson_vertices.solve_c2v(s.omega, s.psi, stats=stats)
+        op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
+                    cfl=p.cfl, dtmax=p.dtmax)
+
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+        p = self.param
+        op.transport_stage(self.mesh, p.compflux, s.omega, alt.omega, s.U, ds_prev,
+                           ds_out, p.integrator, k, dt)
+        s, alt = s._replace(omega=alt.omega), alt._replace(omega=s.omega)
+        self.diag_fused(s, dt_next, stats)
+        return s, alt
+
+    def materialize(self, s):
+        op.centerstovertices(self.mesh, s.omega, s.vomega)
+
+
+class Advection(_Base):
+    """Tracer advection by a fixed velocity (equations.py:356-366)."""
+    prognostic = ("q",)
+
+    def rhs(self, s, ds):
+        m, p = self.mesh, self.param
+        op.compflux(s.flx.x, s.U.x, s.q, m.d_oc.x, m.xshift, p.compflux)
+        op.compflux(s.flx.y, s.U.y, s.q, m.d_oc.y, m.yshift, p.compflux)
+        op.transport_rhs(m, p.compflux, s.q, s.U, ds.q)
+
+    def diag(self, s):
+        pass
+
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+        p = self.param
+        op.transport_stage(self.mesh, p.compflux, s.q, alt.q, s.U, ds_prev, ds_out,
+                           p.integrator, k, dt)
+        s, alt = s._replace(q=alt.q), alt._replace(q=s.q)
+        if dt_next is not None:
+            op.cfl_dt(s.U, p.cfl, p.dtmax, dt_next)
+        return s, alt
+
+
+class QG(_Base):
+    """Quasi-geostrophic PV transport (equations.py:285-299)."""
+    prognostic = ("pv",)
+
+    def __init__(self, param, mesh):
+        super().__init__(param, mesh)
+        if param.beta != 0:
+            raise NotImplementedError("qg with beta != 0 needs mesh.f on the device")
+
+    def rhs(self, s, ds):
+        m, p = self.mesh, self.param
+        op.compflux(s.flx.x, s.U.x, s.pv, m.d_oc.x, m.xshift, p.compflux)
+        op.compflux(s.flx.y, s.U.y, s.pv, m.d_oc.y, m.yshift, p.compflux)
+        op.transport_rhs(m, p.compflux, s.pv, s.U, ds.pv)
+
+    def diag(self, s):
+        # qg_inversion (operators.py:186-191) with hb = 0, beta = 0
+        m = self.mesh
+        op.centerstovertices(m, s.pv, s.work)
+        m.qg_helmholtz.solve(s.work, s.psi)
+        op.perpgrad(m, s.psi, s.U, contravariant=True)
+
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+        m, p = self.mesh, self.param
+        op.transport_stage(m, p.compflux, s.pv, alt.pv, s.U, ds_prev, ds_out,
+                           p.integrator, k, dt)
+        s, alt = s._replace(pv=alt.pv), alt._replace(pv=s.pv)
+        m.qg_helmholtz.solve_c2v(s.pv, s.psi, stats=stats)
+        op.perpgrad(m, s.psi, s.U, contravariant=True, fill=False, dt_dev=dt_next,
+                    cfl=p.cfl, dtmax=p.dtmax)
+        return s, alt
+
+    def materialize(self, s):
+        op.centerstovertices(self.mesh, s.pv, s.work)
+
+
+MODELS = {"eulerpsi": EulerPsi, "advection": Advection, "qg": QG}
+
+
+def get_device_model(param, mesh):
+    if param.model in MODELS:
+        return MODELS[param.model](param, mesh)
+    try:
+        from . import rsw
+        return rsw.get_device_model(param, mesh)
+    except (ImportError, KeyError):
+        raise NotImplementedError(f"model {param.model!r} has no GPU path") from None

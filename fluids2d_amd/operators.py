@@ -1,0 +1,93 @@
+"""Device operators: thin wrappers over the library's C ABI, named after the
+reference's operators (operators.py, weno.py).  Every argument is a device
+tensor in the padded layout; every wrapper raises on a library error.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_ptr
+
+
+def _dt(t):
+    return _lib.dtype_code(t.dtype)
+
+
+def compflux(flx, U, q, o, s, method):
+    """weno.compflux (weno.py:412-420)"""
+    call("f2d_compflux", _dt(q), ptr(flx), ptr(U), ptr(q), ptr(o), q.numel(), s,
+         _lib.METHODS[method], stream_ptr())
+
+
+def vortexforce(du, V, q, o, s, s2, sign, method):
+    """weno.vortexforce (weno.py:423-431)"""
+    call("f2d_vortexforce", _dt(q), ptr(du), ptr(V), ptr(q), ptr(o), q.numel(), s, s2,
+         sign, _lib.METHODS[method], stream_ptr())
+
+
+def innerproduct(ke, U, q, o, s, method):
+    """weno.innerproduct (weno.py:434-436)"""
+    call("f2d_innerproduct", _dt(q), ptr(ke), ptr(U), ptr(q), ptr(o), q.numel(), s,
+         _lib.METHODS[method], stream_ptr())
+
+
+def transport_rhs(mesh, method, q, U, dq):
+    """divflux + fill (operators.py:21-29, equations.py:273-274):
+    dq = fill(-div(flux(q, U)) * msk), without storing the face fluxes."""
+    call("f2d_transport_stage", _dt(q), mesh.grid_ref(), _lib.METHODS[method], ptr(q),
+         None, ptr(U.x), ptr(U.y), ptr(mesh.d_oc.x), ptr(mesh.d_oc.y), ptr(mesh.d_msk),
+         None, None, ptr(dq), -1, 0, None, stream_ptr())
+
+
+def transport_stage(mesh, method, q_in, q_out, U, ds_prev, ds_out, integrator, stage,
+                    dt_dev):
+    """Fused rhs + fill + RK addto of one stage (see f2d_transport_stage)."""
+    d0 = ds_prev[0] if len(ds_prev) > 0 else None
+    d1 = ds_prev[1] if len(ds_prev) > 1 else None
+    call("f2d_transport_stage", _dt(q_in), mesh.grid_ref(), _lib.METHODS[method],
+         ptr(q_in), ptr(q_out), ptr(U.x), ptr(U.y), ptr(mesh.d_oc.x), ptr(mesh.d_oc.y),
+         ptr(mesh.d_msk), ptr(d0), ptr(d1), ptr(ds_out), _lib.INTEGRATORS[integrator],
+         stage, ptr(dt_dev), stream_ptr())
+
+
+def rk_addto(y, xs, integrator, stage, dt_dev):
+    """addto (integrators.py:154-196) with the stage coefficients from dt."""
+    x0 = xs[0]
+    x1 = xs[1] if len(xs) > 1 else None
+    x2 = xs[2] if len(xs) > 2 else None
+    call("f2d_rk_addto", _dt(y), ptr(y), ptr(x0), ptr(x1), ptr(x2), y.numel(),
+         _lib.INTEGRATORS[integrator], stage, ptr(dt_dev), stream_ptr())
+
+
+def centerstovertices(mesh, omega, vomega, addto=False):
+    """operators.py:126-133"""
+    call("f2d_c2v", _dt(omega), mesh.grid_ref(), ptr(omega), ptr(vomega),
+         ptr(mesh.d_mskv), int(addto), stream_ptr())
+
+
+def perpgrad(mesh, psi, U, contravariant=False, fill=False, dt_dev=None, cfl=0.0,
+             dtmax=0.0):
+    """operators.py:144-149 (+ fill, + the CFL step of model.py:84-87)."""
+    if contravariant:
+        sx, sy = 1 / mesh.dy ** 2, 1 / mesh.dx ** 2
+    else:
+        sx, sy = 1.0, 1.0
+    call("f2d_perpgrad_ex", _dt(psi), mesh.grid_ref(), ptr(psi), ptr(U.x), ptr(U.y),
+         ptr(mesh.d_mskx), ptr(mesh.d_msky), sx, sy, int(fill), None, ptr(dt_dev),
+         float(cfl), float(dtmax), stream_ptr())
+
+
+def fill(mesh, a):
+    """meshes.py:106-111 on a device tensor"""
+    call("f2d_fill_halo", _dt(a), mesh.grid_ref(), ptr(a), stream_ptr())
+
+
+def cfl_dt(U, cfl, dtmax, dt_dev):
+    """Model.set_dt (model.py:84-87) from device velocities."""
+    call("f2d_cfl_dt", _dt(U.x), ptr(U.x), ptr(U.y), U.x.numel(), float(cfl),
+         float(dtmax), ptr(dt_dev), stream_ptr())
+
+
+def set_scalar(dst, value):
+    call("f2d_set_scalar", ptr(dst), float(value), stream_ptr())

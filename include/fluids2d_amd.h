@@ -1,0 +1,198 @@
+/*
+ * fluids2d_amd.h -- C ABI of the MI355X (gfx950) time-stepping hot path of
+ * Fluids2d.  Implemented in the .hip files of fluids2d_amd/csrc, built into
+ * fluids2d_amd/libfluids2d_amd.so.
+ *
+ * Conventions (all entry points):
+ *   - Return 0 on success, a negative code on error; f2d_last_error() gives
+ *     the message of the last error on the calling thread.
+ *   - Pointers named in the signatures are device pointers unless the name
+ *     ends in `_host`.  Arrays are row-major, C-contiguous, of the padded
+ *     shape (ny + 2*halowidth, nx + 2*halowidth) unless stated otherwise.
+ *   - `dtype` is F2D_F64 or F2D_F32 and applies to every floating array of
+ *     the call.  Masks and stencil orders are int8, index maps int32.
+ *   - `stream` is a hipStream_t (NULL = the null stream).  No entry point
+ *     synchronises except where stated; none allocates on the stepping path
+ *     (the Poisson handle allocates its workspace in f2d_poisson_create).
+ *   - The caller owns every array; outputs are written in place.
+ */
+#ifndef FLUIDS2D_AMD_H
+#define FLUIDS2D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define F2D_F64 0
+#define F2D_F32 1
+
+/* flux families, weno.py:338-343 */
+#define F2D_WENO 0
+#define F2D_UPWIND 1
+#define F2D_CENTERED 2
+#define F2D_CWENO 3
+
+/* integrators, integrators.py:127-131 */
+#define F2D_RK3 0
+#define F2D_EF 1
+#define F2D_ENRK3 2
+
+/* Poisson precision modes */
+#define F2D_POISSON_F64 0        /* FP64 MG-PCG                                 */
+#define F2D_POISSON_F32_REFINE 1 /* FP32 MG-PCG inside FP64 iterative refinement */
+#define F2D_POISSON_F32 2        /* FP32 MG-PCG only                             */
+
+int f2d_abi_version(void);
+const char* f2d_last_error(void);
+/* Synchronise the current device (host-side wait). */
+int f2d_device_synchronize(void);
+
+/* ------------------------------------------------------------------------
+ * Grid description shared by the stencil entry points.
+ * ------------------------------------------------------------------------ */
+typedef struct f2d_grid {
+  int64_t ny, nx;   /* interior cells                                   */
+  int32_t halowidth;/* must be 3 (param.py:23)                          */
+  int32_t xperiodic, yperiodic;
+  double dx, dy;    /* Lx/nx, Ly/ny (meshes.py:24-25)                    */
+} f2d_grid;
+
+/* ------------------------------------------------------------------------
+ * Mesh: masks, stencil orders, slip coefficient, unknown numbering.
+ * ------------------------------------------------------------------------ */
+/* Mesh.set_masks, meshes.py:77-86: mskx, msky, mskv from msk (padded). */
+int f2d_set_masks(const int8_t* msk, int64_t ny_pad, int64_t nx_pad,
+                  int8_t* mskx, int8_t* msky, int8_t* mskv, void* stream);
+/* set_order, meshes.py:146-186, on the flattened array of n entries. */
+int f2d_set_order(const int8_t* m, int64_t n, int64_t shift, int maxorder,
+                  int8_t* o, void* stream);
+/* get_slipcoef, noslip.py:4-36.  mode 0 free-slip, 1 no-slip everywhere,
+ * 2 free-slip with no-slip on the sides flagged in `sides` (bit 0 left,
+ * 1 right, 2 bottom, 3 top).  coef is float64 of the padded shape. */
+int f2d_slipcoef(const int8_t* msk, int64_t ny_pad, int64_t nx_pad, int mode,
+                 int sides, int halowidth, double* coef, void* stream);
+/* get_Gindex, elliptic.py:198-203: G = running index of wet points,
+ * -1 elsewhere.  *count_host receives the number of wet points
+ * (synchronises `stream`). */
+int f2d_gindex(const int8_t* m, int64_t n, int32_t* G, int64_t* count_host,
+               void* stream);
+
+/* ------------------------------------------------------------------------
+ * Flat-index WENO loops (weno.py:412-436), one thread per padded point.
+ * ------------------------------------------------------------------------ */
+/* weno.compflux(flx, U, q, o, s, method), weno.py:412-420 */
+int f2d_compflux(int dtype, void* flx, const void* U, const void* q,
+                 const int8_t* o, int64_t n, int64_t s, int method,
+                 void* stream);
+/* weno.vortexforce(du, V, q, o, s, s2, sign, method), weno.py:423-431 */
+int f2d_vortexforce(int dtype, void* du, const void* V, const void* q,
+                    const int8_t* o, int64_t n, int64_t s, int64_t s2,
+                    int sign, int method, void* stream);
+/* weno.innerproduct(ke, U, q, o, s, method), weno.py:434-436 (ke += ...) */
+int f2d_innerproduct(int dtype, void* ke, const void* U, const void* q,
+                     const int8_t* o, int64_t n, int64_t s, int method,
+                     void* stream);
+
+/* ------------------------------------------------------------------------
+ * Fused transport stage: divflux + fill + RK addto in one pass.
+ *   operators.py:21-29 (divflux), :104-107 (div), meshes.py:106-111 (fill),
+ *   integrators.py:92-124, 154-196 (rk3/ef/enrk3 stage `stage`, addto).
+ * Computes ds = fill(-div(flux(q, U)) * msk) and
+ *   q_out = q_in + ((c0*ds0 + c1*ds1) + c2*ds)
+ * with the stage coefficients of `integrator` evaluated from *dt (device
+ * scalar, float64) exactly as the reference's Python expressions.
+ * ds_prev[k] are the tendencies of the earlier stages; ds_out (may be NULL)
+ * receives this stage's tendency.  q_out must not alias q_in.
+ * With integrator < 0 no update is made (q_out unused): rhs only.
+ * ------------------------------------------------------------------------ */
+int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
+                        const void* q_in, void* q_out, const void* Ux,
+                        const void* Uy, const int8_t* ocx, const int8_t* ocy,
+                        const int8_t* msk, const void* ds_prev0,
+                        const void* ds_prev1, void* ds_out, int integrator,
+                        int stage, const double* dt, void* stream);
+
+/* y += ((c0*x0 + c1*x1) + c2*x2) over n entries (addto, integrators.py:174);
+ * coefficients of `integrator` stage `stage` from *dt as above. */
+int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,
+                 const void* x2, int64_t n, int integrator, int stage,
+                 const double* dt, void* stream);
+
+/* centerstovertices, operators.py:126-133 (addto=0/1) */
+int f2d_c2v(int dtype, const f2d_grid* g, const void* c, void* v,
+            const int8_t* mskv, int addto, void* stream);
+/* perpgrad (+ fill), operators.py:144-149, equations.py:79-84.
+ * If maxabs (device float64[2]) is not NULL it receives
+ * (max|Ux|, max|Uy|) over the padded arrays. */
+int f2d_perpgrad(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                 void* Uy, const int8_t* mskx, const int8_t* msky,
+                 int contravariant, double* maxabs, void* stream);
+/* perpgrad with the scale factors given by the caller (the reference's
+ * 1/mesh.dy**2 and 1/mesh.dx**2 as Python evaluates them, or 1.0 for the
+ * covariant form); fill != 0 applies the halo fill; dt (device float64, may
+ * be NULL) receives the CFL step of model.py:84-87 computed from the new
+ * velocities with `cfl` and `dtmax`. */
+int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                    void* Uy, const int8_t* mskx, const int8_t* msky,
+                    double sx, double sy, int fill, double* maxabs,
+                    double* dt, double cfl, double dtmax, void* stream);
+/* fill_halo_array, meshes.py:135-143 (+ the y-periodic extension) */
+int f2d_fill_halo(int dtype, const f2d_grid* g, void* a, void* stream);
+/* Model.set_dt, model.py:71-87 (non-rsw branch):
+ * dt = min(cfl / (max|Ux| + max|Uy| + 1e-99), dtmax) over the padded
+ * arrays; result in *dt (device float64). */
+int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
+               double cfl, double dtmax, double* dt, void* stream);
+/* dt = value (device scalar) -- fixed-dt runs and the rsw branch */
+int f2d_set_scalar(double* dst, double value, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Masked Poisson / Helmholtz on the vertex grid (elliptic.py:71-203).
+ * Solves A x = b at the wet points of `unknowns_host` where A is the
+ * masked 5-point operator of get_Laplacian_sparse ("v", Dirichlet):
+ *   (A x)_P = (dy/dx)(x_E + x_W) + (dx/dy)(x_N + x_S)
+ *             - (2(dy/dx + dx/dy) + maindiag) x_P,
+ * missing (dry) neighbours contributing zero, with x-periodic (and, as an
+ * extension, y-periodic) wrap.  The doubly periodic operator without mass
+ * term is singular: b is projected to zero mean and x has zero mean.
+ * Geometric multigrid (red-black Gauss-Seidel V-cycle) preconditioned CG.
+ * ------------------------------------------------------------------------ */
+typedef struct f2d_poisson f2d_poisson;
+
+typedef struct f2d_poisson_opts {
+  int32_t mode;          /* F2D_POISSON_*                                 */
+  int32_t nu;            /* RB-GS sweeps before and after (1 or 2)        */
+  int32_t coarse_sweeps; /* symmetric sweeps on the coarsest level (0=auto) */
+  int32_t refine_inner;  /* F32_REFINE: FP32 PCG iterations per refinement */
+  int32_t tail_points;   /* max points of the single-workgroup tail levels */
+  int32_t reserved[3];
+} f2d_poisson_opts;
+
+/* unknowns_host: int8, padded shape, 1 at unknowns (elliptic.py:102-111:
+ * mskv with the periodic halos cleared).  opts may be NULL (defaults). */
+int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
+                       double maindiag, const f2d_poisson_opts* opts,
+                       f2d_poisson** out);
+int f2d_poisson_destroy(f2d_poisson* h);
+/* Poisson2D.solve(b, x), elliptic.py:80-87: x (in/out, initial guess) is
+ * written at the unknowns only, then its halo is filled.  The solve stops
+ * when ||r||_2 <= rtol ||b||_2 or after maxit iterations.  iters/relres
+ * (host, may be NULL) receive the iteration count and the final relative
+ * residual; giving either synchronises `stream`. */
+int f2d_poisson_solve(f2d_poisson* h, const void* b, void* x, double rtol,
+                      int maxit, int* iters, double* relres, void* stream);
+/* Fused eulerpsi diagnostic solve (equations.py:80-82): b = c2v(omega)*mskv
+ * computed on the fly from the cell field omega, then solve as above. */
+int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
+                          double rtol, int maxit, int* iters, double* relres,
+                          void* stream);
+/* Device-side solve statistics of the last solve: stats_dev[0] iterations,
+ * [1] relative residual, [2] converged flag (float64 device array). */
+const double* f2d_poisson_stats(f2d_poisson* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUIDS2D_AMD_H */

@@ -1,0 +1,346 @@
+/*
+ * oracle.c -- CPU restatement of the Fluids2d hot-path arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library in fluids2d_amd/csrc.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * Every routine restates one reference routine (file:line given in its
+ * comment), keeping the reference's evaluation order so that a build with
+ * -O2 -ffp-contract=off reproduces the reference's IEEE double results:
+ *   - `x**2` is x*x (numba lowers a literal integer power to a multiply),
+ *   - Python's left-to-right `a + b + c` is ((a + b) + c),
+ *   - numba wraps negative array indices (q[i-3*s] with i-3*s < 0).
+ *
+ * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fPIC -shared).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_WENO 0
+#define ORC_UPWIND 1
+#define ORC_CENTERED 2
+#define ORC_CWENO 3
+
+/* ---------------------------------------------------------------------- */
+/* pointwise reconstructions: weno.py:21-43, 75-100, 103-134, 167-197,      */
+/* 255-300                                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* weno.py:21-23 */
+static double ce2(double U, double qm, double qp) {
+  (void)U;
+  return (qm + qp) * 0.5;
+}
+/* weno.py:26-28 : (-qmm+7*(qm+qp)-qpp)/12 */
+static double ce4(double U, double qmm, double qm, double qp, double qpp) {
+  (void)U;
+  return ((-qmm + 7.0 * (qm + qp)) - qpp) / 12.0;
+}
+/* weno.py:31-33 : (qmmm-8*(qmm+qpp)+37*(qm+qp)+qppp)/60 */
+static double ce6(double U, double qmmm, double qmm, double qm, double qp,
+                  double qpp, double qppp) {
+  (void)U;
+  return (((qmmm - 8.0 * (qmm + qpp)) + 37.0 * (qm + qp)) + qppp) / 60.0;
+}
+/* weno.py:36-38 : (5*q0+2*qp-qm)/6 */
+static double up3(double qm, double q0, double qp) {
+  return ((5.0 * q0 + 2.0 * qp) - qm) / 6.0;
+}
+/* weno.py:41-43 : (2*qmmm - 13*qmm + 47*qm + 27*qp - 3*qpp)/60 */
+static double up5(double qmmm, double qmm, double qm, double qp, double qpp) {
+  return ((((2.0 * qmmm - 13.0 * qmm) + 47.0 * qm) + 27.0 * qp) - 3.0 * qpp) /
+         60.0;
+}
+
+/* weno.py:75-100 (Borges et al. 2008, 3-point WENO-Z) */
+double orc_weno3z(double qm, double q0, double qp) {
+  const double eps = 1e-14;
+  double qi1 = (-1. / 2.) * qm + (3. / 2.) * q0;
+  double qi2 = (1. / 2.) * (q0 + qp);
+  double d1 = q0 - qm, d2 = qp - q0;
+  double beta1 = d1 * d1;
+  double beta2 = d2 * d2;
+  double tau = fabs(beta2 - beta1);
+  const double g1 = 1. / 3., g2 = 2. / 3.;
+  double w1 = g1 * (1. + tau / (beta1 + eps));
+  double w2 = g2 * (1. + tau / (beta2 + eps));
+  return (w1 * qi1 + w2 * qi2) / (w1 + w2);
+}
+
+/* weno.py:103-134 (centred WENO-Z3, "cweno" order 4) */
+static double cweno3z(double U, double qmm, double qm, double qp, double qpp) {
+  const double eps = 1e-14;
+  double qi1 = (-1. / 2.) * qmm + (3. / 2.) * qm;
+  double qi2 = (1. / 2.) * (qm + qp);
+  double qi3 = (-1. / 2.) * qpp + (3. / 2.) * qp;
+  double beta1, beta2, a, b;
+  if (U > 0) {
+    a = qm - qmm; b = qp - qm;
+  } else {
+    a = qp - qpp; b = qp - qm;
+  }
+  beta1 = a * a;
+  beta2 = b * b;
+  double tau = fabs(beta2 - beta1);
+  const double g1 = 1. / 3., g2 = 2. / 3.;
+  double w1 = g1 * (1. + tau / (beta1 + eps));
+  double w2 = g2 * (1. + tau / (beta2 + eps));
+  return ((w1 * (qi1 + qi3)) * 0.5 + w2 * qi2) / (w1 + w2);
+}
+
+/* weno.py:167-197 (Borges et al. 2008, 5-point WENO-Z) */
+double orc_weno5z(double qmm, double qm, double q0, double qp, double qpp) {
+  const double eps = 1e-16;
+  double qi1 = ((1. / 3.) * qmm - (7. / 6.) * qm) + (11. / 6.) * q0;
+  double qi2 = ((-1. / 6.) * qm + (5. / 6.) * q0) + (1. / 3.) * qp;
+  double qi3 = ((1. / 3.) * q0 + (5. / 6.) * qp) - (1. / 6.) * qpp;
+  const double k1 = 13. / 12., k2 = 0.25;
+  double a1 = (qmm - 2.0 * qm) + q0, b1 = (qmm - 4.0 * qm) + 3.0 * q0;
+  double a2 = (qm - 2.0 * q0) + qp, b2 = qm - qp;
+  double a3 = (q0 - 2.0 * qp) + qpp, b3 = (3.0 * q0 - 4.0 * qp) + qpp;
+  double beta1 = k1 * (a1 * a1) + k2 * (b1 * b1);
+  double beta2 = k1 * (a2 * a2) + k2 * (b2 * b2);
+  double beta3 = k1 * (a3 * a3) + k2 * (b3 * b3);
+  double tau5 = fabs(beta1 - beta3);
+  const double g1 = 0.1, g2 = 0.6, g3 = 0.3;
+  double w1 = g1 * (1.0 + tau5 / (beta1 + eps));
+  double w2 = g2 * (1.0 + tau5 / (beta2 + eps));
+  double w3 = g3 * (1.0 + tau5 / (beta3 + eps));
+  return ((w1 * qi1 + w2 * qi2) + w3 * qi3) / ((w1 + w2) + w3);
+}
+
+/* weno.py:255-300 (cweno5z_v0, used by "cweno" at order 6) */
+static double cweno5z_v0(double U, double qmmm, double qmm, double qm,
+                         double qp, double qpp, double qppp) {
+  (void)U; /* both branches of weno.py:276-283 compute the same betas */
+  const double eps = 1e-16;
+  double qi1 = ((1. / 3.) * qmmm - (7. / 6.) * qmm) + (11. / 6.) * qm;
+  double qi2 = ((-1. / 6.) * qmm + (5. / 6.) * qm) + (1. / 3.) * qp;
+  double qi3 = ((1. / 3.) * qm + (5. / 6.) * qp) - (1. / 6.) * qpp;
+  double qi4 = ((1. / 3.) * qp + (5. / 6.) * qm) - (1. / 6.) * qmm;
+  double qi5 = ((-1. / 6.) * qpp + (5. / 6.) * qp) + (1. / 3.) * qm;
+  double qi6 = ((1. / 3.) * qppp - (7. / 6.) * qpp) + (11. / 6.) * qp;
+  const double k1 = 13. / 12., k2 = 0.25;
+  double a1 = (qmmm - 2.0 * qmm) + qm, b1 = (qmmm - 4.0 * qmm) + 3.0 * qm;
+  double a2 = (qmm - 2.0 * qm) + qp, b2 = qmm - qp;
+  double a3 = (qm - 2.0 * qp) + qpp, b3 = (3.0 * qm - 4.0 * qp) + qpp;
+  double beta1 = k1 * (a1 * a1) + k2 * (b1 * b1);
+  double beta2 = k1 * (a2 * a2) + k2 * (b2 * b2);
+  double beta3 = k1 * (a3 * a3) + k2 * (b3 * b3);
+  double tau5 = fabs(beta1 - beta3);
+  const double g1 = 0.1, g2 = 0.6, g3 = 0.3;
+  double w1 = g1 * (1.0 + tau5 / (beta1 + eps));
+  double w2 = g2 * (1.0 + tau5 / (beta2 + eps));
+  double w3 = g3 * (1.0 + tau5 / (beta3 + eps));
+  return ((w1 * (qi1 + qi6) + w2 * (qi2 + qi5)) + w3 * (qi3 + qi4)) /
+         (2.0 * ((w1 + w2) + w3));
+}
+
+/* ---------------------------------------------------------------------- */
+/* upwind selection: weno.py:303-343                                       */
+/* ---------------------------------------------------------------------- */
+double orc_f1(int method, double U, double qm, double qp) {
+  if (method == ORC_CENTERED || method == ORC_CWENO) return ce2(U, qm, qp);
+  return U > 0 ? qm : qp; /* flx1, weno.py:303-305 */
+}
+
+double orc_f3(int method, double U, double qmm, double qm, double qp,
+              double qpp) {
+  switch (method) {
+    case ORC_WENO: /* flx3, weno.py:308-310 */
+      return U > 0 ? orc_weno3z(qmm, qm, qp) : orc_weno3z(qpp, qp, qm);
+    case ORC_UPWIND: /* flxup3, weno.py:318-320 */
+      return U > 0 ? up3(qmm, qm, qp) : up3(qpp, qp, qm);
+    case ORC_CENTERED:
+      return ce4(U, qmm, qm, qp, qpp);
+    default: /* cflx3, weno.py:313-315 */
+      return cweno3z(U, qmm, qm, qp, qpp);
+  }
+}
+
+double orc_f5(int method, double U, double qmmm, double qmm, double qm,
+              double qp, double qpp, double qppp) {
+  switch (method) {
+    case ORC_WENO: /* flx5, weno.py:323-325 */
+      return U > 0 ? orc_weno5z(qmmm, qmm, qm, qp, qpp)
+                   : orc_weno5z(qppp, qpp, qp, qm, qmm);
+    case ORC_UPWIND: /* flxup5, weno.py:333-335 */
+      return U > 0 ? up5(qmmm, qmm, qm, qp, qpp) : up5(qppp, qpp, qp, qm, qmm);
+    case ORC_CENTERED:
+      return ce6(U, qmmm, qmm, qm, qp, qpp, qppp);
+    default: /* cflx5, weno.py:328-330 */
+      return cweno5z_v0(U, qmmm, qmm, qm, qp, qpp, qppp);
+  }
+}
+
+/* numba's negative-index wraparound (no check on the high side) */
+static inline int64_t W(int64_t i, int64_t n) { return i < 0 ? i + n : i; }
+
+/* ---------------------------------------------------------------------- */
+/* flat-index loops: weno.py:346-405                                       */
+/* ---------------------------------------------------------------------- */
+
+/* compflux_on_interval, weno.py:349-363, over [i0, i1) */
+void orc_compflux(double* flx, const double* U, const double* q,
+                  const int8_t* o, int64_t n, int64_t s, int method,
+                  int64_t i0, int64_t i1) {
+  for (int64_t i = i0; i < i1; i++) {
+    if (o[i] > 4) {
+      flx[i] = orc_f5(method, U[i], q[W(i - 3 * s, n)], q[W(i - 2 * s, n)],
+                      q[W(i - s, n)], q[i], q[i + s], q[i + 2 * s]) *
+               U[i];
+    } else if (o[i] > 2) {
+      flx[i] = orc_f3(method, U[i], q[W(i - 2 * s, n)], q[W(i - s, n)], q[i],
+                      q[i + s]) *
+               U[i];
+    } else if (o[i] > 0) {
+      flx[i] = orc_f1(method, U[i], q[W(i - s, n)], q[i]) * U[i];
+    } else {
+      flx[i] = 0;
+    }
+  }
+}
+
+/* vortexforce_on_interval, weno.py:370-384 */
+void orc_vortexforce(double* du, const double* V, const double* q,
+                     const int8_t* o, int64_t n, int64_t s, int64_t s2,
+                     int sign, int method, int64_t i0, int64_t i1) {
+  const double sg = (double)sign;
+  for (int64_t i = i0; i < i1; i++) {
+    if (o[i] > 0) {
+      double Vm = 0.25 * (((V[i] + V[i + s]) + V[W(i - s2, n)]) +
+                          V[W(i + s - s2, n)]);
+      double f;
+      if (o[i] > 4)
+        f = orc_f5(method, Vm, q[W(i - 2 * s, n)], q[W(i - s, n)], q[i],
+                   q[i + s], q[i + 2 * s], q[i + 3 * s]);
+      else if (o[i] > 2)
+        f = orc_f3(method, Vm, q[W(i - s, n)], q[i], q[i + s], q[i + 2 * s]);
+      else
+        f = orc_f1(method, Vm, q[i], q[i + s]);
+      du[i] = (sg * f) * Vm;
+    } else {
+      du[i] = 0;
+    }
+  }
+}
+
+/* innerproduct_on_interval, weno.py:391-403 (accumulates into ke) */
+void orc_innerproduct(double* ke, const double* U, const double* q,
+                      const int8_t* o, int64_t n, int64_t s, int method,
+                      int64_t i0, int64_t i1) {
+  for (int64_t i = i0; i < i1; i++) {
+    if (o[i] > 0) {
+      double Um = 0.5 * (U[i] + U[i + s]);
+      double f;
+      if (o[i] > 4)
+        f = orc_f5(method, Um, q[W(i - 2 * s, n)], q[W(i - s, n)], q[i],
+                   q[i + s], q[i + 2 * s], q[i + 3 * s]);
+      else if (o[i] > 2)
+        f = orc_f3(method, Um, q[W(i - s, n)], q[i], q[i + s], q[i + 2 * s]);
+      else
+        f = orc_f1(method, Um, q[i], q[i + s]);
+      ke[i] += f * Um;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* mask and stencil-order logic: meshes.py:77-86, 146-186                  */
+/* ---------------------------------------------------------------------- */
+
+/* set_order, meshes.py:146-186.  m holds small integers (0/1). */
+void orc_set_order(const int8_t* m, int64_t n, int64_t shift, int maxorder,
+                   int8_t* o) {
+  for (int64_t i = 0; i < n; i++) {
+    int s2, s4, s6;
+    if (shift > 0) {
+      s2 = (i - shift >= 0) ? m[i - shift] + m[i] : 0;
+      s4 = ((i - shift * 2 >= 0) && (i + shift < n))
+               ? m[i - shift * 2] + m[i + shift] + s2
+               : 0;
+      s6 = ((i - shift * 3 >= 0) && (i + 2 * shift < n))
+               ? m[i - shift * 3] + m[i + shift * 2] + s4
+               : 0;
+    } else {
+      s2 = (i - shift < n) ? m[i - shift] + m[i] : 0;
+      s4 = ((i - shift * 2 < n) && (i + shift >= 0))
+               ? m[i - shift * 2] + m[i + shift] + s2
+               : 0;
+      s6 = ((i - shift * 3 < n) && (i + 2 * shift >= 0))
+               ? m[i - shift * 3] + m[i + shift * 2] + s4
+               : 0;
+    }
+    int v = (s6 == 6) ? 6 : ((s4 == 4) ? 4 : ((s2 == 2) ? 2 : 0));
+    o[i] = (int8_t)(v < maxorder ? v : maxorder);
+  }
+}
+
+/* set_masks, meshes.py:77-86 */
+void orc_set_masks(const int8_t* msk, int64_t ny, int64_t nx, int8_t* mskx,
+                   int8_t* msky, int8_t* mskv) {
+  for (int64_t j = 0; j < ny; j++)
+    for (int64_t i = 0; i < nx; i++) {
+      int64_t k = j * nx + i;
+      mskx[k] = (i > 0) ? (int8_t)(msk[k] * msk[k - 1]) : 0;
+      msky[k] = (j > 0) ? (int8_t)(msk[k] * msk[k - nx]) : 0;
+      mskv[k] = (i > 0 && j > 0) ? (int8_t)(msk[k - nx] * msk[k - nx - 1] *
+                                            msk[k] * msk[k - 1])
+                                 : 0;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Laplacian assembly: elliptic.py:114-203                                 */
+/* ---------------------------------------------------------------------- */
+
+/* get_Gindex, elliptic.py:198-203: row-major numbering of wet points.
+ * Returns the number of unknowns. */
+int64_t orc_gindex(const int8_t* msk, int64_t n, int32_t* G) {
+  int64_t c = 0;
+  for (int64_t i = 0; i < n; i++) G[i] = msk[i] ? (int32_t)(c++) : -1;
+  return c;
+}
+
+/* get_Laplacian_sparse, elliptic.py:114-195, emitting COO triplets in the
+ * reference's order (west, east, south, north, diagonal).  `G` is the index
+ * map of the (halo-stripped) mask.  The y-periodic branch is the symmetric
+ * extension of the x one (the reference fixes yperiodic = False at
+ * elliptic.py:142).  Returns the number of triplets. */
+int64_t orc_laplacian_coo(const int32_t* G, int64_t ny, int64_t nx, double dx,
+                          double dy, double maindiag, int neumann,
+                          int xperiodic, int yperiodic, int halowidth,
+                          int32_t* row, int32_t* col, double* data) {
+  const double dx2 = dy / dx, dy2 = dx / dy;
+  const int64_t n1 = xperiodic ? halowidth : 0;
+  const int64_t n2 = yperiodic ? halowidth : 0;
+  int64_t c = 0;
+#define GG(j, i) G[(j) * nx + (i)]
+  for (int64_t j = 0; j < ny; j++)
+    for (int64_t i = 0; i < nx; i++) {
+      int32_t I = GG(j, i);
+      if (I < 0) continue;
+      double sum_extra = 0;
+      int32_t west = (i > n1) ? GG(j, i - 1)
+                              : (xperiodic ? GG(j, nx - n1 - 1) : -1);
+      int32_t east = (i < nx - 1 - n1) ? GG(j, i + 1)
+                                       : (xperiodic ? GG(j, n1) : -1);
+      int32_t south = (j > n2) ? GG(j - 1, i)
+                               : (yperiodic ? GG(ny - n2 - 1, i) : -1);
+      int32_t north = (j < ny - 1 - n2) ? GG(j + 1, i)
+                                        : (yperiodic ? GG(n2, i) : -1);
+      if (west > -1) { row[c] = I; col[c] = west; data[c++] = dx2; sum_extra += dx2; }
+      if (east > -1) { row[c] = I; col[c] = east; data[c++] = dx2; sum_extra += dx2; }
+      if (south > -1) { row[c] = I; col[c] = south; data[c++] = dy2; sum_extra += dy2; }
+      if (north > -1) { row[c] = I; col[c] = north; data[c++] = dy2; sum_extra += dy2; }
+      row[c] = I; col[c] = I;
+      data[c++] = neumann ? (-sum_extra - maindiag) : (-2 * (dx2 + dy2) - maindiag);
+    }
+#undef GG
+  return c;
+}
+
+int orc_version(void) { return 1; }

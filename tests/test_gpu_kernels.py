@@ -1,0 +1,309 @@
+"""Kernel-level parity of the HIP library against the oracle (GPU).
+
+Integer work (masks, orders, index numbering) must be bit-exact; the FP64
+stencils are evaluated in the reference's operation order and must be
+bit-exact too; the Poisson solve is iterative and is compared with SuperLU
+(the reference's direct solve) and with closed forms within a tolerance
+stated per test.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import P, random_mask, assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def N(t):
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def L():
+    from fluids2d_amd import _lib
+    return _lib
+
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+# masks and orders
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed,xper,yper", [(0, False, False), (1, True, False),
+                                            (2, True, True), (3, False, True)])
+def test_masks_orders_bitexact(dev, L, seed, xper, yper):
+    ny, nx = 23, 31
+    m = random_mask(ny, nx, seed, xper, yper, nisl=4)
+    NYP, NXP = m.shape
+    dm = T(m, dev)
+    import torch
+    mx, my, mv = (torch.zeros_like(dm) for _ in range(3))
+    L.call("f2d_set_masks", L.ptr(dm), NYP, NXP, L.ptr(mx), L.ptr(my), L.ptr(mv),
+           L.stream_ptr())
+    ex, ey, ev = O.set_masks(m)
+    assert np.array_equal(N(mx), ex) and np.array_equal(N(my), ey)
+    assert np.array_equal(N(mv), ev)
+    for src in (m, ex, ey, ev):
+        for shift in (1, -1, NXP, -NXP):
+            for maxorder in (6, 5, 4, 2):
+                o = torch.zeros_like(dm)
+                dsrc = T(src, dev)
+                L.call("f2d_set_order", L.ptr(dsrc), src.size, shift, maxorder,
+                       L.ptr(o), L.stream_ptr())
+                assert np.array_equal(N(o), O.set_order(src, shift, maxorder)), (shift, maxorder)
+    for mode, noslip in ((0, None), (1, True), (2, ["left", "top"]), (2, ["bottom", "right"])):
+        sides = 0
+        if isinstance(noslip, list):
+            sides = sum(b for s, b in (("left", 1), ("right", 2), ("bottom", 4), ("top", 8))
+                        if s in noslip)
+        c = torch.zeros(m.shape, dtype=torch.float64, device=dev)
+        L.call("f2d_slipcoef", L.ptr(dm), NYP, NXP, mode, sides, 3, L.ptr(c),
+               L.stream_ptr())
+        assert np.array_equal(N(c), O.slipcoef(m, noslip, 3)), noslip
+    G = torch.zeros(m.shape, dtype=torch.int32, device=dev)
+    cnt = ctypes.c_int64()
+    dev_ev = T(ev, dev)
+    L.call("f2d_gindex", L.ptr(dev_ev), ev.size, L.ptr(G), ctypes.byref(cnt),
+           L.stream_ptr())
+    eG, eN = O.gindex(ev)
+    assert cnt.value == eN and np.array_equal(N(G), eG)
+
+
+# ---------------------------------------------------------------------------
+# flat WENO loops
+# ---------------------------------------------------------------------------
+def _fields(shape, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.standard_normal(shape)
+    U = rng.standard_normal(shape)
+    U[rng.random(shape) < 0.1] = 0.0  # U == 0 takes the mirrored branch
+    V = rng.standard_normal(shape)
+    return q, U, V
+
+
+@pytest.mark.parametrize("method", ["weno", "upwind", "centered", "cweno"])
+def test_flat_loops_bitexact(dev, L, method):
+    ny, nx = 20, 27
+    m = random_mask(ny, nx, 5, True, False)
+    mesh = O.Mesh(P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True), msk=m)
+    q, U, V = _fields(m.shape, 7)
+    n = q.size
+    import torch
+    dq, dU, dV = T(q, dev), T(U, dev), T(V, dev)
+    for o, s in ((mesh.oc.x, 1), (mesh.oc.y, nx + 6)):
+        flx = np.zeros_like(q)
+        O.compflux(flx, U, q, o, s, method)
+        d = torch.zeros(m.shape, dtype=torch.float64, device=dev)
+        do = T(o, dev)
+        L.call("f2d_compflux", L.F64, L.ptr(d), L.ptr(dU), L.ptr(dq), L.ptr(do), n, s,
+               L.METHODS[method], L.stream_ptr())
+        assert_same(N(d), flx, f'compflux s={s}')
+    for o, s, s2, sg in ((mesh.ov.y, nx + 6, 1, 1), (mesh.ov.x, 1, nx + 6, -1)):
+        du = np.zeros_like(q)
+        O.vortexforce(du, V, q, o, s, s2, sg, method)
+        d = torch.zeros(m.shape, dtype=torch.float64, device=dev)
+        do = T(o, dev)
+        L.call("f2d_vortexforce", L.F64, L.ptr(d), L.ptr(dV), L.ptr(dq), L.ptr(do), n, s,
+               s2, sg, L.METHODS[method], L.stream_ptr())
+        assert_same(N(d), du, f'vortexforce s={s}')
+    for o, s in ((mesh.ok.x, 1), (mesh.ok.y, nx + 6)):
+        ke0 = np.random.default_rng(3).standard_normal(q.shape)
+        ke = ke0.copy()
+        O.innerproduct(ke, U, q, o, s, method)
+        d = T(ke0, dev)
+        do = T(o, dev)
+        L.call("f2d_innerproduct", L.F64, L.ptr(d), L.ptr(dU), L.ptr(dq), L.ptr(do), n, s,
+               L.METHODS[method], L.stream_ptr())
+        assert_same(N(d), ke, f'innerproduct s={s}')
+
+
+# ---------------------------------------------------------------------------
+# fused transport stage == divflux + fill + addto
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("xper,yper,method,integ", [
+    (False, False, "weno", "rk3"), (True, False, "weno", "rk3"),
+    (True, True, "weno", "enrk3"), (False, True, "upwind", "rk3"),
+    (True, False, "centered", "ef"), (False, False, "cweno", "rk3")])
+def test_transport_stage_bitexact(dev, L, xper, yper, method, integ):
+    import torch
+    ny, nx = 37, 70  # partial tiles in both directions
+    m = random_mask(ny, nx, 11, xper, yper, nisl=3)
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper,
+          compflux=method)
+    mesh = O.Mesh(p, msk=m)
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal(m.shape)
+    Ux, Uy = rng.standard_normal(m.shape), rng.standard_normal(m.shape)
+    F = O.Vector(Ux, Uy)
+    dt = 0.0123456789
+    g = L.make_grid(p)
+    dq, dUx, dUy = T(q, dev), T(Ux, dev), T(Uy, dev)
+    docx, docy, dm = T(mesh.oc.x, dev), T(mesh.oc.y, dev), T(m, dev)
+    ddt = torch.tensor([dt], dtype=torch.float64, device=dev)
+    nst = L.NSTAGES[integ]
+    # oracle: rhs; fill; addto with the reference coefficients
+    dss, qref = [], q.copy()
+    dprev = []
+    qcur = dq
+    for k in range(nst):
+        ds = np.zeros_like(q)
+        O.divflux(p, mesh, O.Vector(np.zeros_like(q), np.zeros_like(q)), qref, F, ds)
+        mesh.fill(ds)
+        dss.append(ds)
+        coefs = {"rk3": [(dt,), (-3 * dt / 4, dt / 4), (-dt / 12, -dt / 12, 2 * dt / 3)],
+                 "ef": [(dt,)],
+                 "enrk3": [(dt / 3,), (-dt / 3 - 5 * dt / 48, 15 * dt / 16),
+                           (5 * dt / 48 + dt / 10, -7 * dt / 16, 2 * dt / 5)]}[integ][k]
+        args = []
+        for c, x in zip(coefs, dss):
+            args += [c, x]
+        O.addto((qref,), *[(a if i % 2 == 0 else (a,)) for i, a in enumerate(args)])
+        # device
+        qo = torch.empty_like(dq)
+        dso = torch.zeros_like(dq)
+        L.call("f2d_transport_stage", L.F64, ctypes.byref(g), L.METHODS[method],
+               L.ptr(qcur), L.ptr(qo), L.ptr(dUx), L.ptr(dUy), L.ptr(docx), L.ptr(docy),
+               L.ptr(dm), L.ptr(dprev[0] if len(dprev) > 0 else None),
+               L.ptr(dprev[1] if len(dprev) > 1 else None), L.ptr(dso),
+               L.INTEGRATORS[integ], k, L.ptr(ddt), L.stream_ptr())
+        dprev.append(dso)
+        sync()
+        assert_same(N(dso), ds, f"tendency stage {k}")
+        assert_same(N(qo), qref, f"update stage {k}")
+        qcur = qo
+
+
+# ---------------------------------------------------------------------------
+# c2v, perpgrad + fill, CFL
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("xper,yper", [(False, False), (True, False), (True, True)])
+def test_point_operators_bitexact(dev, L, xper, yper):
+    import torch
+    ny, nx = 30, 44
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper, Lx=1.3)
+    m = random_mask(ny, nx, 2, xper, yper)
+    mesh = O.Mesh(p, msk=m)
+    rng = np.random.default_rng(9)
+    w = rng.standard_normal(m.shape)
+    g = L.make_grid(p)
+    for addto in (0, 1):
+        v0 = rng.standard_normal(m.shape)
+        v = v0.copy()
+        O.centerstovertices(mesh, w, v, addto=bool(addto))
+        dv, dw, dmv = T(v0, dev), T(w, dev), T(mesh.mskv, dev)
+        L.call("f2d_c2v", L.F64, ctypes.byref(g), L.ptr(dw), L.ptr(dv),
+               L.ptr(dmv), addto, L.stream_ptr())
+        assert_same(N(dv), v, 'c2v')
+    psi = rng.standard_normal(m.shape)
+    for contra in (False, True):
+        U = O.Vector(np.zeros(m.shape), np.zeros(m.shape))
+        O.perpgrad(mesh, psi, U, contravariant=contra)
+        mesh.fill(U)
+        dUx = torch.zeros(m.shape, dtype=torch.float64, device=dev)
+        dUy = torch.zeros_like(dUx)
+        sx, sy = (1 / mesh.dy ** 2, 1 / mesh.dx ** 2) if contra else (1.0, 1.0)
+        ddt = torch.zeros(1, dtype=torch.float64, device=dev)
+        dpsi, dmx, dmy = T(psi, dev), T(mesh.mskx, dev), T(mesh.msky, dev)
+        L.call("f2d_perpgrad_ex", L.F64, ctypes.byref(g), L.ptr(dpsi), L.ptr(dUx),
+               L.ptr(dUy), L.ptr(dmx), L.ptr(dmy), sx, sy, 1,
+               None, L.ptr(ddt), 0.8, 9e99, L.stream_ptr())
+        assert_same(N(dUx), U.x, 'Ux')
+        assert_same(N(dUy), U.y, 'Uy')
+        mref = np.max(np.abs(U.x)) + np.max(np.abs(U.y)) + 1e-99
+        assert N(ddt)[0] == min(0.8 / mref, 9e99)
+
+
+# ---------------------------------------------------------------------------
+# Poisson / Helmholtz
+# ---------------------------------------------------------------------------
+def _poisson_case(dev, L, p, m, maindiag, mode, nu=2, seed=0):
+    import torch
+    mesh = O.Mesh(p, msk=m)
+    unk = O.unknown_mask(p, mesh.mskv)
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal(m.shape) * mesh.mskv
+    x_ref = np.zeros(m.shape)
+    ref = O.Poisson2D(p, mesh, "v", maindiag=maindiag)
+    ref.solve(b, x_ref)
+    g = L.make_grid(p)
+    opts = L.PoissonOpts()
+    opts.mode, opts.nu = mode, nu
+    h = ctypes.c_void_p()
+    L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+           maindiag, ctypes.byref(opts), ctypes.byref(h))
+    dt = torch.float64 if mode == L.POISSON_F64 else torch.float32
+    db = T(b, dev).to(dt)
+    dx = torch.zeros(m.shape, dtype=dt, device=dev)
+    it, rr = ctypes.c_int(), ctypes.c_double()
+    rtol = 1e-13 if mode == L.POISSON_F64 else (1e-10 if mode == L.POISSON_F32_REFINE else 1e-5)
+    maxit = 200 if mode != L.POISSON_F32_REFINE else 6
+    L.call("f2d_poisson_solve", h, L.ptr(db), L.ptr(dx), rtol, maxit, ctypes.byref(it),
+           ctypes.byref(rr), L.stream_ptr())
+    x = N(dx).astype(np.float64)
+    L.call("f2d_poisson_destroy", h)
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    return err, it.value, rr.value, x, x_ref, unk
+
+
+@pytest.mark.parametrize("ny,nx,xper,yper,islands,maindiag", [
+    (64, 64, False, False, False, 0.0), (40, 40, False, False, False, 0.0),
+    (64, 128, True, False, True, 0.0), (128, 128, True, True, False, 0.0),
+    (96, 64, False, False, True, 0.3), (256, 256, False, False, False, 0.0),
+    (256, 256, True, True, False, 0.0), (50, 36, True, True, True, 0.0)])
+def test_poisson_f64_vs_superlu(dev, L, ny, nx, xper, yper, islands, maindiag):
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper,
+          Lx=nx / 64.0, Ly=ny / 64.0)
+    m = random_mask(ny, nx, 3, xper, yper, nisl=5) if islands else \
+        O.default_mask(ny, nx, 3, xper, yper)
+    err, it, rr, x, xr, unk = _poisson_case(dev, L, p, m, maindiag, L.POISSON_F64)
+    print(f"\n{ny}x{nx} xper={xper} yper={yper} isl={islands} md={maindiag}: "
+          f"iters={it} relres={rr:.2e} err={err:.2e}")
+    assert rr <= 1e-13
+    assert err < 1e-10
+    # entries outside the unknowns: untouched except for the halo fill
+    inner = np.zeros_like(unk, dtype=bool)
+    inner[3:-3, 3:-3] = True
+    assert np.all(x[inner & (unk == 0)] == 0)
+
+
+def test_poisson_periodic_vs_fft(dev, L):
+    ny = nx = 128
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True)
+    m = O.default_mask(ny, nx, 3, True, True)
+    err, it, rr, x, xr, unk = _poisson_case(dev, L, p, m, 0.0, L.POISSON_F64, seed=5)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(m.shape) * O.Mesh(p, msk=m).mskv
+    xf = O.fft_poisson_periodic(b[3:-3, 3:-3], 1 / nx, 1 / ny)
+    e2 = np.linalg.norm(x[3:-3, 3:-3] - xf) / np.linalg.norm(xf)
+    assert e2 < 1e-10, e2
+
+
+@pytest.mark.parametrize("mode,ny,nx,xper,yper,islands", [
+    ("refine", 256, 256, True, True, False), ("f32", 256, 256, True, True, False),
+    ("refine", 256, 256, False, False, False), ("refine", 128, 256, True, False, True),
+    ("f32", 128, 256, True, False, True)])
+def test_poisson_f32_modes(dev, L, mode, ny, nx, xper, yper, islands):
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper,
+          Lx=nx / 256.0, Ly=ny / 256.0)
+    m = random_mask(ny, nx, 3, xper, yper, nisl=5) if islands else \
+        O.default_mask(ny, nx, 3, xper, yper)
+    md = L.POISSON_F32_REFINE if mode == "refine" else L.POISSON_F32
+    err, it, rr, x, xr, unk = _poisson_case(dev, L, p, m, 0.0, md)
+    print(f"\nmode={mode} {ny}x{nx} inner iters={it} relres={rr:.2e} err={err:.2e}")
+    if mode == "refine":
+        # FP64 residual, FP32 inner solves: error limited by the FP32 output
+        assert rr <= 1e-10
+        assert err < 1e-6
+    else:
+        assert err < 1e-3
