@@ -22,7 +22,8 @@ EXPORTS = [
     "f2d_transport_stage", "f2d_rk_addto", "f2d_c2v", "f2d_perpgrad",
     "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar",
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
-    "f2d_poisson_solve_c2v", "f2d_poisson_stats",
+    "f2d_poisson_solve_c2v", "f2d_poisson_stats", "f2d_poisson_reset_stats",
+    "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
 ]
 
 
@@ -42,7 +43,9 @@ class PoissonOpts(ctypes.Structure):
                 ("coarse_sweeps", ctypes.c_int32),
                 ("refine_inner", ctypes.c_int32),
                 ("tail_points", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("tail_threads", ctypes.c_int32),
+                ("nu_coarse", ctypes.c_int32),
+                ("nu_tail", ctypes.c_int32)]
 
 
 _lib = None
@@ -88,6 +91,10 @@ def lib():
         "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
                                         ctypes.POINTER(d), vp]),
         "f2d_poisson_stats": (vp, [vp]),
+        "f2d_poisson_reset_stats": (i32, [vp, vp]),
+        "f2d_poisson_read_stats": (i32, [vp, ctypes.POINTER(d), vp]),
+        "f2d_poisson_bench_kernel": (i32, [vp, i32, i32, ctypes.POINTER(d),
+                                           ctypes.POINTER(d), vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)

@@ -17,16 +17,21 @@
 // (red, black)^nu and post-smoothing (black, red)^nu, so the V-cycle is a
 // symmetric positive-definite operator.
 //
-// Kernels per CG iteration (level 0 = the fine grid):
-//   k_down<CG>   r -= alpha q, x += alpha p, ||r||^2 + convergence test,
-//                then pre-smooth / residual / restrict  (one LDS tile pass)
+// Kernels of CG iteration k (level 0 = the fine grid, 32x32 tiles):
+//   k_down<CG>   alpha from the previous k_pupdate's partial sums;
+//                r -= alpha q, x += alpha p, partial ||r||^2, then
+//                pre-smooth / residual / restrict in the same LDS tile pass
 //   k_down       the same smoothing/restriction on levels 1 .. Lt-1
 //   k_tail       one workgroup runs the levels Lt .. L-1 entirely in LDS
 //   k_up         prolongate + post-smooth on levels Lt-1 .. 1
-//   k_up<FINE>   level 0 post-smoothing, (r, z), sum z -> beta
-//   k_pupdate    p = z + beta p, q = K p, (p, q) -> alpha
-// Scalars live in a device control block; a converged solve turns the rest
-// of the enqueued kernels into early returns.
+//   k_up<FINE>   convergence test on ||r|| (partials of k_down<CG>), then
+//                level-0 post-smoothing, partial (r, z), sum z, sum r
+//   k_pupdate    beta from k_up<FINE>'s partials; p = z + beta p, q = K p,
+//                partial (p, q), sum q
+// Grid reductions never use atomics: every block writes its partial sums and
+// the next kernel of the sequence (or a one-block finaliser) adds them in a
+// fixed order, so results are reproducible run to run.  A converged solve
+// turns the rest of the enqueued kernels into early returns.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -43,27 +48,24 @@ struct LevelDev {
 };
 
 struct Ctl {
-  double bb;      // ||g||^2
-  double rr;      // ||r||^2 (latest)
-  double rz;      // (r, z) of the latest V-cycle
-  double pq;
-  double alpha, beta;
-  double mean_r;  // mean of the initial residual (singular operator)
-  double mean_z;  // mean of z (singular operator)
+  double bb;      // ||g||^2 (projected for the singular operator)
+  double rr;      // ||r||^2 at the last convergence test
   double relres;
   double rtol;
-  double nwet;
-  double pad0;
+  double nwet;    // number of unknowns
+  double mean_r;  // mean of the initial residual (singular operator)
+  double rz[2];   // (r, z - mean z) of iteration k at index k & 1
+  double alpha, beta, mean_z, mean_q;  // last values (diagnostics)
   int done;
   int iters;
-  int x_valid;  // inner solve: the iterate has been written
-  int pad1;
+  int x_valid;    // inner solve of a refinement: the iterate has been written
+  int pad;
   // outer (refinement) control
-  double obb, orr, orelres;
-  int odone, orefine;
-  int xcur;        // which FP64 iterate buffer is current
-  int inner_total; // inner iterations summed over the refinements
-  double stats[4];  // iterations, relres, converged, refinements
+  double obb, orr, orelres, ortol, inner_rtol;
+  int odone, orefine, xcur, inner_total;
+  // iterations, relres, converged, refinements of the last solve; then, since
+  // the last reset: non-converged solves, max iterations, solves
+  double stats[8];
 };
 
 __device__ __forceinline__ bool lw(const LevelDev& L, int& y, int& x) {
@@ -93,6 +95,36 @@ __device__ __forceinline__ T finite_or_zero(T v) {
 }
 
 constexpr int BS = 256;
+constexpr int FC0 = 32;  // level-0 tile (points per side)
+
+// Partial-sum regions (each REGION_SLOTS x nblocks doubles).
+enum Region { R_INIT = 0, R_RR = 1, R_RZ = 2, R_PQ = 3, R_REF = 4, R_MEAN = 5,
+              R_COUNT = 6 };
+constexpr int REGION_SLOTS = 4;
+
+// Sum K arrays of n partials (arr k at p + k*n) in a fixed order: thread t
+// adds entries t, t+BS, ...; then a fixed-shape tree.  Every block that calls
+// this gets bit-identical results.
+template <int K>
+__device__ __forceinline__ void sum_partials(const double* p, int n,
+                                             double* sh, double out[K]) {
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < n; i += BS) {
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] += p[k * n + i];
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) out[k] = block_sum<BS>(acc[k], sh);
+}
+
+// 2-D tile of a level handled by one block: points [y0, y0+TS) x [x0, x0+TS)
+template <int TS>
+__device__ __forceinline__ void tile_origin(int& y0, int& x0) {
+  y0 = blockIdx.y * TS;
+  x0 = blockIdx.x * TS;
+}
 
 // ---------------------------------------------------------------------------
 // initial residual r0 = m (g - K x0), g = -b, -c2v(omega) or given
@@ -106,80 +138,93 @@ struct InitArgs {
   T* x;         // initial guess (sanitised in place), may be null (x0 = 0)
   T* r;
   Ctl* ctl;
-  double* partials;
-  unsigned* counter;
-  int singular;
+  double* part;
   double rtol;
 };
 
-template <class T, class TB, int GM>
-__device__ __forceinline__ double g_at(const InitArgs<T, TB>& a, int64_t k) {
-  if (GM == G_FROM_B) return -(double)a.b[k];
-  const int64_t P = a.L.stride;
+template <class TB, int GM>
+__device__ __forceinline__ double g_at(const TB* b, int64_t k, int64_t P) {
+  if (GM == G_FROM_B) return -(double)b[k];
   // centerstovertices, operators.py:131-132
-  TB s = TB(0.25) * (((a.b[k - P - 1] + a.b[k - 1]) + a.b[k - P]) + a.b[k]);
+  TB s = TB(0.25) * (((b[k - P - 1] + b[k - 1]) + b[k - P]) + b[k]);
   return -(double)s;
 }
 
 template <class T, class TB, int GM>
 __global__ __launch_bounds__(BS) void k_init(InitArgs<T, TB> a) {
   __shared__ double sh[BS];
-  __shared__ int flag;
   const LevelDev& L = a.L;
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
-  double gg = 0.0, rs = 0.0;
-  if (p < (int64_t)L.ny * L.nx) {
-    int y = (int)(p / L.nx), x = (int)(p - (int64_t)y * L.nx);
+  int y0, x0;
+  tile_origin<FC0>(y0, x0);
+  const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+  if (bid == 0 && threadIdx.x == 0) {
+    Ctl* c = a.ctl;
+    c->rtol = a.rtol;
+    c->done = 0;
+    c->iters = 0;
+    c->x_valid = 0;
+    c->rz[0] = c->rz[1] = 0.0;
+    c->alpha = c->beta = c->mean_z = c->mean_q = 0.0;
+    c->relres = 1.0;
+  }
+  double gg = 0.0, rs = 0.0, gs = 0.0;
+  for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
+    int y = y0 + e / FC0, x = x0 + e % FC0;
+    if (y >= L.ny || x >= L.nx) continue;
     int64_t k = li(L, y, x);
     T rv = T(0);
     if (L.m[k]) {
-      double g = g_at<T, TB, GM>(a, k);
+      double g = g_at<TB, GM>(a.b, k, L.stride);
       T gv = T(g);
       rv = gv;
       if (a.x) {
         T xc = finite_or_zero(a.x[k]);
         a.x[k] = xc;
-        T nb[4];
+        T nb4[4];
         const int dy[4] = {0, 0, -1, 1}, dx[4] = {-1, 1, 0, 0};
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           int yy = y + dy[q], xx = x + dx[q];
-          nb[q] = T(0);
+          nb4[q] = T(0);
           if (lw(L, yy, xx)) {
             int64_t kk = li(L, yy, xx);
-            if (L.m[kk]) nb[q] = finite_or_zero(a.x[kk]);
+            if (L.m[kk]) nb4[q] = finite_or_zero(a.x[kk]);
           }
         }
-        T Kx = T(L.d) * xc - T(L.cx) * (nb[0] + nb[1]) - T(L.cy) * (nb[2] + nb[3]);
+        T Kx = T(L.d) * xc - T(L.cx) * (nb4[0] + nb4[1]) -
+               T(L.cy) * (nb4[2] + nb4[3]);
         rv = gv - Kx;
       }
-      gg = (double)gv * (double)gv;
-      rs = (double)rv;
+      gg += (double)gv * (double)gv;
+      gs += (double)gv;
+      rs += (double)rv;
     }
     a.r[k] = rv;
   }
-  double bg = block_sum<BS>(gg, sh);
-  double br = block_sum<BS>(rs, sh);
+  double v0 = block_sum<BS>(gg, sh);
+  double v1 = block_sum<BS>(rs, sh);
+  double v2 = block_sum<BS>(gs, sh);
   if (threadIdx.x == 0) {
-    store_partial(a.partials + blockIdx.x, bg);
-    store_partial(a.partials + gridDim.x + blockIdx.x, br);
+    a.part[bid] = v0;
+    a.part[nb + bid] = v1;
+    a.part[2 * nb + bid] = v2;
   }
-  if (!last_block_arrive<BS>(a.counter, &flag)) return;
-  double tg = reduce_partials<BS>(a.partials, gridDim.x, sh, false);
-  double tr = reduce_partials<BS>(a.partials + gridDim.x, gridDim.x, sh, false);
+}
+
+// one block: ||g||^2 (projected if singular) and the mean of r0
+__global__ __launch_bounds__(BS) void k_init_fin(Ctl* c, const double* part,
+                                                 int nb, int singular) {
+  __shared__ double sh[BS];
+  double v[3];
+  sum_partials<3>(part, nb, sh, v);
   if (threadIdx.x == 0) {
-    Ctl* c = a.ctl;
-    c->bb = tg;
-    c->mean_r = a.singular ? tr / c->nwet : 0.0;
-    c->rtol = a.rtol;
-    c->done = 0;
-    c->iters = 0;
-    c->x_valid = 0;
-    c->alpha = 0.0;
-    c->beta = 0.0;
-    c->mean_z = 0.0;
-    c->rz = 0.0;
-    c->relres = 1.0;
+    double bb = v[0];
+    if (singular) {
+      bb = v[0] - v[2] * v[2] / c->nwet;
+      if (bb < 0.0) bb = 0.0;
+    }
+    c->bb = bb;
+    c->mean_r = singular ? v[1] / c->nwet : 0.0;
   }
 }
 
@@ -198,34 +243,49 @@ struct DownArgs {
   T* rc;        // restricted residual (coarse level)
   Ctl* ctl;
   const Ctl* outer;
-  double* partials;
-  unsigned* counter;
+  const double* part_in;  // CG, k > 0: partials of k_pupdate (pq, sum q)
+  double* part_out;       // CG: partial ||r||^2
+  int nb_in;
   int first;    // CG: iteration 0 (no update; singular projection)
   int iter;
   int inner;    // inner solve of a refinement: first update writes x
+  int singular;
 };
 
-template <class T, int NU, bool CG, bool COARSE>
+template <class T, int NU, bool CG, bool COARSE, int FC>
 __global__ __launch_bounds__(BS) void k_down(DownArgs<T> a) {
-  constexpr int CT = 16, FC = 2 * CT, R = 2 * NU + 1, W = FC + 2 * R;
+  constexpr int CT = FC / 2, R = 2 * NU + 1, W = FC + 2 * R;
   __shared__ T sr[W * W];
   __shared__ T sz[W * W];
   __shared__ int8_t sm[W * W];
   __shared__ double sh[BS];
-  __shared__ int flag;
   if (stop(a.ctl, a.outer)) return;
   const LevelDev& F = a.F;
   const int t = threadIdx.x;
   const int cy0 = blockIdx.y * CT, cx0 = blockIdx.x * CT;
   const int fy0 = 2 * cy0, fx0 = 2 * cx0;
-  T alpha = T(0), shift = T(0);
+  T alpha = T(0), shift = T(0), mq = T(0);
   bool zero_x = false;
   if (CG) {
+    Ctl* c = a.ctl;
     if (a.first) {
-      shift = T(a.ctl->mean_r);
-      zero_x = a.ctl->bb == 0.0;
+      shift = T(c->mean_r);
+      zero_x = c->bb == 0.0;
     } else {
-      alpha = T(a.ctl->alpha);
+      double v[2];
+      sum_partials<2>(a.part_in, a.nb_in, sh, v);
+      double rzk = c->rz[(a.iter + 1) & 1];
+      double al = v[0] > 0.0 ? rzk / v[0] : 0.0;
+      alpha = T(al);
+      mq = T(a.singular ? v[1] / c->nwet : 0.0);
+      if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+        c->alpha = al;
+        c->mean_q = (double)mq;
+      }
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+      c->iters = a.iter;
+      if (!a.first && a.inner) c->x_valid = 1;
     }
   }
   for (int k = t; k < W * W; k += BS) {
@@ -238,7 +298,7 @@ __global__ __launch_bounds__(BS) void k_down(DownArgs<T> a) {
       mv = F.m[g];
       if (mv) {
         rv = a.r_in[g];
-        if (CG) rv = a.first ? rv - shift : rv - alpha * a.q[g];
+        if (CG) rv = a.first ? rv - shift : rv - alpha * (a.q[g] - mq);
       }
     }
     sr[k] = rv;
@@ -287,19 +347,19 @@ __global__ __launch_bounds__(BS) void k_down(DownArgs<T> a) {
     }
     __syncthreads();
   }
-  // residual within radius 1 (into sr)
-  for (int k = t; k < W * W; k += BS) {
-    int ly = k / W, lx = k - ly * W;
-    if (ly < R - 1 || ly >= R + FC + 1 || lx < R - 1 || lx >= R + FC + 1)
-      continue;
-    T res = T(0);
-    if (sm[k])
-      res = sr[k] - (dd * sz[k] - cx * (sz[k - 1] + sz[k + 1]) -
-                     cy * (sz[k - W] + sz[k + W]));
-    sr[k] = res;
-  }
-  __syncthreads();
   if (COARSE) {
+    // residual within radius 1 (into sr)
+    for (int k = t; k < W * W; k += BS) {
+      int ly = k / W, lx = k - ly * W;
+      if (ly < R - 1 || ly >= R + FC + 1 || lx < R - 1 || lx >= R + FC + 1)
+        continue;
+      T res = T(0);
+      if (sm[k])
+        res = sr[k] - (dd * sz[k] - cx * (sz[k - 1] + sz[k + 1]) -
+                       cy * (sz[k - W] + sz[k + W]));
+      sr[k] = res;
+    }
+    __syncthreads();
     const LevelDev& C = a.C;
     for (int e = t; e < CT * CT; e += BS) {
       int jj = e / CT, ii = e - jj * CT;
@@ -321,24 +381,12 @@ __global__ __launch_bounds__(BS) void k_down(DownArgs<T> a) {
   }
   if (!CG) return;
   double brr = block_sum<BS>(rr, sh);
-  const int nb = gridDim.x * gridDim.y;
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-  if (t == 0) store_partial(a.partials + bid, brr);
-  if (!last_block_arrive<BS>(a.counter, &flag)) return;
-  double tot = reduce_partials<BS>(a.partials, nb, sh, false);
-  if (t == 0) {
-    Ctl* c = a.ctl;
-    c->rr = tot;
-    c->iters = a.iter;
-    if (!a.first && a.inner) c->x_valid = 1;
-    double rel = c->bb > 0.0 ? sqrt(tot / c->bb) : 0.0;
-    c->relres = rel;
-    if (c->bb == 0.0 || rel <= c->rtol || !(rel == rel)) c->done = 1;
-  }
+  if (t == 0) a.part_out[blockIdx.y * gridDim.x + blockIdx.x] = brr;
 }
 
 // ---------------------------------------------------------------------------
-// up pass: x = zpre + P e_coarse, post-smooth; on level 0 also (r,z), sum z
+// up pass: x = zpre + P e_coarse, post-smooth; on level 0 the convergence
+// test first, then (r,z), sum z, sum r partials
 // ---------------------------------------------------------------------------
 template <class T>
 struct UpArgs {
@@ -349,25 +397,37 @@ struct UpArgs {
   T* out;         // correction of this level (level 0: z)
   Ctl* ctl;
   const Ctl* outer;
-  double* partials;
-  unsigned* counter;
-  int first;
-  int singular;
+  const double* part_in;  // FINE: ||r||^2 partials of k_down<CG>
+  double* part_out;       // FINE: (r,z), sum z, sum r
+  int nb_in;
+  int bench;              // timing run: no convergence test
 };
 
-template <class T, int NU, bool FINE, bool COARSE>
+template <class T, int NU, bool FINE, bool COARSE, int FC>
 __global__ __launch_bounds__(BS) void k_up(UpArgs<T> a) {
-  constexpr int FC = 32, R = 2 * NU, W = FC + 2 * R;
+  constexpr int R = 2 * NU, W = FC + 2 * R;
   constexpr int CW = FC / 2 + 2 * NU + 2;
   __shared__ T sx[W * W];
   __shared__ T sr[W * W];
   __shared__ int8_t sm[W * W];
   __shared__ T sc[COARSE ? CW * CW : 1];
   __shared__ double sh[BS];
-  __shared__ int flag;
   if (stop(a.ctl, a.outer)) return;
   const LevelDev& F = a.F;
   const int t = threadIdx.x;
+  if (FINE) {
+    double v[1];
+    sum_partials<1>(a.part_in, a.nb_in, sh, v);
+    Ctl* c = a.ctl;
+    double rel = c->bb > 0.0 ? sqrt(v[0] / c->bb) : 0.0;
+    bool conv = !a.bench && (c->bb == 0.0 || rel <= c->rtol || !(rel == rel));
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t == 0) {
+      c->rr = v[0];
+      c->relres = rel;
+      if (conv) c->done = 1;
+    }
+    if (conv) return;
+  }
   const int fy0 = blockIdx.y * FC, fx0 = blockIdx.x * FC;
   for (int k = t; k < W * W; k += BS) {
     int ly = k / W, lx = k - ly * W;
@@ -433,7 +493,7 @@ __global__ __launch_bounds__(BS) void k_up(UpArgs<T> a) {
     }
     __syncthreads();
   }
-  double rz = 0.0, zs = 0.0;
+  double rz = 0.0, zs = 0.0, rs = 0.0;
   for (int e = t; e < FC * FC; e += BS) {
     int cy_ = e / FC, cx_ = e - cy_ * FC;
     int y = fy0 + cy_, x = fx0 + cx_;
@@ -444,34 +504,23 @@ __global__ __launch_bounds__(BS) void k_up(UpArgs<T> a) {
     if (FINE && sm[k]) {
       rz += (double)sr[k] * (double)v;
       zs += (double)v;
+      rs += (double)sr[k];
     }
   }
   if (!FINE) return;
-  double brz = block_sum<BS>(rz, sh);
-  double bzs = block_sum<BS>(zs, sh);
-  const int nb = gridDim.x * gridDim.y;
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  double b0 = block_sum<BS>(rz, sh);
+  double b1 = block_sum<BS>(zs, sh);
+  double b2 = block_sum<BS>(rs, sh);
+  const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
   if (t == 0) {
-    store_partial(a.partials + bid, brz);
-    store_partial(a.partials + nb + bid, bzs);
-  }
-  if (!last_block_arrive<BS>(a.counter, &flag)) return;
-  double trz = reduce_partials<BS>(a.partials, nb, sh, false);
-  double tzs = reduce_partials<BS>(a.partials + nb, nb, sh, false);
-  if (t == 0) {
-    Ctl* c = a.ctl;
-    double mz = a.singular ? tzs / c->nwet : 0.0;
-    // (r, z - mz) = (r, z) - mz * sum(r); sum(r) vanishes for the singular
-    // operator up to rounding, so (r, z) is kept.
-    c->beta = a.first ? 0.0 : trz / c->rz;
-    c->rz = trz;
-    c->mean_z = mz;
-    if (!(trz > 0.0)) c->done = 1;  // breakdown / exact solution
+    a.part_out[bid] = b0;
+    a.part_out[nb + bid] = b1;
+    a.part_out[2 * nb + bid] = b2;
   }
 }
 
 // ---------------------------------------------------------------------------
-// p = (z - mean z) + beta p_old ;  q = K p ;  alpha = (r,z) / (p,q)
+// p = (z - mean z) + beta p_old ;  q = K p ;  partial (p, q), sum q
 // ---------------------------------------------------------------------------
 template <class T>
 struct PArgs {
@@ -482,9 +531,13 @@ struct PArgs {
   T* q;
   Ctl* ctl;
   const Ctl* outer;
-  double* partials;
-  unsigned* counter;
+  const double* part_in;  // (r,z), sum z, sum r of k_up<FINE>
+  double* part_out;       // (p,q), sum q
+  int nb_in;
   int first;
+  int iter;
+  int singular;
+  int bench;
 };
 
 template <class T>
@@ -501,208 +554,373 @@ __device__ __forceinline__ T pnew_at(const PArgs<T>& a, int y, int x, T beta,
 template <class T>
 __global__ __launch_bounds__(BS) void k_pupdate(PArgs<T> a) {
   __shared__ double sh[BS];
-  __shared__ int flag;
   if (stop(a.ctl, a.outer)) return;
+  Ctl* c = a.ctl;
+  double v[3];
+  sum_partials<3>(a.part_in, a.nb_in, sh, v);
+  const double mzd = a.singular ? v[1] / c->nwet : 0.0;
+  // singular operator: the search direction uses z - mean(z), so the scalar
+  // is (r, z - mz) = (r, z) - mz * sum(r)
+  const double rzk = a.singular ? v[0] - mzd * v[2] : v[0];
+  const double rzo = c->rz[(a.iter + 1) & 1];
+  const double bet = a.first ? 0.0 : (rzo != 0.0 ? rzk / rzo : 0.0);
+  const bool brk = !a.bench && !(rzk > 0.0);  // breakdown / exact solution
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  if (bid == 0 && threadIdx.x == 0) {
+    c->rz[a.iter & 1] = rzk;
+    c->beta = bet;
+    c->mean_z = mzd;
+    if (brk) c->done = 1;
+  }
+  if (brk) return;
   const LevelDev& L = a.L;
-  const T beta = T(a.ctl->beta), mz = T(a.ctl->mean_z);
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
-  double pq = 0.0;
-  if (p < (int64_t)L.ny * L.nx) {
-    int y = (int)(p / L.nx), x = (int)(p - (int64_t)y * L.nx);
+  const T beta = T(bet), mz = T(mzd);
+  int y0, x0;
+  tile_origin<FC0>(y0, x0);
+  double pq = 0.0, qs = 0.0;
+  for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
+    int y = y0 + e / FC0, x = x0 + e % FC0;
+    if (y >= L.ny || x >= L.nx) continue;
     int64_t k = li(L, y, x);
     T pc = T(0), qv = T(0);
     if (L.m[k]) {
       pc = pnew_at(a, y, x, beta, mz);
-      T w = pnew_at(a, y, x - 1, beta, mz), e = pnew_at(a, y, x + 1, beta, mz);
+      T w = pnew_at(a, y, x - 1, beta, mz), e2 = pnew_at(a, y, x + 1, beta, mz);
       T s = pnew_at(a, y - 1, x, beta, mz), n = pnew_at(a, y + 1, x, beta, mz);
-      qv = T(L.d) * pc - T(L.cx) * (w + e) - T(L.cy) * (s + n);
-      pq = (double)pc * (double)qv;
+      qv = T(L.d) * pc - T(L.cx) * (w + e2) - T(L.cy) * (s + n);
+      pq += (double)pc * (double)qv;
+      qs += (double)qv;
     }
     a.p[k] = pc;
     a.q[k] = qv;
   }
-  double b = block_sum<BS>(pq, sh);
-  if (threadIdx.x == 0) store_partial(a.partials + blockIdx.x, b);
-  if (!last_block_arrive<BS>(a.counter, &flag)) return;
-  double tot = reduce_partials<BS>(a.partials, gridDim.x, sh, false);
+  double b0 = block_sum<BS>(pq, sh);
+  double b1 = block_sum<BS>(qs, sh);
+  const int nb = gridDim.x * gridDim.y;
   if (threadIdx.x == 0) {
-    Ctl* c = a.ctl;
-    c->pq = tot;
-    if (tot > 0.0)
-      c->alpha = c->rz / tot;
-    else
-      c->done = 1;
+    a.part_out[bid] = b0;
+    a.part_out[nb + bid] = b1;
   }
 }
 
+// one block after the last iteration: final residual and statistics
+__global__ __launch_bounds__(BS) void k_finish(Ctl* c, const double* part_rr,
+                                               int nb) {
+  __shared__ double sh[BS];
+  double v[1];
+  sum_partials<1>(part_rr, nb, sh, v);
+  if (threadIdx.x != 0) return;
+  double rel = c->bb > 0.0 ? sqrt(v[0] / c->bb) : 0.0;
+  c->rr = v[0];
+  c->relres = rel;
+  double conv = (c->bb == 0.0 || rel <= c->rtol) ? 1.0 : 0.0;
+  if (conv > 0.0) c->done = 1;
+  c->stats[0] = (double)c->iters;
+  c->stats[1] = rel;
+  c->stats[2] = conv;
+  c->stats[3] = 0.0;
+  c->stats[4] += conv > 0.0 ? 0.0 : 1.0;
+  if ((double)c->iters > c->stats[5]) c->stats[5] = (double)c->iters;
+  c->stats[6] += 1.0;
+}
+
 // ---------------------------------------------------------------------------
-// tail: levels Lt .. L-1 of the V-cycle in one workgroup, all in LDS
+// tail: levels Lt .. L-1 of the V-cycle in one workgroup, all in LDS.
+//
+// The host writes the V-cycle as a schedule of operations (half-sweep,
+// residual, restriction, prolongation, mean projection); the kernel runs the
+// schedule with one copy of each operation's code, which keeps it small
+// enough for the instruction cache.  Levels larger than WAVE_PTS points run
+// on the whole workgroup, smaller ones on wavefront 0 alone (LDS operations
+// of one wavefront complete in order, so a wave-level barrier suffices).
 // ---------------------------------------------------------------------------
 constexpr int MAXL = 16;
-constexpr int TAIL_BS = 1024;
+constexpr int WAVE_PTS = 256;
+constexpr int MAXOPS = 512;
+
+enum TailOp { OP_SWEEP = 0, OP_RES = 1, OP_RESTRICT = 2, OP_PROLONG = 3,
+              OP_MEAN_R = 4, OP_MEAN_X = 5 };
+
+struct TailStep {
+  int8_t op, level, col, fromzero, wave;
+  int8_t pad[3];
+};
 
 template <class T>
 struct TailArgs {
   LevelDev lv[MAXL];
   int off_r[MAXL], off_x[MAXL], off_m[MAXL];
+  int lgx[MAXL];  // log2(nx) when nx is a power of two, else -1
+  int off_tmp;    // residual scratch of the largest tail level
   int nl;
-  int coarse_sweeps;
+  int nsteps;
+  const TailStep* steps;  // device schedule
   const T* r_in;
   T* e_out;
   Ctl* ctl;
   const Ctl* outer;
 };
 
-template <class T>
-__device__ __forceinline__ T lds_at(const T* a, const int8_t* m,
-                                    const LevelDev& L, int y, int x) {
-  if (!lw(L, y, x)) return T(0);
-  int k = y * L.nx + x;
-  return m[k] ? a[k] : T(0);
-}
-
-template <class T>
-__device__ void tail_halfsweep(T* xa, const T* ra, const int8_t* m,
-                               const LevelDev& L, int col, bool fromzero) {
-  const T cx = T(L.cx), cy = T(L.cy), id = T(L.inv_d);
-  const int n = L.ny * L.nx;
-  for (int k = threadIdx.x; k < n; k += TAIL_BS) {
-    int y = k / L.nx, x = k - y * L.nx;
-    if (((y + x) & 1) != col || !m[k]) continue;
-    if (fromzero) {
-      xa[k] = ra[k] * id;
-    } else {
-      T w = lds_at(xa, m, L, y, x - 1), e = lds_at(xa, m, L, y, x + 1);
-      T s = lds_at(xa, m, L, y - 1, x), nn = lds_at(xa, m, L, y + 1, x);
-      xa[k] = (ra[k] + cx * (w + e) + cy * (s + nn)) * id;
-    }
+// point j of a level in row-major order -> (y, x)
+__device__ __forceinline__ void yx_of(int j, int nx, int lg, int& y, int& x) {
+  if (lg >= 0) {
+    y = j >> lg;
+    x = j & (nx - 1);
+  } else {
+    y = j / nx;
+    x = j - y * nx;
   }
-  __syncthreads();
+}
+
+// Neighbour (y+dy, x+dx), |dy|,|dx| <= 1, of an LDS level array: wrapped in
+// periodic directions, zero outside closed ones, evaluated without branches
+// (the index is clamped, the value selected).  Level arrays vanish at dry
+// points by construction, so no mask test is needed.
+template <class T>
+__device__ __forceinline__ T nb(const T* a, const LevelDev& L, int y, int x,
+                                int dy, int dx) {
+  int yy = y + dy, xx = x + dx;
+  const bool xo = xx < 0 || xx >= L.nx, yo = yy < 0 || yy >= L.ny;
+  xx = xx < 0 ? xx + L.nx : (xx >= L.nx ? xx - L.nx : xx);
+  yy = yy < 0 ? yy + L.ny : (yy >= L.ny ? yy - L.ny : yy);
+  const T v = a[yy * L.nx + xx];
+  const bool ok = (!xo || L.xper) && (!yo || L.yper);
+  return ok ? v : T(0);
 }
 
 template <class T>
-__device__ __forceinline__ T tail_res(const T* xa, const T* ra,
-                                      const int8_t* m, const LevelDev& L,
-                                      int y, int x) {
-  if (!lw(L, y, x)) return T(0);
-  int k = y * L.nx + x;
-  if (!m[k]) return T(0);
-  T w = lds_at(xa, m, L, y, x - 1), e = lds_at(xa, m, L, y, x + 1);
-  T s = lds_at(xa, m, L, y - 1, x), nn = lds_at(xa, m, L, y + 1, x);
-  return ra[k] - (T(L.d) * xa[k] - T(L.cx) * (w + e) - T(L.cy) * (s + nn));
+__device__ __forceinline__ void nbsum(const T* a, const LevelDev& L, int y,
+                                      int x, int k, T& sx, T& sy) {
+  sx = nb(a, L, y, x, 0, -1) + nb(a, L, y, x, 0, 1);
+  sy = nb(a, L, y, x, -1, 0) + nb(a, L, y, x, 1, 0);
+  (void)k;
 }
 
-template <class T, int NU>
-__global__ __launch_bounds__(TAIL_BS) void k_tail(TailArgs<T> a) {
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_tail(TailArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // Level descriptors, offsets and the schedule are indexed at run time:
+  // staged once in LDS, so the per-step accesses do not go back to the
+  // kernel-argument segment.
+  __shared__ LevelDev s_lv[MAXL];
+  __shared__ int s_off[4][MAXL];
+  __shared__ TailStep s_st[MAXOPS];
   if (stop(a.ctl, a.outer)) return;
-  const int t = threadIdx.x;
-  auto R_ = [&](int l) { return reinterpret_cast<T*>(smem + a.off_r[l]); };
-  auto X_ = [&](int l) { return reinterpret_cast<T*>(smem + a.off_x[l]); };
-  auto M_ = [&](int l) { return reinterpret_cast<int8_t*>(smem + a.off_m[l]); };
-  for (int l = 0; l < a.nl; l++) {
-    const LevelDev& L = a.lv[l];
-    int n = L.ny * L.nx;
-    int8_t* m = M_(l);
-    T* r = R_(l);
-    T* x = X_(l);
-    for (int k = t; k < n; k += TAIL_BS) {
-      int y = k / L.nx, xx = k - y * L.nx;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int wave = tid >> 6;
+  for (int i = tid; i < a.nl; i += nth) {
+    s_lv[i] = a.lv[i];
+    s_off[0][i] = a.off_r[i];
+    s_off[1][i] = a.off_x[i];
+    s_off[2][i] = a.off_m[i];
+    s_off[3][i] = a.lgx[i];
+  }
+  for (int i = tid; i < a.nsteps; i += nth) s_st[i] = a.steps[i];
+  __syncthreads();
+  T* tmp = reinterpret_cast<T*>(smem + a.off_tmp);
+  const int nl = a.nl, nsteps = a.nsteps;
+  for (int l = 0; l < nl; l++) {
+    const LevelDev L = s_lv[l];
+    const int n = L.ny * L.nx;
+    int8_t* m = reinterpret_cast<int8_t*>(smem + s_off[2][l]);
+    T* r = reinterpret_cast<T*>(smem + s_off[0][l]);
+    T* x = reinterpret_cast<T*>(smem + s_off[1][l]);
+    const int lg = s_off[3][l];
+    for (int k = tid; k < n; k += nth) {
+      int y, xx;
+      yx_of(k, L.nx, lg, y, xx);
       int64_t g = li(L, y, xx);
-      m[k] = L.m[g];
+      int8_t mv = L.m[g];
+      m[k] = mv;
       x[k] = T(0);
-      if (l == 0) r[k] = a.r_in[g];
+      if (l == 0) r[k] = mv ? a.r_in[g] : T(0);
+    }
+  }
+  // every wave walks the schedule: a block-mode step, or a wave-mode step
+  // that follows a block-mode one, starts with a workgroup barrier (the same
+  // decision in every wave); consecutive wave-mode steps are separated by a
+  // wave-level barrier in wavefront 0 while the others skip them
+  int prev_wave = 0;
+#pragma unroll 1
+  for (int i = 0; i < nsteps; i++) {
+    const TailStep st = s_st[i];
+    if (!st.wave || !prev_wave) __syncthreads();
+    prev_wave = st.wave;
+    int NT;
+    if (st.wave) {
+      if (wave != 0) continue;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      NT = 64;
+    } else {
+      NT = nth;
+    }
+    const int l = st.level;
+    const LevelDev L = s_lv[l];
+    const int lg = s_off[3][l];
+    const int8_t* m = reinterpret_cast<const int8_t*>(smem + s_off[2][l]);
+    T* r = reinterpret_cast<T*>(smem + s_off[0][l]);
+    T* x = reinterpret_cast<T*>(smem + s_off[1][l]);
+    const int n = L.ny * L.nx;
+    if (st.op == OP_SWEEP) {
+      const T cx = T(L.cx), cy = T(L.cy), id = T(L.inv_d);
+      const int col = st.col;
+      const bool even = !(L.nx & 1);
+      const int hx = L.nx >> 1, lgh = lg > 0 ? lg - 1 : -1;
+      const int cnt = even ? L.ny * hx : n;
+      for (int j = tid; j < cnt; j += NT) {
+        int y, xx;
+        if (even) {
+          int i2;
+          yx_of(j, hx, lgh, y, i2);
+          xx = 2 * i2 + ((col ^ y) & 1);
+        } else {
+          yx_of(j, L.nx, lg, y, xx);
+          if (((y + xx) & 1) != col) continue;
+        }
+        const int k = y * L.nx + xx;
+        const bool wet = m[k] != 0;
+        T v;
+        if (st.fromzero) {
+          v = r[k] * id;
+        } else {
+          T sx, sy;
+          nbsum(x, L, y, xx, k, sx, sy);
+          v = (r[k] + cx * sx + cy * sy) * id;
+        }
+        if (wet) x[k] = v;
+      }
+    } else if (st.op == OP_RES) {
+      const T cx = T(L.cx), cy = T(L.cy), dd = T(L.d);
+      for (int k = tid; k < n; k += NT) {
+        int y, xx;
+        yx_of(k, L.nx, lg, y, xx);
+        const bool wet = m[k] != 0;
+        T sx, sy;
+        nbsum(x, L, y, xx, k, sx, sy);
+        const T res = r[k] - (dd * x[k] - cx * sx - cy * sy);
+        tmp[k] = wet ? res : T(0);
+      }
+    } else if (st.op == OP_RESTRICT) {  // level l -> l + 1, from tmp
+      const LevelDev C = s_lv[l + 1];
+      const int lgc = s_off[3][l + 1];
+      const int8_t* mc = reinterpret_cast<const int8_t*>(smem + s_off[2][l + 1]);
+      T* rc = reinterpret_cast<T*>(smem + s_off[0][l + 1]);
+      const int nc = C.ny * C.nx;
+      for (int k = tid; k < nc; k += NT) {
+        int J, I;
+        yx_of(k, C.nx, lgc, J, I);
+        const int y = 2 * J, xx = 2 * I;
+        const bool wet = mc[k] != 0;
+        T v = nb(tmp, L, y, xx, 0, 0) +
+              T(0.5) * ((nb(tmp, L, y, xx, 0, -1) + nb(tmp, L, y, xx, 0, 1)) +
+                        (nb(tmp, L, y, xx, -1, 0) + nb(tmp, L, y, xx, 1, 0))) +
+              T(0.25) * ((nb(tmp, L, y, xx, -1, -1) + nb(tmp, L, y, xx, -1, 1)) +
+                         (nb(tmp, L, y, xx, 1, -1) + nb(tmp, L, y, xx, 1, 1)));
+        rc[k] = wet ? T(0.25) * v : T(0);
+      }
+    } else if (st.op == OP_PROLONG) {  // level l + 1 -> l, added to x
+      const LevelDev C = s_lv[l + 1];
+      const T* xc = reinterpret_cast<const T*>(smem + s_off[1][l + 1]);
+      for (int k = tid; k < n; k += NT) {
+        int y, xx;
+        yx_of(k, L.nx, lg, y, xx);
+        const bool wet = m[k] != 0;
+        const int J = y >> 1, I = xx >> 1;
+        const bool oy = y & 1, ox = xx & 1;
+        const T c00 = xc[J * C.nx + I];
+        const T c01 = nb(xc, C, J, I, 0, 1), c10 = nb(xc, C, J, I, 1, 0);
+        const T c11 = nb(xc, C, J, I, 1, 1);
+        T v = c00;
+        if (ox && !oy)
+          v = T(0.5) * (c00 + c01);
+        else if (oy && !ox)
+          v = T(0.5) * (c00 + c10);
+        else if (oy && ox)
+          v = T(0.25) * ((c00 + c01) + (c10 + c11));
+        const T xv = x[k];
+        if (wet) x[k] = xv + v;
+      }
+    } else {  // OP_MEAN_R / OP_MEAN_X: zero-mean projection (wave mode)
+      T* v = st.op == OP_MEAN_R ? r : x;
+      double sm = 0.0, c = 0.0;
+      for (int k = tid; k < n; k += 64)
+        if (m[k]) {
+          sm += (double)v[k];
+          c += 1.0;
+        }
+      sm = wave_sum(sm);
+      c = wave_sum(c);
+      const double mean = c > 0 ? sm / c : 0.0;
+      for (int k = tid; k < n; k += 64)
+        if (m[k]) v[k] = v[k] - T(mean);
     }
   }
   __syncthreads();
-  // down
-  for (int l = 0; l + 1 < a.nl; l++) {
-    const LevelDev& L = a.lv[l];
-    const LevelDev& C = a.lv[l + 1];
-    T* x = X_(l);
-    T* r = R_(l);
-    int8_t* m = M_(l);
-    for (int h = 1; h <= 2 * NU; h++)
-      tail_halfsweep(x, r, m, L, (h - 1) & 1, h == 1);
-    int nc = C.ny * C.nx;
-    T* rc = R_(l + 1);
-    int8_t* mc = M_(l + 1);
-    for (int k = t; k < nc; k += TAIL_BS) {
-      int J = k / C.nx, I = k - J * C.nx;
-      T v = T(0);
-      if (mc[k]) {
-        int y = 2 * J, xx = 2 * I;
-        v = tail_res(x, r, m, L, y, xx) +
-            T(0.5) * ((tail_res(x, r, m, L, y, xx - 1) +
-                       tail_res(x, r, m, L, y, xx + 1)) +
-                      (tail_res(x, r, m, L, y - 1, xx) +
-                       tail_res(x, r, m, L, y + 1, xx))) +
-            T(0.25) * ((tail_res(x, r, m, L, y - 1, xx - 1) +
-                        tail_res(x, r, m, L, y - 1, xx + 1)) +
-                       (tail_res(x, r, m, L, y + 1, xx - 1) +
-                        tail_res(x, r, m, L, y + 1, xx + 1)));
-        v = T(0.25) * v;
-      }
-      rc[k] = v;
-    }
-    __syncthreads();
-  }
-  // coarsest: palindromic symmetric Gauss-Seidel (red, black)^S (black, red)^S
   {
-    int l = a.nl - 1;
-    const LevelDev& L = a.lv[l];
-    T* x = X_(l);
-    T* r = R_(l);
-    int8_t* m = M_(l);
-    int S = a.coarse_sweeps;
-    for (int s = 0; s < S; s++) {
-      tail_halfsweep(x, r, m, L, 0, s == 0);
-      tail_halfsweep(x, r, m, L, 1, false);
-    }
-    for (int s = 0; s < S; s++) {
-      tail_halfsweep(x, r, m, L, 1, false);
-      tail_halfsweep(x, r, m, L, 0, false);
-    }
-  }
-  // up
-  for (int l = a.nl - 2; l >= 0; l--) {
-    const LevelDev& L = a.lv[l];
-    const LevelDev& C = a.lv[l + 1];
-    T* x = X_(l);
-    T* r = R_(l);
-    int8_t* m = M_(l);
-    const T* xc = X_(l + 1);
-    const int8_t* mc = M_(l + 1);
-    int n = L.ny * L.nx;
-    for (int k = t; k < n; k += TAIL_BS) {
-      if (!m[k]) continue;
-      int y = k / L.nx, xx = k - y * L.nx;
-      int J = y >> 1, I = xx >> 1;
-      T v;
-      if (!(y & 1) && !(xx & 1))
-        v = lds_at(xc, mc, C, J, I);
-      else if (!(y & 1))
-        v = T(0.5) * (lds_at(xc, mc, C, J, I) + lds_at(xc, mc, C, J, I + 1));
-      else if (!(xx & 1))
-        v = T(0.5) * (lds_at(xc, mc, C, J, I) + lds_at(xc, mc, C, J + 1, I));
-      else
-        v = T(0.25) * ((lds_at(xc, mc, C, J, I) + lds_at(xc, mc, C, J, I + 1)) +
-                       (lds_at(xc, mc, C, J + 1, I) +
-                        lds_at(xc, mc, C, J + 1, I + 1)));
-      x[k] = x[k] + v;
-    }
-    __syncthreads();
-    for (int h = 1; h <= 2 * NU; h++) tail_halfsweep(x, r, m, L, h & 1, false);
-  }
-  {
-    const LevelDev& L = a.lv[0];
-    const T* x = X_(0);
-    int n = L.ny * L.nx;
-    for (int k = t; k < n; k += TAIL_BS) {
-      int y = k / L.nx, xx = k - y * L.nx;
+    const LevelDev L = s_lv[0];
+    const T* x = reinterpret_cast<const T*>(smem + s_off[1][0]);
+    const int n = L.ny * L.nx;
+    const int lg = s_off[3][0];
+    for (int k = tid; k < n; k += nth) {
+      int y, xx;
+      yx_of(k, L.nx, lg, y, xx);
       a.e_out[li(L, y, xx)] = x[k];
     }
   }
+}
+
+// The tail's schedule: pre-smoothing (red, black)^nu, residual, restriction
+// down to the coarsest level; (red, black)^S (black, red)^S there (with the
+// zero-mean projection of the singular operator); prolongation and
+// post-smoothing (black, red)^nu back up.
+static std::vector<TailStep> tail_schedule(const std::vector<int>& npts,
+                                           int nu, int coarse_sweeps,
+                                           bool singular) {
+  std::vector<TailStep> v;
+  const int nl = (int)npts.size();
+  auto add = [&](int op, int l, int col, int fz, int wmode) {
+    TailStep s{};
+    s.op = (int8_t)op;
+    s.level = (int8_t)l;
+    s.col = (int8_t)col;
+    s.fromzero = (int8_t)fz;
+    s.wave = (int8_t)wmode;
+    v.push_back(s);
+  };
+  for (int l = 0; l + 1 < nl; l++) {
+    const int w = npts[l] <= WAVE_PTS;
+    for (int h = 1; h <= 2 * nu; h++) add(OP_SWEEP, l, (h - 1) & 1, h == 1, w);
+    add(OP_RES, l, 0, 0, w);
+    add(OP_RESTRICT, l, 0, 0, w);
+  }
+  {
+    const int l = nl - 1;
+    const int w = npts[l] <= WAVE_PTS;
+    if (singular) add(OP_MEAN_R, l, 0, 0, 1);
+    for (int s = 0; s < coarse_sweeps; s++) {
+      add(OP_SWEEP, l, 0, s == 0, w);
+      add(OP_SWEEP, l, 1, 0, w);
+    }
+    for (int s = 0; s < coarse_sweeps; s++) {
+      add(OP_SWEEP, l, 1, 0, w);
+      add(OP_SWEEP, l, 0, 0, w);
+    }
+    if (singular) add(OP_MEAN_X, l, 0, 0, 1);
+  }
+  for (int l = nl - 2; l >= 0; l--) {
+    const int w = npts[l] <= WAVE_PTS;
+    add(OP_PROLONG, l, 0, 0, w);
+    for (int h = 1; h <= 2 * nu; h++) add(OP_SWEEP, l, h & 1, 0, w);
+  }
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -717,43 +935,46 @@ __device__ __forceinline__ double g64_at(const TB* b, int64_t k, int64_t P,
   return -s;
 }
 
-// mean of g over the unknowns, in FP64 (the singular operator's right-hand
-// side is projected before any FP32 rounding)
+// partial sums of g over the unknowns, in FP64 (the singular operator's
+// right-hand side is projected before any FP32 rounding)
 template <class TB>
-__global__ __launch_bounds__(BS) void k_gmean(LevelDev L, const TB* b,
-                                              int gmode, double* partials,
-                                              unsigned* counter, double nwet,
-                                              double* out) {
+__global__ __launch_bounds__(BS) void k_gsum(LevelDev L, const TB* b,
+                                             int gmode, double* part) {
   __shared__ double sh[BS];
-  __shared__ int flag;
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
+  int y0, x0;
+  tile_origin<FC0>(y0, x0);
   double s = 0.0;
-  if (p < (int64_t)L.ny * L.nx) {
-    int y = (int)(p / L.nx), xx = (int)(p - (int64_t)y * L.nx);
-    int64_t k = li(L, y, xx);
-    if (L.m[k]) s = g64_at(b, k, L.stride, gmode);
+  for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
+    int y = y0 + e / FC0, x = x0 + e % FC0;
+    if (y >= L.ny || x >= L.nx) continue;
+    int64_t k = li(L, y, x);
+    if (L.m[k]) s += g64_at(b, k, L.stride, gmode);
   }
   double bs = block_sum<BS>(s, sh);
-  if (threadIdx.x == 0) store_partial(partials + blockIdx.x, bs);
-  if (!last_block_arrive<BS>(counter, &flag)) return;
-  double tot = reduce_partials<BS>(partials, gridDim.x, sh, false);
-  if (threadIdx.x == 0) *out = tot / nwet;
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = bs;
+}
+
+// one block: out = (sum of partials) / n
+__global__ __launch_bounds__(BS) void k_mean_fin(const double* part, int nb,
+                                                 double n, double* out) {
+  __shared__ double sh[BS];
+  double v[1];
+  sum_partials<1>(part, nb, sh, v);
+  if (threadIdx.x == 0) *out = v[0] / n;
 }
 
 template <class TB>
 struct ResArgs {
   LevelDev L;
-  const TB* b;           // b or omega (padded)
-  double* x64a;          // FP64 iterate buffers; outer->xcur selects the
-  double* x64b;          // current one, the kernel writes the other
-  const float* e;        // inner FP32 correction
-  float* r32;            // inner right-hand side (= its initial residual)
-  Ctl* inner;
-  Ctl* outer;
-  double* partials;
-  unsigned* counter;
-  int gmode, singular, add_e, last;
-  double rtol, inner_rtol;
+  const TB* b;      // b or omega (padded)
+  double* x64a;     // FP64 iterate buffers; outer->xcur selects the current
+  double* x64b;     // one, the kernel writes the other
+  const float* e;   // inner FP32 correction
+  float* r32;       // inner right-hand side (= its initial residual)
+  const Ctl* inner;
+  const Ctl* outer;
+  double* part;
+  int gmode, singular, add_e;
   const double* gmean;  // singular operator: mean of g over the unknowns
 };
 
@@ -769,84 +990,87 @@ __device__ __forceinline__ double res_x(const ResArgs<TB>& a,
   return v;
 }
 
+// r = g - K (x + e) in FP64; x + e stored, r rounded to FP32 for the inner
+// solve; partials sum g^2, sum r^2, sum r, sum g
 template <class TB>
 __global__ __launch_bounds__(BS) void k_refine_res(ResArgs<TB> a) {
   __shared__ double sh[BS];
-  __shared__ int flag;
   if (a.outer->odone) return;
   const LevelDev& L = a.L;
   const bool addv = a.add_e && a.inner->x_valid;
   const int cur = a.outer->xcur;
   const double* xin = cur ? a.x64b : a.x64a;
   double* xout = cur ? a.x64a : a.x64b;
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const double gm = a.singular ? *a.gmean : 0.0;
+  int y0, x0;
+  tile_origin<FC0>(y0, x0);
   double gg = 0.0, rr = 0.0, rs = 0.0, gs = 0.0;
-  if (p < (int64_t)L.ny * L.nx) {
-    int y = (int)(p / L.nx), x = (int)(p - (int64_t)y * L.nx);
+  for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
+    int y = y0 + e / FC0, x = x0 + e % FC0;
+    if (y >= L.ny || x >= L.nx) continue;
     int64_t k = li(L, y, x);
     float r32 = 0.f;
     if (L.m[k]) {
-      double g = g64_at(a.b, k, L.stride, a.gmode);
-      if (a.singular) g -= *a.gmean;
+      double g = g64_at(a.b, k, L.stride, a.gmode) - gm;
       double xc = res_x(a, xin, y, x, addv);
-      double w = res_x(a, xin, y, x - 1, addv), e = res_x(a, xin, y, x + 1, addv);
+      double w = res_x(a, xin, y, x - 1, addv), ee = res_x(a, xin, y, x + 1, addv);
       double s = res_x(a, xin, y - 1, x, addv), n = res_x(a, xin, y + 1, x, addv);
-      double r = g - (L.d * xc - L.cx * (w + e) - L.cy * (s + n));
+      double r = g - (L.d * xc - L.cx * (w + ee) - L.cy * (s + n));
       xout[k] = xc;
       r32 = (float)r;
-      gg = g * g;
-      rr = r * r;
-      rs = r;
-      gs = g;
+      gg += g * g;
+      rr += r * r;
+      rs += r;
+      gs += g;
     }
     a.r32[k] = r32;
   }
-  double bg = block_sum<BS>(gg, sh);
-  double br = block_sum<BS>(rr, sh);
-  double bs = block_sum<BS>(rs, sh);
-  double bgs = block_sum<BS>(gs, sh);
-  const int nb = gridDim.x;
+  double b0 = block_sum<BS>(gg, sh);
+  double b1 = block_sum<BS>(rr, sh);
+  double b2 = block_sum<BS>(rs, sh);
+  double b3 = block_sum<BS>(gs, sh);
+  const int nb = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
   if (threadIdx.x == 0) {
-    store_partial(a.partials + blockIdx.x, bg);
-    store_partial(a.partials + nb + blockIdx.x, br);
-    store_partial(a.partials + 2 * nb + blockIdx.x, bs);
-    store_partial(a.partials + 3 * nb + blockIdx.x, bgs);
+    a.part[bid] = b0;
+    a.part[nb + bid] = b1;
+    a.part[2 * nb + bid] = b2;
+    a.part[3 * nb + bid] = b3;
   }
-  if (!last_block_arrive<BS>(a.counter, &flag)) return;
-  double tg = reduce_partials<BS>(a.partials, nb, sh, false);
-  double tr = reduce_partials<BS>(a.partials + nb, nb, sh, false);
-  double ts = reduce_partials<BS>(a.partials + 2 * nb, nb, sh, false);
-  double tgs = reduce_partials<BS>(a.partials + 3 * nb, nb, sh, false);
-  if (threadIdx.x == 0) {
-    Ctl* o = a.outer;
-    Ctl* c = a.inner;
-    Ctl* in = a.inner;
-    (void)in;
-    // singular operator: norms of the zero-mean projections
-    if (a.singular) {
-      tg = tg - tgs * tgs / c->nwet;
-      tr = tr - ts * ts / c->nwet;
-      if (tg < 0.0) tg = 0.0;
-      if (tr < 0.0) tr = 0.0;
-    }
-    o->obb = tg;
-    o->orr = tr;
-    double rel = tg > 0.0 ? sqrt(tr / tg) : 0.0;
-    o->orelres = rel;
-    o->orefine += 1;
-    o->inner_total += c->iters;
-    o->xcur = cur ^ 1;
-    if (tg == 0.0 || rel <= a.rtol || !(rel == rel) || a.last) o->odone = 1;
-    // the inner solve starts from x = 0 with g = r (its initial residual)
-    c->bb = tr;
-    c->rtol = a.inner_rtol;
-    c->mean_r = a.singular ? ts / c->nwet : 0.0;
-    c->done = 0;
-    c->iters = 0;
-    c->x_valid = 0;
-    c->alpha = c->beta = c->mean_z = c->rz = 0.0;
-    c->relres = 1.0;
+}
+
+// one block after each k_refine_res: outer test, buffer flip, reset of the
+// inner control block (its right-hand side is the new residual)
+__global__ __launch_bounds__(BS) void k_refine_check(Ctl* o, Ctl* c,
+                                                     const double* part, int nb,
+                                                     int singular, int last) {
+  __shared__ double sh[BS];
+  if (o->odone) return;
+  double v[4];
+  sum_partials<4>(part, nb, sh, v);
+  if (threadIdx.x != 0) return;
+  double tg = v[0], tr = v[1], ts = v[2], tgs = v[3];
+  if (singular) {  // norms of the zero-mean projections
+    tg = tg - tgs * tgs / c->nwet;
+    tr = tr - ts * ts / c->nwet;
+    if (tg < 0.0) tg = 0.0;
+    if (tr < 0.0) tr = 0.0;
   }
+  o->obb = tg;
+  o->orr = tr;
+  double rel = tg > 0.0 ? sqrt(tr / tg) : 0.0;
+  o->orelres = rel;
+  o->orefine += 1;
+  if (o->orefine > 1) o->inner_total += c->iters;
+  o->xcur ^= 1;
+  bool conv = tg == 0.0 || rel <= o->ortol || !(rel == rel);
+  if (conv || last) o->odone = 1;
+  c->bb = tr;
+  c->rtol = o->inner_rtol;
+  c->mean_r = singular ? ts / c->nwet : 0.0;
+  c->done = (conv || last) ? 1 : 0;
+  c->iters = 0;
+  c->x_valid = 0;
+  c->rz[0] = c->rz[1] = 0.0;
 }
 
 // write the refined solution (minus `shift`, the gauge of the singular
@@ -864,7 +1088,8 @@ __global__ void k_refine_out(LevelDev L, const double* x64a,
   if (L.m[k]) x[k] = TX(x64[k] - sh);
 }
 
-__global__ void k_reset_outer(Ctl* o, double nwet, double rtol) {
+__global__ void k_reset_outer(Ctl* o, double nwet, double rtol,
+                              double inner_rtol) {
   if (threadIdx.x != 0) return;
   o->odone = 0;
   o->orefine = 0;
@@ -872,31 +1097,39 @@ __global__ void k_reset_outer(Ctl* o, double nwet, double rtol) {
   o->obb = o->orr = 0.0;
   o->orelres = 1.0;
   o->nwet = nwet;
-  o->rtol = rtol;
+  o->ortol = rtol;
+  o->inner_rtol = inner_rtol;
 }
 
-// mean of the current FP64 iterate (singular operator)
-__global__ __launch_bounds__(BS) void k_mean64(LevelDev L, const double* x64a,
-                                               const double* x64b,
-                                               const Ctl* outer,
-                                               double* partials,
-                                               unsigned* counter, double nwet,
-                                               double* out) {
+__global__ void k_finish_refine(Ctl* c, const Ctl* o) {
+  if (threadIdx.x != 0) return;
+  double conv = (o->orelres <= o->ortol || o->obb == 0.0) ? 1.0 : 0.0;
+  c->stats[0] = (double)o->inner_total;
+  c->stats[1] = o->orelres;
+  c->stats[2] = conv;
+  c->stats[3] = (double)o->orefine;
+  c->stats[4] += conv > 0.0 ? 0.0 : 1.0;
+  if ((double)o->orefine > c->stats[5]) c->stats[5] = (double)o->orefine;
+  c->stats[6] += 1.0;
+}
+
+// partial sums of the current FP64 iterate (singular gauge)
+__global__ __launch_bounds__(BS) void k_sum64(LevelDev L, const double* x64a,
+                                              const double* x64b,
+                                              const Ctl* outer, double* part) {
   __shared__ double sh[BS];
-  __shared__ int flag;
   const double* x = outer->xcur ? x64b : x64a;
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
+  int y0, x0;
+  tile_origin<FC0>(y0, x0);
   double s = 0.0;
-  if (p < (int64_t)L.ny * L.nx) {
-    int y = (int)(p / L.nx), xx = (int)(p - (int64_t)y * L.nx);
+  for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
+    int y = y0 + e / FC0, xx = x0 + e % FC0;
+    if (y >= L.ny || xx >= L.nx) continue;
     int64_t k = li(L, y, xx);
-    if (L.m[k]) s = x[k];
+    if (L.m[k]) s += x[k];
   }
   double b = block_sum<BS>(s, sh);
-  if (threadIdx.x == 0) store_partial(partials + blockIdx.x, b);
-  if (!last_block_arrive<BS>(counter, &flag)) return;
-  double tot = reduce_partials<BS>(partials, gridDim.x, sh, false);
-  if (threadIdx.x == 0) *out = tot / nwet;
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = b;
 }
 
 template <class TX>
@@ -915,24 +1148,20 @@ __global__ void k_refine_seed(LevelDev L, const TX* x, double* x64) {
 
 // gauge of the singular operator: x -= mean(x) over the unknowns
 template <class T>
-__global__ __launch_bounds__(BS) void k_mean(LevelDev L, const T* x,
-                                             double* partials,
-                                             unsigned* counter, double nwet,
-                                             double* out) {
+__global__ __launch_bounds__(BS) void k_sum(LevelDev L, const T* x,
+                                            double* part) {
   __shared__ double sh[BS];
-  __shared__ int flag;
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
+  int y0, x0;
+  tile_origin<FC0>(y0, x0);
   double s = 0.0;
-  if (p < (int64_t)L.ny * L.nx) {
-    int y = (int)(p / L.nx), xx = (int)(p - (int64_t)y * L.nx);
+  for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
+    int y = y0 + e / FC0, xx = x0 + e % FC0;
+    if (y >= L.ny || xx >= L.nx) continue;
     int64_t k = li(L, y, xx);
-    if (L.m[k]) s = (double)x[k];
+    if (L.m[k]) s += (double)x[k];
   }
   double b = block_sum<BS>(s, sh);
-  if (threadIdx.x == 0) store_partial(partials + blockIdx.x, b);
-  if (!last_block_arrive<BS>(counter, &flag)) return;
-  double tot = reduce_partials<BS>(partials, gridDim.x, sh, false);
-  if (threadIdx.x == 0) *out = tot / nwet;
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = b;
 }
 
 template <class T>
@@ -944,21 +1173,9 @@ __global__ void k_sub_mean(LevelDev L, T* x, const double* mean) {
   if (L.m[k]) x[k] = x[k] - T(*mean);
 }
 
-template <class T>
-__global__ void k_stats(Ctl* c, const Ctl* outer_or_null) {
+__global__ void k_reset_stats(Ctl* c) {
   if (threadIdx.x != 0) return;
-  if (outer_or_null) {
-    const Ctl* o = outer_or_null;
-    c->stats[0] = (double)o->inner_total;
-    c->stats[1] = o->orelres;
-    c->stats[2] = (o->orelres <= c->rtol || o->obb == 0.0) ? 1.0 : 0.0;
-    c->stats[3] = (double)o->orefine;
-  } else {
-    c->stats[0] = (double)c->iters;
-    c->stats[1] = c->relres;
-    c->stats[2] = (c->bb == 0.0 || c->relres <= c->rtol) ? 1.0 : 0.0;
-    c->stats[3] = 0.0;
-  }
+  for (int i = 0; i < 8; i++) c->stats[i] = 0.0;
 }
 
 // fill the halo of the solution (Poisson2D.solve calls mesh.fill)
@@ -985,6 +1202,7 @@ struct HostLevel {
   void* z = nullptr;   // pre-smoothed (all levels)
   void* e = nullptr;   // correction (l >= 1)
   size_t n = 0;        // storage entries
+  int fc = 32;         // tile of the multi-block kernels on this level
 };
 
 template <class T>
@@ -1015,14 +1233,18 @@ struct f2d_poisson {
   f2d::Pcg<float> p32;
   double* x64[2] = {nullptr, nullptr};
   float* e32 = nullptr;
-  double* partials = nullptr;
-  unsigned* counters = nullptr;  // 8 counters
-  double* scal = nullptr;        // device scalars
+  double* part = nullptr;    // partial-sum regions
+  int64_t region = 0;        // doubles per region
+  double* scal = nullptr;    // device scalars
   f2d::Ctl* outer = nullptr;
+  f2d::Ctl* bench_ctl = nullptr;  // scratch control block for kernel timing
   int have_x64 = 0;
   int last_iters = 0;
+  int nu_coarse = 2, nu_tail = 2;  // smoothing sweeps: multi-block coarse
+                                   // levels, single-workgroup tail
   double inner_rtol = 1e-4;
   std::vector<void*> allocs;
+  double* reg(int r) { return part + (int64_t)r * region; }
 };
 
 namespace f2d {
@@ -1035,17 +1257,20 @@ static void* dalloc(f2d_poisson* h, size_t bytes) {
   return p;
 }
 
+static dim3 tiles(const LevelDev& L, int fc) {
+  return dim3((unsigned)((L.nx + fc - 1) / fc), (unsigned)((L.ny + fc - 1) / fc));
+}
+
 template <class T>
 static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
   const int tail_points = h->opts.tail_points > 0 ? h->opts.tail_points : 4096;
-  // level 0 in the padded layout of the caller's arrays
   HostLevel l0;
   l0.d = h->base;
   l0.m = h->m0;
   l0.d.m = h->m0;
   l0.n = (size_t)(h->NYP * h->NXP);
+  l0.fc = FC0;
   P.lv.push_back(l0);
-  // host copy of the level-0 mask to derive the coarse masks
   std::vector<int8_t> mprev((size_t)h->ny * h->nx);
   {
     std::vector<int8_t> full((size_t)(h->NYP * h->NXP));
@@ -1061,7 +1286,8 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
     int cny = ny / 2, cnx = nx / 2;
     std::vector<int8_t> mc((size_t)cny * cnx);
     for (int J = 0; J < cny; J++)
-      for (int I = 0; I < cnx; I++) mc[(size_t)J * cnx + I] = mprev[(size_t)(2 * J) * nx + 2 * I];
+      for (int I = 0; I < cnx; I++)
+        mc[(size_t)J * cnx + I] = mprev[(size_t)(2 * J) * nx + 2 * I];
     cx *= 0.25;
     cy *= 0.25;
     HostLevel c;
@@ -1079,29 +1305,34 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
     c.m = (int8_t*)dalloc(h, c.n);
     F2D_HIP(hipMemcpy(c.m, mc.data(), c.n, hipMemcpyHostToDevice));
     c.d.m = c.m;
+    // small levels: smaller tiles, more blocks in flight
+    c.fc = ((size_t)((cny + 31) / 32) * ((cnx + 31) / 32) < 512) ? 16 : 32;
     P.lv.push_back(c);
     mprev.swap(mc);
     ny = cny;
     nx = cnx;
   }
   P.L = (int)P.lv.size();
-  // tail start: first coarse level small enough for one workgroup's LDS
+  auto tail_bytes = [&](int l) {
+    size_t bytes = 0, nmax = 0;
+    for (int k = l; k < P.L; k++) {
+      size_t n = (size_t)P.lv[k].d.ny * P.lv[k].d.nx;
+      bytes += 2 * ((n * sizeof(T) + 15) / 16) * 16 + ((n + 15) / 16) * 16;
+      if (n > nmax) nmax = n;
+    }
+    size_t tmp = nmax * sizeof(T);
+    if (tmp < 64 * sizeof(double)) tmp = 64 * sizeof(double);
+    return bytes + ((tmp + 15) / 16) * 16;
+  };
   P.Lt = P.L;
   for (int l = 1; l < P.L; l++) {
     size_t pts = (size_t)P.lv[l].d.ny * P.lv[l].d.nx;
     if ((int)pts > tail_points) continue;
-    size_t bytes = 0;
-    for (int k = l; k < P.L; k++) {
-      size_t n = (size_t)P.lv[k].d.ny * P.lv[k].d.nx;
-      bytes += 2 * ((n * sizeof(T) + 15) / 16) * 16 + ((n + 15) / 16) * 16;
-    }
-    if (bytes <= 150 * 1024 && P.L - l <= MAXL) {
+    if (tail_bytes(l) <= 150 * 1024 && P.L - l <= MAXL) {
       P.Lt = l;
-      P.tail_lds = bytes;
       break;
     }
   }
-  // vectors
   for (int l = 0; l < P.L; l++) {
     HostLevel& v = P.lv[l];
     v.z = dalloc(h, v.n * sizeof(T));
@@ -1125,10 +1356,18 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
     TailArgs<T>& ta = P.tail;
     ta.nl = P.L - P.Lt;
     int off = 0;
+    size_t nmax = 0;
     for (int k = 0; k < ta.nl; k++) {
       const HostLevel& v = P.lv[P.Lt + k];
       ta.lv[k] = v.d;
+      int nxl = v.d.nx, lg = -1;
+      if ((nxl & (nxl - 1)) == 0) {
+        lg = 0;
+        while ((1 << lg) < nxl) lg++;
+      }
+      ta.lgx[k] = lg;
       size_t n = (size_t)v.d.ny * v.d.nx;
+      if (n > nmax) nmax = n;
       ta.off_r[k] = off;
       off += (int)(((n * sizeof(T) + 15) / 16) * 16);
       ta.off_x[k] = off;
@@ -1136,60 +1375,109 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
       ta.off_m[k] = off;
       off += (int)(((n + 15) / 16) * 16);
     }
+    size_t tmp = nmax * sizeof(T);
+    if (tmp < 64 * sizeof(double)) tmp = 64 * sizeof(double);
+    ta.off_tmp = off;
+    off += (int)(((tmp + 15) / 16) * 16);
     P.tail_lds = off;
     const LevelDev& cl = ta.lv[ta.nl - 1];
     int cs = h->opts.coarse_sweeps;
-    if (cs <= 0) cs = 2 * (cl.ny > cl.nx ? cl.ny : cl.nx) + 2;
-    ta.coarse_sweeps = cs;
+    if (cs <= 0) {
+      int mx = cl.ny > cl.nx ? cl.ny : cl.nx;
+      cs = mx <= 4 ? 2 : (mx + 1) / 2 + 2;
+    }
+    std::vector<int> npts;
+    for (int k = 0; k < ta.nl; k++) npts.push_back(ta.lv[k].ny * ta.lv[k].nx);
+    std::vector<TailStep> sched = tail_schedule(npts, h->nu_tail, cs, h->singular);
+    F2D_REQUIRE((int)sched.size() <= MAXOPS, "tail schedule too long");
+    TailStep* dsched = (TailStep*)dalloc(h, sched.size() * sizeof(TailStep));
+    F2D_HIP(hipMemcpy(dsched, sched.data(), sched.size() * sizeof(TailStep),
+                      hipMemcpyHostToDevice));
+    ta.steps = dsched;
+    ta.nsteps = (int)sched.size();
     ta.r_in = (const T*)P.lv[P.Lt].r;
     ta.e_out = (T*)P.lv[P.Lt].e;
     ta.ctl = P.ctl;
   }
 }
 
-template <class T, int NU>
+template <class T>
 static void set_tail_attr(size_t bytes) {
   if (bytes > 64 * 1024)
-    F2D_HIP(hipFuncSetAttribute((const void*)k_tail<T, NU>,
+    F2D_HIP(hipFuncSetAttribute((const void*)k_tail<T>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)bytes));
 }
 
+template <class T, int NU, bool CG, bool COARSE>
+static void launch_down_fc(const DownArgs<T>& a, int fc, hipStream_t s) {
+  if (fc == 16)
+    k_down<T, NU, CG, COARSE, 16><<<tiles(a.F, 16), BS, 0, s>>>(a);
+  else
+    k_down<T, NU, CG, COARSE, 32><<<tiles(a.F, 32), BS, 0, s>>>(a);
+}
+
 template <class T, int NU>
-static void launch_down(const DownArgs<T>& a, bool cg, bool coarse, dim3 g,
-                        hipStream_t s) {
+static void launch_down_nu(const DownArgs<T>& a, bool cg, bool coarse, int fc,
+                           hipStream_t s) {
   if (cg) {
-    if (coarse) k_down<T, NU, true, true><<<g, BS, 0, s>>>(a);
-    else k_down<T, NU, true, false><<<g, BS, 0, s>>>(a);
+    if (coarse) k_down<T, NU, true, true, FC0><<<tiles(a.F, FC0), BS, 0, s>>>(a);
+    else k_down<T, NU, true, false, FC0><<<tiles(a.F, FC0), BS, 0, s>>>(a);
   } else {
-    if (coarse) k_down<T, NU, false, true><<<g, BS, 0, s>>>(a);
-    else k_down<T, NU, false, false><<<g, BS, 0, s>>>(a);
+    if (coarse) launch_down_fc<T, NU, false, true>(a, fc, s);
+    else launch_down_fc<T, NU, false, false>(a, fc, s);
   }
+}
+
+template <class T>
+static void launch_down(const DownArgs<T>& a, int nu, bool cg, bool coarse,
+                        int fc, hipStream_t s) {
+  if (nu == 1) launch_down_nu<T, 1>(a, cg, coarse, fc, s);
+  else launch_down_nu<T, 2>(a, cg, coarse, fc, s);
+}
+
+template <class T, int NU, bool FINE, bool COARSE>
+static void launch_up_fc(const UpArgs<T>& a, int fc, hipStream_t s) {
+  if (fc == 16)
+    k_up<T, NU, FINE, COARSE, 16><<<tiles(a.F, 16), BS, 0, s>>>(a);
+  else
+    k_up<T, NU, FINE, COARSE, 32><<<tiles(a.F, 32), BS, 0, s>>>(a);
 }
 
 template <class T, int NU>
-static void launch_up(const UpArgs<T>& a, bool fine, bool coarse, dim3 g,
-                      hipStream_t s) {
+static void launch_up_nu(const UpArgs<T>& a, bool fine, bool coarse, int fc,
+                         hipStream_t s) {
   if (fine) {
-    if (coarse) k_up<T, NU, true, true><<<g, BS, 0, s>>>(a);
-    else k_up<T, NU, true, false><<<g, BS, 0, s>>>(a);
+    if (coarse) k_up<T, NU, true, true, FC0><<<tiles(a.F, FC0), BS, 0, s>>>(a);
+    else k_up<T, NU, true, false, FC0><<<tiles(a.F, FC0), BS, 0, s>>>(a);
   } else {
-    if (coarse) k_up<T, NU, false, true><<<g, BS, 0, s>>>(a);
-    else k_up<T, NU, false, false><<<g, BS, 0, s>>>(a);
+    if (coarse) launch_up_fc<T, NU, false, true>(a, fc, s);
+    else launch_up_fc<T, NU, false, false>(a, fc, s);
   }
 }
 
-static dim3 tiles32(const LevelDev& L) {
-  return dim3((unsigned)((L.nx + 31) / 32), (unsigned)((L.ny + 31) / 32));
+template <class T>
+static void launch_up(const UpArgs<T>& a, int nu, bool fine, bool coarse,
+                      int fc, hipStream_t s) {
+  if (nu == 1) launch_up_nu<T, 1>(a, fine, coarse, fc, s);
+  else launch_up_nu<T, 2>(a, fine, coarse, fc, s);
+}
+
+static int nb0(const LevelDev& F) {
+  dim3 g = tiles(F, FC0);
+  return (int)(g.x * g.y);
 }
 
 // One PCG iteration k (k = 0: preconditioning of the initial residual).
-template <class T, int NU>
+// `last`: only the update and its residual (the iteration cap was reached).
+template <class T>
 static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
                           const Ctl* outer, bool inner, bool last,
                           hipStream_t s) {
+  const int nu = h->opts.nu, nuc = h->nu_coarse;
   const LevelDev& F = P.lv[0].d;
   const bool has_coarse = P.L > 1;
+  const int nb = nb0(F);
   T* r_new = P.r[(k + 1) & 1];
   {
     DownArgs<T> a{};
@@ -1204,15 +1492,16 @@ static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
     a.rc = has_coarse ? (T*)P.lv[1].r : nullptr;
     a.ctl = P.ctl;
     a.outer = outer;
-    a.partials = h->partials;
-    a.counter = h->counters + 0;
+    a.part_in = h->reg(R_PQ);
+    a.part_out = h->reg(R_RR);
+    a.nb_in = nb;
     a.first = (k == 0);
     a.iter = k;
     a.inner = inner;
-    launch_down<T, NU>(a, true, has_coarse, tiles32(F), s);
+    a.singular = h->singular;
+    launch_down<T>(a, nu, true, has_coarse, FC0, s);
   }
   if (last) return;
-  // coarse levels down
   const int Lm = P.Lt;  // multi-block levels 0 .. Lt-1
   for (int l = 1; l < Lm; l++) {
     DownArgs<T> a{};
@@ -1224,12 +1513,12 @@ static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
     a.rc = c ? (T*)P.lv[l + 1].r : nullptr;
     a.ctl = P.ctl;
     a.outer = outer;
-    launch_down<T, NU>(a, false, c, tiles32(a.F), s);
+    launch_down<T>(a, nuc, false, c, P.lv[l].fc, s);
   }
   if (P.Lt < P.L) {
     TailArgs<T> ta = P.tail;
     ta.outer = outer;
-    k_tail<T, NU><<<1, TAIL_BS, P.tail_lds, s>>>(ta);
+    launch_tail<T>(h, ta, P.tail_lds, s);
   }
   for (int l = Lm - 1; l >= 1; l--) {
     UpArgs<T> a{};
@@ -1242,7 +1531,7 @@ static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
     a.out = (T*)P.lv[l].e;
     a.ctl = P.ctl;
     a.outer = outer;
-    launch_up<T, NU>(a, false, c, tiles32(a.F), s);
+    launch_up<T>(a, nuc, false, c, P.lv[l].fc, s);
   }
   {
     UpArgs<T> a{};
@@ -1254,11 +1543,10 @@ static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
     a.out = P.z;
     a.ctl = P.ctl;
     a.outer = outer;
-    a.partials = h->partials;
-    a.counter = h->counters + 1;
-    a.first = (k == 0);
-    a.singular = h->singular;
-    launch_up<T, NU>(a, true, has_coarse, tiles32(F), s);
+    a.part_in = h->reg(R_RR);
+    a.part_out = h->reg(R_RZ);
+    a.nb_in = nb;
+    launch_up<T>(a, nu, true, has_coarse, FC0, s);
   }
   {
     PArgs<T> a{};
@@ -1269,21 +1557,31 @@ static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
     a.q = P.q;
     a.ctl = P.ctl;
     a.outer = outer;
-    a.partials = h->partials;
-    a.counter = h->counters + 2;
+    a.part_in = h->reg(R_RZ);
+    a.part_out = h->reg(R_PQ);
+    a.nb_in = nb;
     a.first = (k == 0);
-    int nb = grid1d((int64_t)F.ny * F.nx, BS);
-    k_pupdate<T><<<nb, BS, 0, s>>>(a);
+    a.iter = k;
+    a.singular = h->singular;
+    k_pupdate<T><<<tiles(F, FC0), BS, 0, s>>>(a);
   }
 }
 
-template <class T, int NU>
-static void pcg_loop(f2d_poisson* h, Pcg<T>& P, T* x, const Ctl* outer,
-                     bool inner, int maxit, bool sync_checks, hipStream_t s) {
-  int* pinned = nullptr;
+// tail workgroup size: opts.tail_threads (256, 512 or 1024; default 1024)
+template <class T>
+static void launch_tail(f2d_poisson* h, const TailArgs<T>& ta, size_t lds,
+                        hipStream_t s) {
+  int tb = h->opts.tail_threads;
+  if (tb != 256 && tb != 512) tb = 1024;
+  k_tail<T><<<1, tb, lds, s>>>(ta);
+}
+
+template <class T>
+static void pcg_loop(f2d_poisson* h, Pcg<T>& P, T* x, int maxit,
+                     bool sync_checks, hipStream_t s) {
   int est = h->last_iters > 0 ? h->last_iters : 1;
   for (int k = 0; k <= maxit; k++) {
-    pcg_iteration<T, NU>(h, P, k, x, outer, inner, k == maxit, s);
+    pcg_iteration<T>(h, P, k, x, nullptr, false, k == maxit, s);
     if (sync_checks && k + 1 >= est && k < maxit) {
       int done = 0;
       F2D_HIP(hipMemcpyAsync(&done, &P.ctl->done, sizeof(int),
@@ -1292,10 +1590,10 @@ static void pcg_loop(f2d_poisson* h, Pcg<T>& P, T* x, const Ctl* outer,
       if (done) break;
     }
   }
-  (void)pinned;
+  k_finish<<<1, BS, 0, s>>>(P.ctl, h->reg(R_RR), nb0(P.lv[0].d));
 }
 
-template <class T, class TB, int NU>
+template <class T, class TB>
 static void solve_plain(f2d_poisson* h, Pcg<T>& P, const TB* b, T* x,
                         int gmode, double rtol, int maxit, bool sync,
                         hipStream_t s) {
@@ -1306,39 +1604,41 @@ static void solve_plain(f2d_poisson* h, Pcg<T>& P, const TB* b, T* x,
   ia.x = x;
   ia.r = P.r[0];
   ia.ctl = P.ctl;
-  ia.partials = h->partials;
-  ia.counter = h->counters + 3;
-  ia.singular = h->singular;
+  ia.part = h->reg(R_INIT);
   ia.rtol = rtol;
-  int nb = grid1d((int64_t)F.ny * F.nx, BS);
+  dim3 g = tiles(F, FC0);
+  const int nb = nb0(F);
   if (gmode == G_FROM_B)
-    k_init<T, TB, G_FROM_B><<<nb, BS, 0, s>>>(ia);
+    k_init<T, TB, G_FROM_B><<<g, BS, 0, s>>>(ia);
   else
-    k_init<T, TB, G_FROM_C2V><<<nb, BS, 0, s>>>(ia);
-  pcg_loop<T, NU>(h, P, x, nullptr, false, maxit, sync, s);
+    k_init<T, TB, G_FROM_C2V><<<g, BS, 0, s>>>(ia);
+  k_init_fin<<<1, BS, 0, s>>>(P.ctl, h->reg(R_INIT), nb, h->singular);
+  pcg_loop<T>(h, P, x, maxit, sync, s);
   if (h->singular) {
-    k_mean<T><<<nb, BS, 0, s>>>(F, x, h->partials, h->counters + 4, h->nwet,
-                                 h->scal);
-    k_sub_mean<T><<<nb, BS, 0, s>>>(F, x, h->scal);
+    k_sum<T><<<g, BS, 0, s>>>(F, x, h->reg(R_MEAN));
+    k_mean_fin<<<1, BS, 0, s>>>(h->reg(R_MEAN), nb, h->nwet, h->scal);
+    k_sub_mean<T><<<grid1d((int64_t)F.ny * F.nx, BS), BS, 0, s>>>(F, x, h->scal);
   }
-  k_stats<T><<<1, 64, 0, s>>>(P.ctl, nullptr);
 }
 
-template <class TB, class TX, int NU>
+template <class TB, class TX>
 static void solve_refine(f2d_poisson* h, const TB* b, TX* x, int gmode,
                          double rtol, int maxit, bool sync, hipStream_t s) {
   Pcg<float>& P = h->p32;
   const LevelDev& F = P.lv[0].d;
-  int nb = grid1d((int64_t)F.ny * F.nx, BS);
+  const int npt = grid1d((int64_t)F.ny * F.nx, BS);
+  dim3 g = tiles(F, FC0);
+  const int nb = nb0(F);
   if (!h->have_x64) {
     // seed the FP64 iterate from the caller's array (buffer 0 is current)
-    k_refine_seed<TX><<<nb, BS, 0, s>>>(F, x, h->x64[0]);
+    k_refine_seed<TX><<<npt, BS, 0, s>>>(F, x, h->x64[0]);
     h->have_x64 = 1;
   }
-  k_reset_outer<<<1, 64, 0, s>>>(h->outer, h->nwet, rtol);
-  if (h->singular)
-    k_gmean<TB><<<nb, BS, 0, s>>>(F, b, gmode, h->partials, h->counters + 6,
-                                  h->nwet, h->scal + 1);
+  k_reset_outer<<<1, 64, 0, s>>>(h->outer, h->nwet, rtol, h->inner_rtol);
+  if (h->singular) {
+    k_gsum<TB><<<g, BS, 0, s>>>(F, b, gmode, h->reg(R_MEAN));
+    k_mean_fin<<<1, BS, 0, s>>>(h->reg(R_MEAN), nb, h->nwet, h->scal + 1);
+  }
   const int inner_it = h->opts.refine_inner > 0 ? h->opts.refine_inner : 4;
   const int nref = maxit > 0 ? maxit : 4;
   for (int it = 0; it <= nref; it++) {
@@ -1351,43 +1651,36 @@ static void solve_refine(f2d_poisson* h, const TB* b, TX* x, int gmode,
     ra.r32 = P.r[0];
     ra.inner = P.ctl;
     ra.outer = h->outer;
-    ra.partials = h->partials;
-    ra.counter = h->counters + 5;
+    ra.part = h->reg(R_REF);
     ra.gmode = gmode;
     ra.singular = h->singular;
     ra.add_e = it > 0;
-    ra.last = it == nref;
-    ra.rtol = rtol;
-    ra.inner_rtol = h->inner_rtol;
     ra.gmean = h->scal + 1;
-    k_refine_res<TB><<<nb, BS, 0, s>>>(ra);
+    k_refine_res<TB><<<g, BS, 0, s>>>(ra);
+    k_refine_check<<<1, BS, 0, s>>>(h->outer, P.ctl, h->reg(R_REF), nb,
+                                    h->singular, it == nref);
     if (it == nref) break;
     // inner FP32 solve of K e = r32 from e = 0 (at most inner_it
     // iterations), predicated off once the outer loop has converged
     for (int k = 0; k <= inner_it; k++)
-      pcg_iteration<float, NU>(h, P, k, h->e32, h->outer, true, k == inner_it,
-                               s);
+      pcg_iteration<float>(h, P, k, h->e32, h->outer, true, k == inner_it, s);
     if (sync && it >= 1) {
       int od = 0;
       F2D_HIP(hipMemcpyAsync(&od, &h->outer->odone, sizeof(int),
                              hipMemcpyDeviceToHost, s));
       F2D_HIP(hipStreamSynchronize(s));
-      if (od) {
-        // one more residual pass is not needed: the last k_refine_res set
-        // odone after writing the current iterate
-        break;
-      }
+      if (od) break;
     }
   }
   const double* shift = nullptr;
   if (h->singular) {
-    k_mean64<<<nb, BS, 0, s>>>(F, h->x64[0], h->x64[1], h->outer, h->partials,
-                               h->counters + 4, h->nwet, h->scal);
+    k_sum64<<<g, BS, 0, s>>>(F, h->x64[0], h->x64[1], h->outer, h->reg(R_MEAN));
+    k_mean_fin<<<1, BS, 0, s>>>(h->reg(R_MEAN), nb, h->nwet, h->scal);
     shift = h->scal;
   }
-  k_refine_out<TX><<<nb, BS, 0, s>>>(F, h->x64[0], h->x64[1], h->outer, shift,
-                                     x);
-  k_stats<float><<<1, 64, 0, s>>>(P.ctl, h->outer);
+  k_refine_out<TX><<<npt, BS, 0, s>>>(F, h->x64[0], h->x64[1], h->outer, shift,
+                                      x);
+  k_finish_refine<<<1, 64, 0, s>>>(P.ctl, h->outer);
 }
 
 template <class T>
@@ -1401,28 +1694,111 @@ static void fill_x(f2d_poisson* h, T* x, hipStream_t s) {
 template <class TB>
 static void do_solve(f2d_poisson* h, const TB* b, void* x, int gmode,
                      double rtol, int maxit, bool sync, hipStream_t s) {
-  const int nu = h->opts.nu;
   const int mode = h->opts.mode;
   if (mode == F2D_POISSON_F64) {
-    if (nu == 1)
-      solve_plain<double, TB, 1>(h, h->p64, b, (double*)x, gmode, rtol, maxit, sync, s);
-    else
-      solve_plain<double, TB, 2>(h, h->p64, b, (double*)x, gmode, rtol, maxit, sync, s);
+    solve_plain<double, TB>(h, h->p64, b, (double*)x, gmode, rtol, maxit, sync, s);
     fill_x<double>(h, (double*)x, s);
   } else if (mode == F2D_POISSON_F32) {
-    if (nu == 1)
-      solve_plain<float, TB, 1>(h, h->p32, b, (float*)x, gmode, rtol, maxit, sync, s);
-    else
-      solve_plain<float, TB, 2>(h, h->p32, b, (float*)x, gmode, rtol, maxit, sync, s);
+    solve_plain<float, TB>(h, h->p32, b, (float*)x, gmode, rtol, maxit, sync, s);
     fill_x<float>(h, (float*)x, s);
   } else {
-    // x has the dtype of b
-    if (nu == 1)
-      solve_refine<TB, TB, 1>(h, b, (TB*)x, gmode, rtol, maxit, sync, s);
-    else
-      solve_refine<TB, TB, 2>(h, b, (TB*)x, gmode, rtol, maxit, sync, s);
+    solve_refine<TB, TB>(h, b, (TB*)x, gmode, rtol, maxit, sync, s);
     fill_x<TB>(h, (TB*)x, s);
   }
+}
+
+// Launch one fine-level kernel of the PCG iteration `reps` times on the
+// handle's current vectors, with a scratch control block and scratch
+// outputs, and return the mean launch time (HIP events on `s`).
+template <class T>
+static double bench_kernel(f2d_poisson* h, Pcg<T>& P, int kind, int reps,
+                           hipStream_t s, double* bytes) {
+  const int nu = h->opts.nu;
+  const LevelDev& F = P.lv[0].d;
+  const bool has_coarse = P.L > 1;
+  const int nb = nb0(F);
+  Ctl c0{};
+  c0.nwet = h->nwet;
+  c0.bb = 1.0;
+  c0.rtol = -1.0;
+  c0.rz[0] = c0.rz[1] = 1.0;
+  F2D_HIP(hipMemcpy(h->bench_ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice));
+  const double pts = (double)F.ny * F.nx;
+  const double t = sizeof(T);
+  hipEvent_t e0, e1;
+  F2D_HIP(hipEventCreate(&e0));
+  F2D_HIP(hipEventCreate(&e1));
+  double* scratch_part = h->reg(R_MEAN);
+  auto launch = [&]() {
+    if (kind == 0) {
+      DownArgs<T> a{};
+      a.F = F;
+      if (has_coarse) a.C = P.lv[1].d;
+      a.r_in = P.r[0];
+      a.r_out = P.r[1];
+      a.q = P.q;
+      a.p = P.p[0];
+      a.x = P.z;  // scratch iterate
+      a.zpre = (T*)P.lv[0].z;
+      a.rc = has_coarse ? (T*)P.lv[1].r : nullptr;
+      a.ctl = h->bench_ctl;
+      a.part_in = h->reg(R_PQ);
+      a.part_out = scratch_part;
+      a.nb_in = nb;
+      a.first = 0;
+      a.iter = 2;
+      a.singular = h->singular;
+      launch_down<T>(a, nu, true, has_coarse, FC0, s);
+    } else if (kind == 1) {
+      UpArgs<T> a{};
+      a.F = F;
+      if (has_coarse) a.C = P.lv[1].d;
+      a.r = P.r[1];
+      a.zpre = (const T*)P.lv[0].z;
+      a.ec = has_coarse ? (const T*)P.lv[1].e : nullptr;
+      a.out = P.q;  // scratch output
+      a.ctl = h->bench_ctl;
+      a.part_in = h->reg(R_RR);
+      a.part_out = scratch_part;
+      a.nb_in = nb;
+      a.bench = 1;
+      launch_up<T>(a, nu, true, has_coarse, FC0, s);
+    } else {
+      PArgs<T> a{};
+      a.L = F;
+      a.z = P.z;
+      a.p_old = P.p[0];
+      a.p = P.p[1];
+      a.q = P.q;
+      a.ctl = h->bench_ctl;
+      a.part_in = h->reg(R_RZ);
+      a.part_out = scratch_part;
+      a.nb_in = nb;
+      a.first = 0;
+      a.iter = 2;
+      a.singular = h->singular;
+      a.bench = 1;
+      k_pupdate<T><<<tiles(F, FC0), BS, 0, s>>>(a);
+    }
+  };
+  // algorithmic bytes: each distinct array read once and written once
+  if (kind == 0)  // r, q, p, x read; r, x, zpre written; rc (1/4); mask
+    *bytes = pts * (7 * t + (has_coarse ? 0.25 * t : 0.0) + 1.0);
+  else if (kind == 1)  // r, zpre, e_c (1/4) read; z written; mask
+    *bytes = pts * (3 * t + (has_coarse ? 0.25 * t : 0.0) + 1.0);
+  else  // z, p_old read; p, q written; mask
+    *bytes = pts * (4 * t + 1.0);
+  launch();
+  F2D_HIP(hipStreamSynchronize(s));
+  F2D_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) launch();
+  F2D_HIP(hipEventRecord(e1, s));
+  F2D_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  F2D_HIP(hipEventElapsedTime(&ms, e0, e1));
+  F2D_HIP(hipEventDestroy(e0));
+  F2D_HIP(hipEventDestroy(e1));
+  return 1e3 * (double)ms / reps;
 }
 
 }  // namespace f2d
@@ -1430,6 +1806,20 @@ static void do_solve(f2d_poisson* h, const TB* b, void* x, int gmode,
 using namespace f2d;
 
 extern "C" {
+
+int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,
+                             double* bytes, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(h && us && bytes, "NULL argument");
+    F2D_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0 (down+CG), 1 (up), 2 (p/q)");
+    F2D_REQUIRE(reps > 0, "reps must be > 0");
+    hipStream_t s = S(stream);
+    if (h->opts.mode == F2D_POISSON_F64)
+      *us = bench_kernel<double>(h, h->p64, kind, reps, s, bytes);
+    else
+      *us = bench_kernel<float>(h, h->p32, kind, reps, s, bytes);
+  });
+}
 
 int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
                        double maindiag, const f2d_poisson_opts* opts,
@@ -1460,6 +1850,10 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
       if (h->opts.tail_points <= 0) h->opts.tail_points = 4096;
     }
     F2D_REQUIRE(h->opts.nu == 1 || h->opts.nu == 2, "opts.nu must be 1 or 2");
+    h->nu_coarse = h->opts.nu_coarse > 0 ? h->opts.nu_coarse : h->opts.nu;
+    h->nu_tail = h->opts.nu_tail > 0 ? h->opts.nu_tail : h->opts.nu;
+    F2D_REQUIRE(h->nu_coarse <= 2 && h->nu_tail <= 2,
+                "coarse/tail sweeps must be 1 or 2");
     F2D_REQUIRE(h->opts.mode >= 0 && h->opts.mode <= 2, "opts.mode unknown");
     // unknowns must lie in the interior block [3, ny+3) x [3, nx+3)
     size_t N = (size_t)(h->NYP * h->NXP);
@@ -1494,23 +1888,19 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
     b.d = 2.0 * (b.cx + b.cy) + maindiag;
     b.inv_d = 1.0 / b.d;
     b.m = h->m0;
-    int nbmax = (int)((h->ny * h->nx + 255) / 256) + 64;
-    h->partials = (double*)dalloc(h, sizeof(double) * 3 * nbmax);
-    h->counters = (unsigned*)dalloc(h, sizeof(unsigned) * 8);
+    // partial-sum regions: REGION_SLOTS values per level-0 tile
+    int64_t ntile = ((h->ny + FC0 - 1) / FC0) * ((h->nx + FC0 - 1) / FC0);
+    h->region = REGION_SLOTS * (ntile + 64);
+    h->part = (double*)dalloc(h, sizeof(double) * R_COUNT * h->region);
     h->scal = (double*)dalloc(h, sizeof(double) * 8);
     h->outer = (Ctl*)dalloc(h, sizeof(Ctl));
+    h->bench_ctl = (Ctl*)dalloc(h, sizeof(Ctl));
     if (h->opts.mode == F2D_POISSON_F64) {
       build_pcg<double>(h, h->p64);
-      if (h->p64.Lt < h->p64.L) {
-        set_tail_attr<double, 1>(h->p64.tail_lds);
-        set_tail_attr<double, 2>(h->p64.tail_lds);
-      }
+      if (h->p64.Lt < h->p64.L) set_tail_attr<double>(h->p64.tail_lds);
     } else {
       build_pcg<float>(h, h->p32);
-      if (h->p32.Lt < h->p32.L) {
-        set_tail_attr<float, 1>(h->p32.tail_lds);
-        set_tail_attr<float, 2>(h->p32.tail_lds);
-      }
+      if (h->p32.Lt < h->p32.L) set_tail_attr<float>(h->p32.tail_lds);
       if (h->opts.mode == F2D_POISSON_F32_REFINE) {
         h->x64[0] = (double*)dalloc(h, N * sizeof(double));
         h->x64[1] = (double*)dalloc(h, N * sizeof(double));
@@ -1553,7 +1943,7 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
       do_solve<float>(h, (const float*)b, x, gmode, rtol, maxit, sync, s);
     if (sync) {
       Ctl* c = mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
-      double st[4];
+      double st[8];
       F2D_HIP(hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, s));
       F2D_HIP(hipStreamSynchronize(s));
       if (iters) *iters = (int)st[0];
@@ -1573,6 +1963,24 @@ int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
                           void* stream) {
   return solve_common(h, omega, x, G_FROM_C2V, rtol, maxit, iters, relres,
                       stream);
+}
+
+int f2d_poisson_reset_stats(f2d_poisson* h, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(h, "handle is NULL");
+    Ctl* c = h->opts.mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
+    k_reset_stats<<<1, 64, 0, S(stream)>>>(c);
+  });
+}
+
+int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(h && stats_host, "NULL argument");
+    Ctl* c = h->opts.mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
+    F2D_HIP(hipMemcpyAsync(stats_host, c->stats, 8 * sizeof(double),
+                           hipMemcpyDeviceToHost, S(stream)));
+    F2D_HIP(hipStreamSynchronize(S(stream)));
+  });
 }
 
 const double* f2d_poisson_stats(f2d_poisson* h) {

@@ -320,25 +320,24 @@ struct PerpArgs {
   int ny, nx;  // interior
   int xper, yper, fill;
   double sx, sy;
-  double* maxabs;    // nullable, [2]
-  double* dt;        // nullable: dt = min(cfl/(mx+my+1e-99), dtmax)
-  double cfl, dtmax;
-  double* partials;  // 2 * nblocks
-  unsigned* counter;
+  double* partials;  // nullable: 2 * nblocks block maxima of |Ux|, |Uy|
 };
 
 __device__ __forceinline__ double nanmax(double a, double b) {
   return (a > b || a != a) ? a : b;
 }
 
+constexpr int PG_PTS = 4;  // points per thread of k_perpgrad
+
 template <class T>
 __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
   __shared__ double sh[256];
-  __shared__ int flag;
   const int NY = a.ny + 6, NX = a.nx + 6;
-  int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = (int64_t)NY * NX;
   double mxv = 0.0, myv = 0.0;
-  if (k < (int64_t)NY * NX) {
+  for (int r = 0; r < PG_PTS; r++) {
+    int64_t k = ((int64_t)blockIdx.x * PG_PTS + r) * 256 + threadIdx.x;
+    if (k >= n) break;
     int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
     int js = j, is = i;
     if (a.fill) {
@@ -346,6 +345,8 @@ __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
       if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
     }
     int64_t ks = (int64_t)js * NX + is;
+    // the last row of Ux / column of Uy is outside numpy's slices: it keeps
+    // its value (0 unless written by the caller), scaled like the rest
     T ux = (js < NY - 1)
                ? (-(a.psi[ks + NX] - a.psi[ks])) * T(a.mx[ks])
                : a.Ux[ks];
@@ -355,64 +356,66 @@ __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
     uy = uy * T(a.sy);
     a.Ux[k] = ux;
     a.Uy[k] = uy;
-    mxv = fabs((double)ux);
-    myv = fabs((double)uy);
+    mxv = nanmax(mxv, fabs((double)ux));
+    myv = nanmax(myv, fabs((double)uy));
   }
-  if (!a.maxabs && !a.dt) return;
-  // block max over the padded arrays, then the last block finalises
-  {
-    const int t = threadIdx.x;
-    sh[t] = mxv;
+  if (!a.partials) return;
+  const int t = threadIdx.x;
+  sh[t] = mxv;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
-      __syncthreads();
+  }
+  double bx = sh[0];
+  __syncthreads();
+  sh[t] = myv;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    a.partials[blockIdx.x] = bx;
+    a.partials[gridDim.x + blockIdx.x] = sh[0];
+  }
+}
+
+// one block: maxima of the partials -> maxabs[2] and the CFL time step
+// dt = min(cfl / (max|Ux| + max|Uy| + 1e-99), dtmax)   (model.py:84-87)
+__global__ __launch_bounds__(256) void k_dt_fin(const double* part, int nb,
+                                                double* maxabs, double* dt,
+                                                double cfl, double dtmax) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double ax = 0.0, ay = 0.0;
+  for (int b = t; b < nb; b += 256) {
+    ax = nanmax(ax, part[b]);
+    ay = nanmax(ay, part[nb + b]);
+  }
+  sh[t] = ax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  ax = sh[0];
+  __syncthreads();
+  sh[t] = ay;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  ay = sh[0];
+  if (t == 0) {
+    if (maxabs) {
+      maxabs[0] = ax;
+      maxabs[1] = ay;
     }
-    double bx = sh[0];
-    __syncthreads();
-    sh[t] = myv;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
-      __syncthreads();
-    }
-    double byv = sh[0];
-    __syncthreads();
-    const int nb = gridDim.x;
-    if (t == 0) {
-      store_partial(a.partials + blockIdx.x, bx);
-      store_partial(a.partials + nb + blockIdx.x, byv);
-    }
-    if (!last_block_arrive<256>(a.counter, &flag)) return;
-    double ax = 0.0, ay = 0.0;
-    for (int b = t; b < nb; b += 256) {
-      ax = nanmax(ax, load_partial(a.partials + b));
-      ay = nanmax(ay, load_partial(a.partials + nb + b));
-    }
-    sh[t] = ax;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
-      __syncthreads();
-    }
-    ax = sh[0];
-    __syncthreads();
-    sh[t] = ay;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
-      __syncthreads();
-    }
-    ay = sh[0];
-    if (t == 0) {
-      if (a.maxabs) {
-        a.maxabs[0] = ax;
-        a.maxabs[1] = ay;
-      }
-      if (a.dt) {
-        double v = a.cfl / ((ax + ay) + 1e-99);
-        a.dt[0] = (a.dtmax < v) ? a.dtmax : v;  // Python min(v, dtmax)
-      }
+    if (dt) {
+      double v = cfl / ((ax + ay) + 1e-99);
+      dt[0] = (dtmax < v) ? dtmax : v;  // Python min(v, dtmax)
     }
   }
 }
@@ -433,14 +436,10 @@ __global__ void k_fill(T* __restrict__ a, int ny, int nx, int xper, int yper) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_cfl(const T* __restrict__ Ux,
-                                             const T* __restrict__ Uy,
-                                             int64_t n, double cfl,
-                                             double dtmax, double* dt,
-                                             double* partials,
-                                             unsigned* counter) {
+__global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ Ux,
+                                                const T* __restrict__ Uy,
+                                                int64_t n, double* part) {
   __shared__ double sh[256];
-  __shared__ int flag;
   const int t = threadIdx.x;
   double mx = 0.0, my = 0.0;
   for (int64_t k = (int64_t)blockIdx.x * 256 + t; k < n;
@@ -462,52 +461,39 @@ __global__ __launch_bounds__(256) void k_cfl(const T* __restrict__ Ux,
     if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
     __syncthreads();
   }
-  my = sh[0];
-  __syncthreads();
-  const int nb = gridDim.x;
   if (t == 0) {
-    store_partial(partials + blockIdx.x, mx);
-    store_partial(partials + nb + blockIdx.x, my);
-  }
-  if (!last_block_arrive<256>(counter, &flag)) return;
-  if (t == 0) {
-    double ax = 0.0, ay = 0.0;
-    for (int b = 0; b < nb; b++) {
-      ax = nanmax(ax, load_partial(partials + b));
-      ay = nanmax(ay, load_partial(partials + nb + b));
-    }
-    double v = cfl / ((ax + ay) + 1e-99);
-    dt[0] = (dtmax < v) ? dtmax : v;
+    part[blockIdx.x] = mx;
+    part[gridDim.x + blockIdx.x] = sh[0];
   }
 }
 
 __global__ void k_set_scalar(double* d, double v) { *d = v; }
 
-// Scratch for the grid reductions of this file (per device, set-up once).
+// Per-device scratch of the grid reductions of this file: partial maxima
+// written by one kernel and read by the one-block finaliser that follows it
+// on the same stream (calls on different streams of one device must not
+// overlap).
 struct ReduceScratch {
   double* partials = nullptr;
-  unsigned* counter = nullptr;
   int cap = 0;
 };
 static ReduceScratch g_red[64];
 
-static ReduceScratch& red_scratch(int nblocks) {
+static double* red_scratch(int nblocks) {
   int dev = 0;
   F2D_HIP(hipGetDevice(&dev));
   F2D_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
   ReduceScratch& r = g_red[dev];
   if (r.cap < nblocks) {
-    F2D_REQUIRE(r.cap == 0 || r.partials == nullptr || true, "");
-    if (r.partials) F2D_HIP(hipFree(r.partials));
-    int cap = nblocks < 4096 ? 4096 : nblocks;
-    F2D_HIP(hipMalloc((void**)&r.partials, sizeof(double) * 2 * cap));
-    if (!r.counter) {
-      F2D_HIP(hipMalloc((void**)&r.counter, sizeof(unsigned) * 4));
-      F2D_HIP(hipMemset(r.counter, 0, sizeof(unsigned) * 4));
+    if (r.partials) {
+      F2D_HIP(hipDeviceSynchronize());
+      F2D_HIP(hipFree(r.partials));
     }
+    int cap = nblocks < 8192 ? 8192 : nblocks;
+    F2D_HIP(hipMalloc((void**)&r.partials, sizeof(double) * 2 * cap));
     r.cap = cap;
   }
-  return r;
+  return r.partials;
 }
 
 }  // namespace f2d
@@ -660,8 +646,9 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
                     double* dt, double cfl, double dtmax, void* stream) {
   return guarded([&] {
     check_grid(g);
-    int NY = (int)g->ny + 6, NX = (int)g->nx + 6;
-    int nb = grid1d((int64_t)NY * NX, 256);
+    int64_t n = (g->ny + 6) * (g->nx + 6);
+    int nb = grid1d(n, 256 * PG_PTS);
+    double* part = (maxabs || dt) ? red_scratch(nb) : nullptr;
     F2D_DTYPE_DISPATCH(dtype, {
       PerpArgs<T> a{};
       a.psi = (const T*)psi;
@@ -676,17 +663,11 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
       a.fill = fill;
       a.sx = sx;
       a.sy = sy;
-      a.maxabs = maxabs;
-      a.dt = dt;
-      a.cfl = cfl;
-      a.dtmax = dtmax;
-      if (maxabs || dt) {
-        ReduceScratch& r = red_scratch(nb);
-        a.partials = r.partials;
-        a.counter = r.counter;
-      }
+      a.partials = part;
       k_perpgrad<T><<<nb, 256, 0, S(stream)>>>(a);
     });
+    if (part)
+      k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, maxabs, dt, cfl, dtmax);
   });
 }
 
@@ -718,12 +699,13 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
                double cfl, double dtmax, double* dt, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(n > 0, "f2d_cfl_dt: empty");
+    F2D_REQUIRE(dt, "dt is NULL");
     int nb = grid1d(n, 256);
     if (nb > 1024) nb = 1024;
-    ReduceScratch& r = red_scratch(nb);
-    F2D_DTYPE_DISPATCH(dtype, k_cfl<T><<<nb, 256, 0, S(stream)>>>(
-                                  (const T*)Ux, (const T*)Uy, n, cfl, dtmax,
-                                  dt, r.partials, r.counter));
+    double* part = red_scratch(nb);
+    F2D_DTYPE_DISPATCH(dtype, k_absmax<T><<<nb, 256, 0, S(stream)>>>(
+                                  (const T*)Ux, (const T*)Uy, n, part));
+    k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, nullptr, dt, cfl, dtmax);
   });
 }
 

@@ -48,6 +48,10 @@ class Poisson2D:
         self._h = h
         self.last_iters = 0
         self.last_relres = 0.0
+        # iterations (refinements in FP32) the no-sync solves enqueue:
+        # learned from the solves checked on the host, plus a margin
+        self.est = None
+        self.margin = 2
 
     def _gindex(self):
         """get_Gindex (elliptic.py:198-203), computed on the device."""
@@ -96,10 +100,21 @@ class Poisson2D:
             x[...] = dx.double().cpu().numpy()
         return None
 
+    def _learn(self, it, refinements=None):
+        n = refinements if self.precision != "fp64" else it
+        if n is not None:
+            self.est = n if self.est is None else max(self.est, n)
+
     def solve_c2v(self, omega, x, stats=True):
         """Fused eulerpsi/qg diagnostic: b = centerstovertices(omega) formed on
-        the fly, then solve (equations.py:80-82, operators.py:186-191)."""
-        if stats:
+        the fly, then solve (equations.py:80-82, operators.py:186-191).
+
+        stats=True: the solve checks convergence on the host and raises if it
+        failed.  stats=False: no host round trip; the solve enqueues the
+        learned iteration count plus a margin (iterations past convergence
+        return at once) and leaves its outcome in the device counters that
+        ``read_stats()`` returns."""
+        if stats or self.est is None:
             it, rr = ctypes.c_int(), ctypes.c_double()
             _lib.call("f2d_poisson_solve_c2v", self._h, _lib.ptr(omega), _lib.ptr(x),
                       self.rtol, self.maxit, ctypes.byref(it), ctypes.byref(rr),
@@ -109,9 +124,39 @@ class Poisson2D:
                 raise PoissonNotConverged(
                     f"Poisson solve reached relres {rr.value:.3e} > {self.rtol:.1e} "
                     f"after {it.value} iterations")
+            ref = None
+            if self.precision != "fp64":
+                ref = int(self._stats_host()[3])
+            self._learn(it.value, ref)
         else:
+            n = min(self.est + self.margin, self.maxit)
             _lib.call("f2d_poisson_solve_c2v", self._h, _lib.ptr(omega), _lib.ptr(x),
-                      self.rtol, self.maxit, None, None, _lib.stream_ptr())
+                      self.rtol, n, None, None, _lib.stream_ptr())
+
+    def _stats_host(self):
+        buf = (ctypes.c_double * 8)()
+        _lib.call("f2d_poisson_read_stats", self._h, buf, _lib.stream_ptr())
+        return list(buf)
+
+    def reset_stats(self):
+        _lib.call("f2d_poisson_reset_stats", self._h, _lib.stream_ptr())
+
+    def read_stats(self):
+        """Counters since the last reset: solves, non-converged solves, max
+        iterations (refinements in FP32), and the last solve's figures."""
+        st = self._stats_host()
+        return {"solves": int(st[6]), "failed": int(st[4]), "max_iters": int(st[5]),
+                "last_iters": int(st[0]), "last_relres": st[1],
+                "last_refinements": int(st[3])}
+
+    def bench_kernel(self, kind, reps=20):
+        """Mean launch time (us) and algorithmic bytes of one fine-level
+        kernel (0: update/pre-smooth/restrict, 1: prolongate/post-smooth,
+        2: search direction/operator) on the current vectors."""
+        us, by = ctypes.c_double(), ctypes.c_double()
+        _lib.call("f2d_poisson_bench_kernel", self._h, kind, reps, ctypes.byref(us),
+                  ctypes.byref(by), _lib.stream_ptr())
+        return us.value, by.value
 
 
 def get_msk_from_mesh(mesh, location):

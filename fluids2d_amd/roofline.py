@@ -1,0 +1,70 @@
+"""Live roofline figures of the hot loop's kernels (HIP events on the stream
+the kernels run on).
+
+Algorithmic bytes per launch count every array a kernel reads once and every
+array it writes once (int8 masks at 1 B), over the grid points one launch
+covers; achieved bandwidth = those bytes / mean launch time.
+"""
+import torch
+
+from . import operators as op
+
+KINDS = {0: "k_down<CG> (CG update + pre-smooth + residual + restrict, fine level)",
+         1: "k_up<FINE> (prolongate + post-smooth + (r,z), fine level)",
+         2: "k_pupdate (p = z + beta p, q = K p, (p,q))"}
+
+
+def transport_timing(model, reps=20):
+    """Mean launch time and algorithmic bytes of the fused transport stage
+    (stage 2 of RK3: reads q, Ux, Uy, ds1, two orders + mask; writes q, ds2)."""
+    integ = model.integrator
+    s, alt = integ.dstate, integ.dalt
+    m = model.mesh
+    name = integ.model.prognostic[0]
+    q, qo = getattr(s, name), getattr(alt, name)
+    ds0, ds1 = integ.dscratch[0][0], integ.dscratch[1][0]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    args = (m, model.param.compflux, q, qo, s.U, [ds0], ds1, model.param.integrator, 1,
+            integ.d_dt)
+    op.transport_stage(*args)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        op.transport_stage(*args)
+    e1.record()
+    torch.cuda.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / reps
+    t = q.element_size()
+    npts = q.numel()
+    byts = npts * (6 * t + 3)
+    return us, byts
+
+
+def profile_dominant(model, reps=20):
+    """Time the fine-level multigrid-CG kernels and the transport stage; the
+    dominant one is the largest per-step total (mean launch time x launches
+    per step, the launches counted from the last solves)."""
+    pois = model.mesh.poisson_vertices
+    st = pois.read_stats()
+    prec = model.param.precision
+    if prec == "fp64":
+        iters = max(st["max_iters"], 1)
+        per_solve = {0: iters + 1, 1: iters, 2: iters}
+    else:
+        nref = max(st["max_iters"], 1)
+        inner = model.mesh.poisson_vertices  # inner iterations per refinement
+        k = 4
+        per_solve = {0: nref * (k + 1), 1: nref * k, 2: nref * k}
+    rows = {}
+    for kind in (0, 1, 2):
+        us, by = pois.bench_kernel(kind, reps)
+        rows[KINDS[kind]] = dict(us=us, bytes=by, per_step=3 * per_solve[kind])
+    us, by = transport_timing(model, reps)
+    rows["k_transport (fused WENO5 flux + div + RK update)"] = dict(us=us, bytes=by,
+                                                                     per_step=3)
+    name = max(rows, key=lambda k: rows[k]["us"] * rows[k]["per_step"])
+    r = rows[name]
+    return {"kernel": name, "us": r["us"], "bytes": r["bytes"],
+            "achieved_gbs": r["bytes"] / (r["us"] * 1e-6) / 1e9,
+            "all": {k: dict(v, gbs=v["bytes"] / (v["us"] * 1e-6) / 1e9)
+                    for k, v in rows.items()}}

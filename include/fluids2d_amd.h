@@ -167,7 +167,9 @@ typedef struct f2d_poisson_opts {
   int32_t coarse_sweeps; /* symmetric sweeps on the coarsest level (0=auto) */
   int32_t refine_inner;  /* F32_REFINE: FP32 PCG iterations per refinement */
   int32_t tail_points;   /* max points of the single-workgroup tail levels */
-  int32_t reserved[3];
+  int32_t tail_threads;  /* workgroup size of the tail kernel (256/512/1024) */
+  int32_t nu_coarse;     /* sweeps on the multi-block coarse levels (0: nu) */
+  int32_t nu_tail;       /* sweeps on the tail levels (0: nu)              */
 } f2d_poisson_opts;
 
 /* unknowns_host: int8, padded shape, 1 at unknowns (elliptic.py:102-111:
@@ -188,9 +190,27 @@ int f2d_poisson_solve(f2d_poisson* h, const void* b, void* x, double rtol,
 int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
                           double rtol, int maxit, int* iters, double* relres,
                           void* stream);
-/* Device-side solve statistics of the last solve: stats_dev[0] iterations,
- * [1] relative residual, [2] converged flag (float64 device array). */
+/* Device-side solve statistics (float64 device array of 8): of the last
+ * solve [0] iterations (F32_REFINE: inner iterations summed), [1] relative
+ * residual, [2] converged flag, [3] refinements; since the last reset
+ * [4] number of solves that did not converge, [5] maximum iterations
+ * (F32_REFINE: refinements), [6] number of solves.  Without iters/relres a
+ * solve enqueues exactly `maxit` iterations (refinements), those after
+ * convergence returning at once, and never synchronises: these counters are
+ * how a caller checks such solves afterwards. */
 const double* f2d_poisson_stats(f2d_poisson* h);
+int f2d_poisson_reset_stats(f2d_poisson* h, void* stream);
+/* Copy the 8 statistics above to host memory (synchronises `stream`). */
+int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream);
+/* Measurement hook (no counterpart in the reference): launch one fine-level
+ * kernel of the CG iteration `reps` times on the handle's current vectors --
+ * kind 0 the update/pre-smooth/restrict pass, 1 the prolongate/post-smooth
+ * pass, 2 the search-direction/operator pass -- with a scratch control block
+ * and scratch outputs (the caller's solution is not touched).  *us receives
+ * the mean launch time from HIP events on `stream`, *bytes the algorithmic
+ * bytes of one launch (each array read once, written once).  Synchronises. */
+int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,
+                             double* bytes, void* stream);
 
 #ifdef __cplusplus
 }
