@@ -637,21 +637,39 @@ __global__ __launch_bounds__(BS) void k_finish(Ctl* c, const double* part_rr,
 constexpr int MAXL = 16;
 constexpr int WAVE_PTS = 256;
 constexpr int MAXOPS = 512;
+// coarsening stops at a level of at most DENSE_PTS points; that level is
+// solved exactly with its dense (pseudo-)inverse
+constexpr int DENSE_PTS = 64;
+constexpr int DENSE_COLS = DENSE_PTS / 4;  // columns per wave, >= 4 waves
 
 enum TailOp { OP_SWEEP = 0, OP_RES = 1, OP_RESTRICT = 2, OP_PROLONG = 3,
-              OP_MEAN_R = 4, OP_MEAN_X = 5 };
+              OP_MEAN_R = 4, OP_MEAN_X = 5, OP_DENSE = 6 };
 
 struct TailStep {
   int8_t op, level, col, fromzero, wave;
   int8_t pad[3];
 };
 
+// One tail level in LDS: ny x nx points stored inside a one-point ring,
+// row pitch P = nx + 2 (point (y, x) at (y + 1) * P + x + 1).  The ring holds
+// the periodic images of the edge points, or zeros in closed directions, so
+// every stencil reads its neighbours at fixed offsets.
+struct TailLev {
+  int ny, nx, P, lg;         // lg = log2(nx) when nx is a power of two, else -1
+  int xper, yper;
+  int off_r, off_x, off_m;   // byte offsets of r, x (T) and the mask (int8)
+  int stride;                // the level's mask in device memory:
+  int64_t origin;            //   gm[origin + y * stride + x]
+  double cx, cy, d, inv_d;
+  const int8_t* gm;
+};
+
 template <class T>
 struct TailArgs {
-  LevelDev lv[MAXL];
-  int off_r[MAXL], off_x[MAXL], off_m[MAXL];
-  int lgx[MAXL];  // log2(nx) when nx is a power of two, else -1
-  int off_tmp;    // residual scratch of the largest tail level
+  TailLev lv[MAXL];          // host copy of the level descriptors
+  const TailLev* lvs;        // the same, in device memory
+  LevelDev top;    // the first tail level in device memory
+  int off_tmp;     // residual scratch (largest level, >= 1024 entries)
   int nl;
   int nsteps;
   const TailStep* steps;  // device schedule
@@ -659,6 +677,11 @@ struct TailArgs {
   T* e_out;
   Ctl* ctl;
   const Ctl* outer;
+  // coarsest level: x = M r over its nw unknowns (M symmetric, nw x nw,
+  // unknown j at point dense_idx[j] = y * nx + x of the level), or nw = 0
+  const T* dense;
+  const int* dense_idx;
+  int dense_n;
 };
 
 // point j of a level in row-major order -> (y, x)
@@ -672,28 +695,54 @@ __device__ __forceinline__ void yx_of(int j, int nx, int lg, int& y, int& x) {
   }
 }
 
-// Neighbour (y+dy, x+dx), |dy|,|dx| <= 1, of an LDS level array: wrapped in
-// periodic directions, zero outside closed ones, evaluated without branches
-// (the index is clamped, the value selected).  Level arrays vanish at dry
-// points by construction, so no mask test is needed.
-template <class T>
-__device__ __forceinline__ T nb(const T* a, const LevelDev& L, int y, int x,
-                                int dy, int dx) {
-  int yy = y + dy, xx = x + dx;
-  const bool xo = xx < 0 || xx >= L.nx, yo = yy < 0 || yy >= L.ny;
-  xx = xx < 0 ? xx + L.nx : (xx >= L.nx ? xx - L.nx : xx);
-  yy = yy < 0 ? yy + L.ny : (yy >= L.ny ? yy - L.ny : yy);
-  const T v = a[yy * L.nx + xx];
-  const bool ok = (!xo || L.xper) && (!yo || L.yper);
-  return ok ? v : T(0);
+// Store v at point (y, x), index k, of a level array together with its ring
+// images: the periodic copy, or a zero on the closed side (the residual
+// scratch is shared by all levels, so its ring must not keep a larger
+// level's values).  Levels have nx, ny >= 2.
+struct RingGeo {  // a level's ring offsets, held in registers
+  int nx1, ny1;     // nx - 1, ny - 1
+  int ex0, ex1;     // offset of the image of column 0 / column nx - 1
+  int ey0, ey1;     // offset of the image of row 0 / row ny - 1
+  bool xper, yper;
+};
+
+__device__ __forceinline__ RingGeo ring_geo(const TailLev& L) {
+  RingGeo g;
+  g.nx1 = L.nx - 1;
+  g.ny1 = L.ny - 1;
+  g.xper = L.xper != 0;
+  g.yper = L.yper != 0;
+  g.ex0 = g.xper ? L.nx : -1;
+  g.ex1 = g.xper ? -L.nx : 1;
+  g.ey0 = g.yper ? L.ny * L.P : -L.P;
+  g.ey1 = g.yper ? -L.ny * L.P : L.P;
+  return g;
 }
 
 template <class T>
-__device__ __forceinline__ void nbsum(const T* a, const LevelDev& L, int y,
-                                      int x, int k, T& sx, T& sy) {
-  sx = nb(a, L, y, x, 0, -1) + nb(a, L, y, x, 0, 1);
-  sy = nb(a, L, y, x, -1, 0) + nb(a, L, y, x, 1, 0);
-  (void)k;
+__device__ __forceinline__ void put(T* a, const RingGeo& g, int k, int y,
+                                    int x, T v) {
+  a[k] = v;
+  const int ex = x == 0 ? g.ex0 : (x == g.nx1 ? g.ex1 : 0);
+  const int ey = y == 0 ? g.ey0 : (y == g.ny1 ? g.ey1 : 0);
+  if (ex | ey) {
+    const T vx = g.xper ? v : T(0), vy = g.yper ? v : T(0);
+    const T vc = (g.xper && g.yper) ? v : T(0);
+    if (ex) a[k + ex] = vx;
+    if (ey) a[k + ey] = vy;
+    if (ex && ey) a[k + ex + ey] = vc;
+  }
+}
+
+__device__ __forceinline__ double lane_bcast(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, j);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ float lane_bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -702,46 +751,196 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// ---- tail operations, specialised on LG = log2(nx) of the level (1..6;
+// 0: any nx, run-time geometry) so that the row pitch and the neighbour
+// offsets are compile-time constants on power-of-two levels ----
+
+template <int LG>
+__device__ __forceinline__ void pt_of(int j, int nx, int& y, int& x) {
+  if (LG > 0) {
+    y = j >> LG;
+    x = j & ((1 << LG) - 1);
+  } else {
+    y = j / nx;
+    x = j - y * nx;
+  }
+}
+
+template <class T, int LG>
+__device__ __forceinline__ void t_sweep(const TailLev& L, const RingGeo& G,
+                                        unsigned char* smem, int col,
+                                        bool fromzero, int tid, int NT) {
+  const int nx = LG > 0 ? (1 << LG) : L.nx;
+  const int P = nx + 2;
+  const int8_t* m = reinterpret_cast<const int8_t*>(smem + L.off_m);
+  const T* r = reinterpret_cast<const T*>(smem + L.off_r);
+  T* x = reinterpret_cast<T*>(smem + L.off_x);
+  const T cx = T(L.cx), cy = T(L.cy), id = T(L.inv_d);
+  const bool even = !(nx & 1);
+  const int hx = nx >> 1;
+  const int cnt = even ? L.ny * hx : L.ny * nx;
+  for (int j = tid; j < cnt; j += NT) {
+    int y, xx;
+    if (even) {
+      int i2;
+      if (LG > 0) {
+        y = j >> (LG - 1);
+        i2 = j & (hx - 1);
+      } else {
+        y = j / hx;
+        i2 = j - y * hx;
+      }
+      xx = 2 * i2 + ((col ^ y) & 1);
+    } else {
+      y = j / nx;
+      xx = j - y * nx;
+      if (((y + xx) & 1) != col) continue;
+    }
+    const int k = (y + 1) * P + xx + 1;
+    const bool wet = m[k] != 0;
+    T v;
+    if (fromzero) {
+      v = r[k] * id;
+    } else {
+      const T sx = x[k - 1] + x[k + 1];
+      const T sy = x[k - P] + x[k + P];
+      v = (r[k] + cx * sx + cy * sy) * id;
+    }
+    if (wet) put(x, G, k, y, xx, v);
+  }
+}
+
+template <class T, int LG>
+__device__ __forceinline__ void t_res(const TailLev& L, const RingGeo& G,
+                                      unsigned char* smem, T* tmp, int tid,
+                                      int NT) {
+  const int nx = LG > 0 ? (1 << LG) : L.nx;
+  const int P = nx + 2;
+  const int8_t* m = reinterpret_cast<const int8_t*>(smem + L.off_m);
+  const T* r = reinterpret_cast<const T*>(smem + L.off_r);
+  const T* x = reinterpret_cast<const T*>(smem + L.off_x);
+  const T cx = T(L.cx), cy = T(L.cy), dd = T(L.d);
+  const int n = L.ny * nx;
+  for (int j = tid; j < n; j += NT) {
+    int y, xx;
+    pt_of<LG>(j, nx, y, xx);
+    const int k = (y + 1) * P + xx + 1;
+    const bool wet = m[k] != 0;
+    const T sx = x[k - 1] + x[k + 1];
+    const T sy = x[k - P] + x[k + P];
+    const T res = r[k] - (dd * x[k] - cx * sx - cy * sy);
+    put(tmp, G, k, y, xx, wet ? res : T(0));
+  }
+}
+
+// full weighting of tmp (fine level L) onto the rhs of level C = L / 2
+template <class T, int LG>
+__device__ __forceinline__ void t_restrict(const TailLev& L, const TailLev& C,
+                                           unsigned char* smem, const T* tmp,
+                                           int tid, int NT) {
+  const int nx = LG > 0 ? (1 << LG) : L.nx;
+  const int P = nx + 2;
+  const int cnx = LG > 0 ? (1 << (LG - 1)) : C.nx;
+  const int CP = cnx + 2;
+  const int8_t* mc = reinterpret_cast<const int8_t*>(smem + C.off_m);
+  T* rc = reinterpret_cast<T*>(smem + C.off_r);
+  const int nc = C.ny * cnx;
+  for (int j = tid; j < nc; j += NT) {
+    int J, I;
+    pt_of<(LG > 0 ? LG - 1 : 0)>(j, cnx, J, I);
+    const int kc = (J + 1) * CP + I + 1;
+    const int kf = (2 * J + 1) * P + 2 * I + 1;
+    const bool wet = mc[kc] != 0;
+    const T v = tmp[kf] +
+                T(0.5) * ((tmp[kf - 1] + tmp[kf + 1]) + (tmp[kf - P] + tmp[kf + P])) +
+                T(0.25) * ((tmp[kf - P - 1] + tmp[kf - P + 1]) +
+                           (tmp[kf + P - 1] + tmp[kf + P + 1]));
+    rc[kc] = wet ? T(0.25) * v : T(0);
+  }
+}
+
+// bilinear interpolation of level C = L / 2, added to x of level L
+template <class T, int LG>
+__device__ __forceinline__ void t_prolong(const TailLev& L, const TailLev& C,
+                                          const RingGeo& G, unsigned char* smem,
+                                          int tid, int NT) {
+  const int nx = LG > 0 ? (1 << LG) : L.nx;
+  const int P = nx + 2;
+  const int cnx = LG > 0 ? (1 << (LG - 1)) : C.nx;
+  const int CP = cnx + 2;
+  const int8_t* m = reinterpret_cast<const int8_t*>(smem + L.off_m);
+  T* x = reinterpret_cast<T*>(smem + L.off_x);
+  const T* xc = reinterpret_cast<const T*>(smem + C.off_x);
+  const int n = L.ny * nx;
+  for (int j = tid; j < n; j += NT) {
+    int y, xx;
+    pt_of<LG>(j, nx, y, xx);
+    const int k = (y + 1) * P + xx + 1;
+    const bool wet = m[k] != 0;
+    const int kc = ((y >> 1) + 1) * CP + (xx >> 1) + 1;
+    const bool oy = y & 1, ox = xx & 1;
+    const T c00 = xc[kc], c01 = xc[kc + 1];
+    const T c10 = xc[kc + CP], c11 = xc[kc + CP + 1];
+    // along x on rows J and J + 1, then along y (the halvings are exact,
+    // so odd-odd points get ((c00 + c01) + (c10 + c11)) / 4)
+    const T a0 = ox ? T(0.5) * (c00 + c01) : c00;
+    const T a1 = ox ? T(0.5) * (c10 + c11) : c10;
+    const T v = oy ? T(0.5) * (a0 + a1) : a0;
+    const T xv = x[k];
+    if (wet) put(x, G, k, y, xx, xv + v);
+  }
+}
+
+#define F2D_TAIL_LG(lg, CALL) \
+  switch (lg) {               \
+    case 6: CALL(6); break;   \
+    case 5: CALL(5); break;   \
+    case 4: CALL(4); break;   \
+    case 3: CALL(3); break;   \
+    case 2: CALL(2); break;   \
+    case 1: CALL(1); break;   \
+    default: CALL(0); break;  \
+  }
+
 template <class T>
 __global__ __launch_bounds__(1024) void k_tail(TailArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  // Level descriptors, offsets and the schedule are indexed at run time:
-  // staged once in LDS, so the per-step accesses do not go back to the
-  // kernel-argument segment.
-  __shared__ LevelDev s_lv[MAXL];
-  __shared__ int s_off[4][MAXL];
-  __shared__ TailStep s_st[MAXOPS];
   if (stop(a.ctl, a.outer)) return;
+  // level descriptors and the schedule are read-only device arrays: their
+  // uniform reads go through the scalar cache into scalar registers
+  const TailLev* __restrict__ lvs = a.lvs;
+  const TailStep* __restrict__ steps = a.steps;
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int wave = tid >> 6;
-  for (int i = tid; i < a.nl; i += nth) {
-    s_lv[i] = a.lv[i];
-    s_off[0][i] = a.off_r[i];
-    s_off[1][i] = a.off_x[i];
-    s_off[2][i] = a.off_m[i];
-    s_off[3][i] = a.lgx[i];
-  }
-  for (int i = tid; i < a.nsteps; i += nth) s_st[i] = a.steps[i];
-  __syncthreads();
-  T* tmp = reinterpret_cast<T*>(smem + a.off_tmp);
-  const int nl = a.nl, nsteps = a.nsteps;
-  for (int l = 0; l < nl; l++) {
-    const LevelDev L = s_lv[l];
-    const int n = L.ny * L.nx;
-    int8_t* m = reinterpret_cast<int8_t*>(smem + s_off[2][l]);
-    T* r = reinterpret_cast<T*>(smem + s_off[0][l]);
-    T* x = reinterpret_cast<T*>(smem + s_off[1][l]);
-    const int lg = s_off[3][l];
-    for (int k = tid; k < n; k += nth) {
-      int y, xx;
-      yx_of(k, L.nx, lg, y, xx);
-      int64_t g = li(L, y, xx);
-      int8_t mv = L.m[g];
-      m[k] = mv;
+  const int wave = tid >> 6, lane = tid & 63;
+  // zero every level (ring included), then fill masks and the input
+  for (int l = 0; l < a.nl; l++) {
+    const TailLev& L = lvs[l];
+    const int np = (L.ny + 2) * L.P;
+    int8_t* m = reinterpret_cast<int8_t*>(smem + L.off_m);
+    T* x = reinterpret_cast<T*>(smem + L.off_x);
+    for (int k = tid; k < np; k += nth) {
+      m[k] = 0;
       x[k] = T(0);
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < a.nl; l++) {
+    const TailLev& L = lvs[l];
+    const int n = L.ny * L.nx;
+    int8_t* m = reinterpret_cast<int8_t*>(smem + L.off_m);
+    T* r = reinterpret_cast<T*>(smem + L.off_r);
+    for (int j = tid; j < n; j += nth) {
+      int y, xx;
+      yx_of(j, L.nx, L.lg, y, xx);
+      const int k = (y + 1) * L.P + xx + 1;
+      const int64_t g = L.origin + (int64_t)y * L.stride + xx;
+      const int8_t mv = L.gm[g];
+      m[k] = mv;
       if (l == 0) r[k] = mv ? a.r_in[g] : T(0);
     }
   }
+  T* tmp = reinterpret_cast<T*>(smem + a.off_tmp);
+  const int nsteps = a.nsteps;
   // every wave walks the schedule: a block-mode step, or a wave-mode step
   // that follows a block-mode one, starts with a workgroup barrier (the same
   // decision in every wave); consecutive wave-mode steps are separated by a
@@ -749,7 +948,7 @@ __global__ __launch_bounds__(1024) void k_tail(TailArgs<T> a) {
   int prev_wave = 0;
 #pragma unroll 1
   for (int i = 0; i < nsteps; i++) {
-    const TailStep st = s_st[i];
+    const TailStep st = steps[i];
     if (!st.wave || !prev_wave) __syncthreads();
     prev_wave = st.wave;
     int NT;
@@ -762,128 +961,109 @@ __global__ __launch_bounds__(1024) void k_tail(TailArgs<T> a) {
     } else {
       NT = nth;
     }
-    const int l = st.level;
-    const LevelDev L = s_lv[l];
-    const int lg = s_off[3][l];
-    const int8_t* m = reinterpret_cast<const int8_t*>(smem + s_off[2][l]);
-    T* r = reinterpret_cast<T*>(smem + s_off[0][l]);
-    T* x = reinterpret_cast<T*>(smem + s_off[1][l]);
-    const int n = L.ny * L.nx;
+    const TailLev& L = lvs[st.level];
+    const RingGeo G = ring_geo(L);
     if (st.op == OP_SWEEP) {
-      const T cx = T(L.cx), cy = T(L.cy), id = T(L.inv_d);
-      const int col = st.col;
-      const bool even = !(L.nx & 1);
-      const int hx = L.nx >> 1, lgh = lg > 0 ? lg - 1 : -1;
-      const int cnt = even ? L.ny * hx : n;
-      for (int j = tid; j < cnt; j += NT) {
-        int y, xx;
-        if (even) {
-          int i2;
-          yx_of(j, hx, lgh, y, i2);
-          xx = 2 * i2 + ((col ^ y) & 1);
-        } else {
-          yx_of(j, L.nx, lg, y, xx);
-          if (((y + xx) & 1) != col) continue;
-        }
-        const int k = y * L.nx + xx;
-        const bool wet = m[k] != 0;
-        T v;
-        if (st.fromzero) {
-          v = r[k] * id;
-        } else {
-          T sx, sy;
-          nbsum(x, L, y, xx, k, sx, sy);
-          v = (r[k] + cx * sx + cy * sy) * id;
-        }
-        if (wet) x[k] = v;
-      }
+#define F2D_CALL(v) t_sweep<T, v>(L, G, smem, st.col, st.fromzero, tid, NT)
+      F2D_TAIL_LG(L.lg, F2D_CALL)
+#undef F2D_CALL
     } else if (st.op == OP_RES) {
-      const T cx = T(L.cx), cy = T(L.cy), dd = T(L.d);
-      for (int k = tid; k < n; k += NT) {
-        int y, xx;
-        yx_of(k, L.nx, lg, y, xx);
-        const bool wet = m[k] != 0;
-        T sx, sy;
-        nbsum(x, L, y, xx, k, sx, sy);
-        const T res = r[k] - (dd * x[k] - cx * sx - cy * sy);
-        tmp[k] = wet ? res : T(0);
+#define F2D_CALL(v) t_res<T, v>(L, G, smem, tmp, tid, NT)
+      F2D_TAIL_LG(L.lg, F2D_CALL)
+#undef F2D_CALL
+    } else if (st.op == OP_RESTRICT) {
+      const TailLev& C = lvs[st.level + 1];
+#define F2D_CALL(v) t_restrict<T, v>(L, C, smem, tmp, tid, NT)
+      F2D_TAIL_LG(L.lg, F2D_CALL)
+#undef F2D_CALL
+    } else if (st.op == OP_PROLONG) {
+      const TailLev& C = lvs[st.level + 1];
+#define F2D_CALL(v) t_prolong<T, v>(L, C, G, smem, tid, NT)
+      F2D_TAIL_LG(L.lg, F2D_CALL)
+#undef F2D_CALL
+    } else if (st.op == OP_DENSE) {
+      // block mode: wave w forms the partial products of columns
+      // [w cpw, (w + 1) cpw), lane i owns unknown i; wave 0 adds the partials
+      const int P = L.P;
+      const T* r = reinterpret_cast<const T*>(smem + L.off_r);
+      T* x = reinterpret_cast<T*>(smem + L.off_x);
+      const int nw = a.dense_n, nwv = nth >> 6;
+      const int cpw = (nw + nwv - 1) / nwv;
+      const int j0 = wave * cpw;
+      const bool act = lane < nw;
+      int y = 0, xx = 0;
+      if (act) yx_of(a.dense_idx[lane], L.nx, L.lg, y, xx);
+      const int k = (y + 1) * P + xx + 1;
+      const T ri = act ? r[k] : T(0);
+      T mv[DENSE_COLS];
+#pragma unroll
+      for (int u = 0; u < DENSE_COLS; u++) {
+        const int j = j0 + u;
+        mv[u] = (u < cpw && j < nw && act) ? a.dense[j * nw + lane] : T(0);
       }
-    } else if (st.op == OP_RESTRICT) {  // level l -> l + 1, from tmp
-      const LevelDev C = s_lv[l + 1];
-      const int lgc = s_off[3][l + 1];
-      const int8_t* mc = reinterpret_cast<const int8_t*>(smem + s_off[2][l + 1]);
-      T* rc = reinterpret_cast<T*>(smem + s_off[0][l + 1]);
-      const int nc = C.ny * C.nx;
-      for (int k = tid; k < nc; k += NT) {
-        int J, I;
-        yx_of(k, C.nx, lgc, J, I);
-        const int y = 2 * J, xx = 2 * I;
-        const bool wet = mc[k] != 0;
-        T v = nb(tmp, L, y, xx, 0, 0) +
-              T(0.5) * ((nb(tmp, L, y, xx, 0, -1) + nb(tmp, L, y, xx, 0, 1)) +
-                        (nb(tmp, L, y, xx, -1, 0) + nb(tmp, L, y, xx, 1, 0))) +
-              T(0.25) * ((nb(tmp, L, y, xx, -1, -1) + nb(tmp, L, y, xx, -1, 1)) +
-                         (nb(tmp, L, y, xx, 1, -1) + nb(tmp, L, y, xx, 1, 1)));
-        rc[k] = wet ? T(0.25) * v : T(0);
+      T part = T(0);
+#pragma unroll
+      for (int u = 0; u < DENSE_COLS; u++) {
+        const int j = j0 + u;
+        if (u < cpw && j < nw) part += mv[u] * lane_bcast(ri, j);
       }
-    } else if (st.op == OP_PROLONG) {  // level l + 1 -> l, added to x
-      const LevelDev C = s_lv[l + 1];
-      const T* xc = reinterpret_cast<const T*>(smem + s_off[1][l + 1]);
-      for (int k = tid; k < n; k += NT) {
-        int y, xx;
-        yx_of(k, L.nx, lg, y, xx);
-        const bool wet = m[k] != 0;
-        const int J = y >> 1, I = xx >> 1;
-        const bool oy = y & 1, ox = xx & 1;
-        const T c00 = xc[J * C.nx + I];
-        const T c01 = nb(xc, C, J, I, 0, 1), c10 = nb(xc, C, J, I, 1, 0);
-        const T c11 = nb(xc, C, J, I, 1, 1);
-        T v = c00;
-        if (ox && !oy)
-          v = T(0.5) * (c00 + c01);
-        else if (oy && !ox)
-          v = T(0.5) * (c00 + c10);
-        else if (oy && ox)
-          v = T(0.25) * ((c00 + c01) + (c10 + c11));
-        const T xv = x[k];
-        if (wet) x[k] = xv + v;
+      tmp[wave * 64 + lane] = part;
+      __syncthreads();
+      if (wave == 0 && act) {
+        T s = T(0);
+        for (int w = 0; w < nwv; w++) s += tmp[w * 64 + lane];
+        put(x, G, k, y, xx, s);
       }
     } else {  // OP_MEAN_R / OP_MEAN_X: zero-mean projection (wave mode)
-      T* v = st.op == OP_MEAN_R ? r : x;
+      const int P = L.P, n = L.ny * L.nx;
+      const int8_t* m = reinterpret_cast<const int8_t*>(smem + L.off_m);
+      const bool isr = st.op == OP_MEAN_R;
+      T* v = reinterpret_cast<T*>(smem + (isr ? L.off_r : L.off_x));
       double sm = 0.0, c = 0.0;
-      for (int k = tid; k < n; k += 64)
+      for (int j = tid; j < n; j += 64) {
+        int y, xx;
+        yx_of(j, L.nx, L.lg, y, xx);
+        const int k = (y + 1) * P + xx + 1;
         if (m[k]) {
           sm += (double)v[k];
           c += 1.0;
         }
+      }
       sm = wave_sum(sm);
       c = wave_sum(c);
       const double mean = c > 0 ? sm / c : 0.0;
-      for (int k = tid; k < n; k += 64)
-        if (m[k]) v[k] = v[k] - T(mean);
+      for (int j = tid; j < n; j += 64) {
+        int y, xx;
+        yx_of(j, L.nx, L.lg, y, xx);
+        const int k = (y + 1) * P + xx + 1;
+        if (m[k]) {
+          const T nv = v[k] - T(mean);
+          if (isr) v[k] = nv;
+          else put(v, G, k, y, xx, nv);
+        }
+      }
     }
   }
   __syncthreads();
   {
-    const LevelDev L = s_lv[0];
-    const T* x = reinterpret_cast<const T*>(smem + s_off[1][0]);
+    const TailLev& L = lvs[0];
+    const T* x = reinterpret_cast<const T*>(smem + L.off_x);
     const int n = L.ny * L.nx;
-    const int lg = s_off[3][0];
-    for (int k = tid; k < n; k += nth) {
+    for (int j = tid; j < n; j += nth) {
       int y, xx;
-      yx_of(k, L.nx, lg, y, xx);
-      a.e_out[li(L, y, xx)] = x[k];
+      yx_of(j, L.nx, L.lg, y, xx);
+      a.e_out[li(a.top, y, xx)] = x[(y + 1) * L.P + xx + 1];
     }
   }
 }
 
 // The tail's schedule: pre-smoothing (red, black)^nu, residual, restriction
-// down to the coarsest level; (red, black)^S (black, red)^S there (with the
-// zero-mean projection of the singular operator); prolongation and
-// post-smoothing (black, red)^nu back up.
+// down to the coarsest level; there either the dense solve or
+// (red, black)^S (black, red)^S (with the zero-mean projection of the
+// singular operator); prolongation and post-smoothing (black, red)^nu back up.
 static std::vector<TailStep> tail_schedule(const std::vector<int>& npts,
                                            int nu, int coarse_sweeps,
-                                           bool singular) {
+                                           bool singular, bool dense) {
   std::vector<TailStep> v;
   const int nl = (int)npts.size();
   auto add = [&](int op, int l, int col, int fz, int wmode) {
@@ -901,7 +1081,13 @@ static std::vector<TailStep> tail_schedule(const std::vector<int>& npts,
     add(OP_RES, l, 0, 0, w);
     add(OP_RESTRICT, l, 0, 0, w);
   }
-  {
+  if (dense) {
+    // the pseudo-inverse annihilates constants only up to rounding in FP32:
+    // the explicit projections keep the correction exactly zero-mean
+    if (singular) add(OP_MEAN_R, nl - 1, 0, 0, 1);
+    add(OP_DENSE, nl - 1, 0, 0, 0);
+    if (singular) add(OP_MEAN_X, nl - 1, 0, 0, 1);
+  } else {
     const int l = nl - 1;
     const int w = npts[l] <= WAVE_PTS;
     if (singular) add(OP_MEAN_R, l, 0, 0, 1);
@@ -1261,6 +1447,84 @@ static dim3 tiles(const LevelDev& L, int fc) {
   return dim3((unsigned)((L.nx + fc - 1) / fc), (unsigned)((L.ny + fc - 1) / fc));
 }
 
+// Dense inverse of a coarse level's operator K (d on the diagonal, -cx / -cy
+// for each wet neighbour, wrapped in periodic directions; a neighbour met
+// twice through the wrap counts twice) over its wet points, row-major in
+// `idx` order.  For the singular operator (all wet, doubly periodic, no
+// diagonal shift) the pseudo-inverse (K + u u^T)^-1 - u u^T, u = 1/sqrt(nw):
+// it annihilates constants and returns zero-mean corrections.
+static void coarse_inverse(const TailLev& L, const std::vector<int8_t>& m,
+                           bool singular, std::vector<int>& idx,
+                           std::vector<double>& M) {
+  const int ny = L.ny, nx = L.nx;
+  std::vector<int> pos((size_t)ny * nx, -1);
+  idx.clear();
+  for (int k = 0; k < ny * nx; k++)
+    if (m[k]) {
+      pos[k] = (int)idx.size();
+      idx.push_back(k);
+    }
+  const int n = (int)idx.size();
+  F2D_REQUIRE(n <= DENSE_PTS, "coarse level too large for the dense solve");
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) {
+    const int y = idx[i] / nx, x = idx[i] % nx;
+    A[(size_t)i * n + i] += L.d;
+    const int dy[4] = {0, 0, -1, 1}, dx[4] = {-1, 1, 0, 0};
+    for (int q = 0; q < 4; q++) {
+      int yy = y + dy[q], xx = x + dx[q];
+      if (xx < 0 || xx >= nx) {
+        if (!L.xper) continue;
+        xx = (xx + nx) % nx;
+      }
+      if (yy < 0 || yy >= ny) {
+        if (!L.yper) continue;
+        yy = (yy + ny) % ny;
+      }
+      const int j = pos[yy * nx + xx];
+      if (j >= 0) A[(size_t)i * n + j] -= (q < 2 ? L.cx : L.cy);
+    }
+  }
+  const double u2 = singular ? 1.0 / n : 0.0;
+  if (singular)
+    for (auto& v : A) v += u2;
+  // Cholesky A = C C^T, then M = A^-1 column by column
+  std::vector<double> C((size_t)n * n, 0.0);
+  for (int j = 0; j < n; j++) {
+    double s = A[(size_t)j * n + j];
+    for (int k = 0; k < j; k++) s -= C[(size_t)j * n + k] * C[(size_t)j * n + k];
+    F2D_REQUIRE(s > 0.0, "coarse operator is not positive definite");
+    const double cjj = std::sqrt(s);
+    C[(size_t)j * n + j] = cjj;
+    for (int i = j + 1; i < n; i++) {
+      double t = A[(size_t)i * n + j];
+      for (int k = 0; k < j; k++) t -= C[(size_t)i * n + k] * C[(size_t)j * n + k];
+      C[(size_t)i * n + j] = t / cjj;
+    }
+  }
+  M.assign((size_t)n * n, 0.0);
+  std::vector<double> w(n);
+  for (int c = 0; c < n; c++) {
+    for (int i = 0; i < n; i++) {
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) t -= C[(size_t)i * n + k] * w[k];
+      w[i] = t / C[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double t = w[i];
+      for (int k = i + 1; k < n; k++) t -= C[(size_t)k * n + i] * w[k];
+      w[i] = t / C[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) M[(size_t)i * n + c] = w[i] - u2;
+  }
+  // exact symmetry (the kernel reads M by columns)
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      const double v = 0.5 * (M[(size_t)i * n + j] + M[(size_t)j * n + i]);
+      M[(size_t)i * n + j] = M[(size_t)j * n + i] = v;
+    }
+}
+
 template <class T>
 static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
   const int tail_points = h->opts.tail_points > 0 ? h->opts.tail_points : 4096;
@@ -1283,6 +1547,7 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
   double cx = h->base.cx, cy = h->base.cy;
   while ((int)P.lv.size() < MAXL) {
     if ((ny & 1) || (nx & 1) || ny < 4 || nx < 4) break;
+    if ((int64_t)ny * nx <= DENSE_PTS && P.lv.size() > 1) break;
     int cny = ny / 2, cnx = nx / 2;
     std::vector<int8_t> mc((size_t)cny * cnx);
     for (int J = 0; J < cny; J++)
@@ -1313,16 +1578,20 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
     nx = cnx;
   }
   P.L = (int)P.lv.size();
+  // LDS of a tail starting at level l: r, x and mask of each level with
+  // their one-point ring, and the residual scratch
+  auto padded = [&](int k) {
+    return (size_t)(P.lv[k].d.ny + 2) * (size_t)(P.lv[k].d.nx + 2);
+  };
+  auto al16 = [](size_t b) { return (b + 15) / 16 * 16; };
   auto tail_bytes = [&](int l) {
-    size_t bytes = 0, nmax = 0;
+    size_t bytes = 0, nmax = 1024;
     for (int k = l; k < P.L; k++) {
-      size_t n = (size_t)P.lv[k].d.ny * P.lv[k].d.nx;
-      bytes += 2 * ((n * sizeof(T) + 15) / 16) * 16 + ((n + 15) / 16) * 16;
+      const size_t n = padded(k);
+      bytes += 2 * al16(n * sizeof(T)) + al16(n);
       if (n > nmax) nmax = n;
     }
-    size_t tmp = nmax * sizeof(T);
-    if (tmp < 64 * sizeof(double)) tmp = 64 * sizeof(double);
-    return bytes + ((tmp + 15) / 16) * 16;
+    return bytes + al16(nmax * sizeof(T));
   };
   P.Lt = P.L;
   for (int l = 1; l < P.L; l++) {
@@ -1356,31 +1625,43 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
     TailArgs<T>& ta = P.tail;
     ta.nl = P.L - P.Lt;
     int off = 0;
-    size_t nmax = 0;
+    size_t nmax = 1024;
     for (int k = 0; k < ta.nl; k++) {
       const HostLevel& v = P.lv[P.Lt + k];
-      ta.lv[k] = v.d;
-      int nxl = v.d.nx, lg = -1;
-      if ((nxl & (nxl - 1)) == 0) {
-        lg = 0;
-        while ((1 << lg) < nxl) lg++;
+      F2D_REQUIRE(v.d.ny >= 2 && v.d.nx >= 2, "tail level narrower than 2 points");
+      TailLev& t = ta.lv[k];
+      t = TailLev{};
+      t.ny = v.d.ny;
+      t.nx = v.d.nx;
+      t.P = v.d.nx + 2;
+      t.lg = -1;
+      if ((t.nx & (t.nx - 1)) == 0) {
+        t.lg = 0;
+        while ((1 << t.lg) < t.nx) t.lg++;
       }
-      ta.lgx[k] = lg;
-      size_t n = (size_t)v.d.ny * v.d.nx;
+      t.xper = v.d.xper;
+      t.yper = v.d.yper;
+      t.stride = v.d.stride;
+      t.origin = v.d.origin;
+      t.gm = v.d.m;
+      t.cx = v.d.cx;
+      t.cy = v.d.cy;
+      t.d = v.d.d;
+      t.inv_d = v.d.inv_d;
+      const size_t n = padded(P.Lt + k);
       if (n > nmax) nmax = n;
-      ta.off_r[k] = off;
-      off += (int)(((n * sizeof(T) + 15) / 16) * 16);
-      ta.off_x[k] = off;
-      off += (int)(((n * sizeof(T) + 15) / 16) * 16);
-      ta.off_m[k] = off;
-      off += (int)(((n + 15) / 16) * 16);
+      t.off_r = off;
+      off += (int)al16(n * sizeof(T));
+      t.off_x = off;
+      off += (int)al16(n * sizeof(T));
+      t.off_m = off;
+      off += (int)al16(n);
     }
-    size_t tmp = nmax * sizeof(T);
-    if (tmp < 64 * sizeof(double)) tmp = 64 * sizeof(double);
+    ta.top = P.lv[P.Lt].d;
     ta.off_tmp = off;
-    off += (int)(((tmp + 15) / 16) * 16);
+    off += (int)al16(nmax * sizeof(T));
     P.tail_lds = off;
-    const LevelDev& cl = ta.lv[ta.nl - 1];
+    const TailLev& cl = ta.lv[ta.nl - 1];
     int cs = h->opts.coarse_sweeps;
     if (cs <= 0) {
       int mx = cl.ny > cl.nx ? cl.ny : cl.nx;
@@ -1388,12 +1669,37 @@ static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
     }
     std::vector<int> npts;
     for (int k = 0; k < ta.nl; k++) npts.push_back(ta.lv[k].ny * ta.lv[k].nx);
-    std::vector<TailStep> sched = tail_schedule(npts, h->nu_tail, cs, h->singular);
+    // the coarsest level's dense inverse, when it is small enough
+    const bool dense = P.L > 1 && (int64_t)cl.ny * cl.nx <= DENSE_PTS;
+    ta.dense = nullptr;
+    ta.dense_idx = nullptr;
+    ta.dense_n = 0;
+    if (dense) {
+      std::vector<int> idx;
+      std::vector<double> M;
+      coarse_inverse(cl, mprev, h->singular, idx, M);
+      const int nw = (int)idx.size();
+      if (nw > 0) {
+        std::vector<T> Mt(M.begin(), M.end());
+        T* dM = (T*)dalloc(h, Mt.size() * sizeof(T));
+        int* dI = (int*)dalloc(h, idx.size() * sizeof(int));
+        F2D_HIP(hipMemcpy(dM, Mt.data(), Mt.size() * sizeof(T), hipMemcpyHostToDevice));
+        F2D_HIP(hipMemcpy(dI, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+        ta.dense = dM;
+        ta.dense_idx = dI;
+      }
+      ta.dense_n = nw;
+    }
+    std::vector<TailStep> sched =
+        tail_schedule(npts, h->nu_tail, cs, h->singular, dense);
     F2D_REQUIRE((int)sched.size() <= MAXOPS, "tail schedule too long");
     TailStep* dsched = (TailStep*)dalloc(h, sched.size() * sizeof(TailStep));
     F2D_HIP(hipMemcpy(dsched, sched.data(), sched.size() * sizeof(TailStep),
                       hipMemcpyHostToDevice));
     ta.steps = dsched;
+    TailLev* dlv = (TailLev*)dalloc(h, sizeof(TailLev) * ta.nl);
+    F2D_HIP(hipMemcpy(dlv, ta.lv, sizeof(TailLev) * ta.nl, hipMemcpyHostToDevice));
+    ta.lvs = dlv;
     ta.nsteps = (int)sched.size();
     ta.r_in = (const T*)P.lv[P.Lt].r;
     ta.e_out = (T*)P.lv[P.Lt].e;

@@ -44,9 +44,15 @@ extern "C" {
 #define F2D_POISSON_F32_REFINE 1 /* FP32 MG-PCG inside FP64 iterative refinement */
 #define F2D_POISSON_F32 2        /* FP32 MG-PCG only                             */
 
+/* ABI version of this header (no counterpart in the reference). */
 int f2d_abi_version(void);
+/* Message of the last failed call on this thread.  Every entry point returns
+ * an int status (0 = success) where the reference raises a Python exception
+ * (no counterpart in the reference: SuperLU's RuntimeError, elliptic.py:78,
+ * and numba's TypeError are its only error paths). */
 const char* f2d_last_error(void);
-/* Synchronise the current device (host-side wait). */
+/* Synchronise the current device (host-side wait; no counterpart in the
+ * reference, whose kernels are synchronous). */
 int f2d_device_synchronize(void);
 
 /* ------------------------------------------------------------------------
@@ -114,8 +120,9 @@ int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
                         const void* ds_prev1, void* ds_out, int integrator,
                         int stage, const double* dt, void* stream);
 
-/* y += ((c0*x0 + c1*x1) + c2*x2) over n entries (addto, integrators.py:174);
- * coefficients of `integrator` stage `stage` from *dt as above. */
+/* addto, integrators.py:154-196 (the sum of :174): over n entries
+ * y += ((c0*x0 + c1*x1) + c2*x2), with the coefficients of `integrator`
+ * stage `stage` evaluated from *dt as above. */
 int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,
                  const void* x2, int64_t n, int integrator, int stage,
                  const double* dt, void* stream);
@@ -145,7 +152,8 @@ int f2d_fill_halo(int dtype, const f2d_grid* g, void* a, void* stream);
  * arrays; result in *dt (device float64). */
 int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
                double cfl, double dtmax, double* dt, void* stream);
-/* dt = value (device scalar) -- fixed-dt runs and the rsw branch */
+/* dt = value (device scalar): the fixed-dt and rsw branches of Model.set_dt,
+ * model.py:71-82 */
 int f2d_set_scalar(double* dst, double value, void* stream);
 
 /* ------------------------------------------------------------------------
@@ -177,6 +185,8 @@ typedef struct f2d_poisson_opts {
 int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
                        double maindiag, const f2d_poisson_opts* opts,
                        f2d_poisson** out);
+/* Frees the handle: the reference's factor lives as long as its Poisson2D
+ * object (elliptic.py:71-78). */
 int f2d_poisson_destroy(f2d_poisson* h);
 /* Poisson2D.solve(b, x), elliptic.py:80-87: x (in/out, initial guess) is
  * written at the unknowns only, then its halo is filled.  The solve stops
@@ -198,9 +208,12 @@ int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
  * solve enqueues exactly `maxit` iterations (refinements), those after
  * convergence returning at once, and never synchronises: these counters are
  * how a caller checks such solves afterwards. */
+/* (no counterpart in the reference: SuperLU is direct, elliptic.py:78-87) */
 const double* f2d_poisson_stats(f2d_poisson* h);
+/* Zero the counters [4..6] (no counterpart in the reference). */
 int f2d_poisson_reset_stats(f2d_poisson* h, void* stream);
-/* Copy the 8 statistics above to host memory (synchronises `stream`). */
+/* Copy the 8 statistics above to host memory, synchronising `stream`
+ * (no counterpart in the reference). */
 int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream);
 /* Measurement hook (no counterpart in the reference): launch one fine-level
  * kernel of the CG iteration `reps` times on the handle's current vectors --

@@ -1,0 +1,149 @@
+"""Pins of the oracle's C-grid operators, integrators and time keeping
+(operators.py:6-211, integrators.py:82-196, timeline.py:14-35, model.py:71-87)
+against discrete identities and closed forms, and the reference's own
+known-answer test (tests/test_models.py:17-26) run on the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import P, random_mask
+
+
+def _vec(shape):
+    return O.Vector(np.zeros(shape), np.zeros(shape))
+
+
+def _psi(mesh, seed):
+    """random streamfunction vanishing at the Dirichlet vertices"""
+    rng = np.random.default_rng(seed)
+    unk = O.unknown_mask(mesh.param, mesh.mskv)
+    psi = rng.standard_normal(mesh.shape) * unk
+    mesh.fill(psi)
+    return psi
+
+
+@pytest.mark.parametrize("xper,yper,seed", [(True, True, 0), (False, False, 1), (True, False, 2)])
+def test_vorticity_of_perpgrad_is_5pt_laplacian(xper, yper, seed):
+    p = P(nx=16, ny=12, xperiodic=xper, yperiodic=yper)
+    mesh = O.Mesh(p, msk=random_mask(12, 16, seed, xper, yper, nisl=2))
+    psi = _psi(mesh, seed)
+    u = _vec(mesh.shape)
+    O.perpgrad(mesh, psi, u)
+    w = np.zeros(mesh.shape)
+    O.compute_vorticity(mesh, u, w)
+    lap = np.zeros(mesh.shape)
+    lap[1:-1, 1:-1] = (psi[2:, 1:-1] + psi[:-2, 1:-1] + psi[1:-1, 2:] + psi[1:-1, :-2]
+                       - 4 * psi[1:-1, 1:-1])
+    sel = (mesh.slipcoef > 0)
+    sel[0, :] = sel[-1, :] = False
+    sel[:, 0] = sel[:, -1] = False
+    assert np.allclose(w[sel], lap[sel], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("xper,seed", [(True, 3), (False, 4)])
+def test_divergence_of_contravariant_perpgrad_vanishes(xper, seed):
+    """Exact only for dx == dy: the sum is dxdy psi (1/dx^2 - 1/dy^2)."""
+    p = P(nx=14, ny=14, xperiodic=xper)
+    mesh = O.Mesh(p, msk=random_mask(14, 14, seed, xper, False, nisl=2))
+    psi = _psi(mesh, seed)
+    U = _vec(mesh.shape)
+    O.perpgrad(mesh, psi, U, contravariant=True)
+    d = np.zeros(mesh.shape)
+    O.div(mesh, U, d)
+    scale = np.abs(U.x).max()
+    # interior cells (the halo columns are refreshed by the halo fill)
+    assert np.abs(d[3:-3, 3:-3]).max() <= 1e-13 * scale
+    # and it does not vanish with dx != dy
+    p2 = P(nx=14, ny=14, Ly=0.7, xperiodic=xper)
+    mesh2 = O.Mesh(p2, msk=mesh.msk)
+    O.perpgrad(mesh2, psi, U, contravariant=True)
+    O.div(mesh2, U, d)
+    assert np.abs(d[3:-3, 3:-3]).max() > 1e-3 * np.abs(U.x).max()
+
+
+@pytest.mark.parametrize("method", ["weno", "upwind", "centered", "cweno"])
+def test_divflux_conserves_and_keeps_constants(method):
+    p = P(nx=18, ny=15, Lx=1.2, compflux=method)  # dx == dy
+    mesh = O.Mesh(p, msk=random_mask(15, 18, 6, False, False, nisl=3))
+    rng = np.random.default_rng(6)
+    U = O.Vector(rng.standard_normal(mesh.shape), rng.standard_normal(mesh.shape))
+    q = rng.standard_normal(mesh.shape) * mesh.msk
+    flx, dq = _vec(mesh.shape), np.zeros(mesh.shape)
+    O.divflux(p, mesh, flx, q, U, dq)
+    # flux form, zero flux through faces touching dry cells: the total is 0
+    assert abs(dq[mesh.msk == 1].sum()) <= 1e-12 * np.abs(dq).sum()
+    # a constant is advected unchanged by a divergence-free (dx == dy) flow
+    psi = _psi(mesh, 7)
+    O.perpgrad(mesh, psi, U, contravariant=True)
+    O.divflux(p, mesh, flx, 2.5 * mesh.msk.astype(float), U, dq)
+    assert np.abs(dq[3:-3, 3:-3]).max() <= 1e-12 * np.abs(U.x).max()
+
+
+def test_centers_vertices_constants():
+    p = P(nx=10, ny=9)
+    mesh = O.Mesh(p, msk=random_mask(9, 10, 8, False, False, nisl=1))
+    w = 3.0 * np.ones(mesh.shape)
+    v = np.zeros(mesh.shape)
+    O.centerstovertices(mesh, w, v)
+    assert np.all(v[mesh.mskv == 1] == 3.0) and np.all(v[mesh.mskv == 0] == 0.0)
+    h = np.zeros(mesh.shape)
+    O.verticestocenters(mesh, v, h)
+    touched = np.zeros(mesh.shape, dtype=bool)
+    m = mesh.mskv
+    touched[:-1, :-1] = (m[:-1, :-1] + m[1:, :-1] + m[:-1, 1:] + m[1:, 1:]) > 0
+    sel = touched & (mesh.msk == 1)
+    assert np.allclose(h[sel], 3.0, rtol=1e-15)
+
+
+@pytest.mark.parametrize("name,poly", [
+    ("rk3", lambda z: 1 + z + z * z / 2 + z ** 3 / 6),
+    ("enrk3", lambda z: 1 + z + z * z / 2 + z ** 3 / 8),
+    ("ef", lambda z: 1 + z)])
+def test_integrator_amplification(name, poly):
+    """Linear test equation s' = lam s: one step multiplies s by the
+    stability polynomial b.(I - zA)^-1 1 of the scheme's tableau.  SSP-RK3
+    (3rd order) gives 1 + z + z^2/2 + z^3/6.  The incremental enrk3 of
+    integrators.py:109-124 has c = (0, 1/3, 5/6), a31 = -5/48, a32 = 15/16,
+    b = (1/10, 1/2, 2/5), so its z^3 coefficient is b3 a32 c2 = 1/8."""
+    integ, nst = O.INTEGRATORS[name]
+    for lam, dt in [(-1.0, 0.3), (2.0, 0.1), (-7.0, 0.05), (0.5, 1.0)]:
+        s = (np.array([1.0, -2.0]),)
+        scratch = [(np.zeros(2),) for _ in range(nst)]
+
+        def rhs(x, dx):
+            dx[0][:] = lam * x[0]
+
+        integ(s, dt, rhs, lambda x: None, scratch)
+        g = poly(lam * dt)
+        assert np.allclose(s[0], g * np.array([1.0, -2.0]), rtol=1e-14, atol=0)
+
+
+def test_time_kahan_summation():
+    p = P(dt=0.1, tend=1e9, maxite=10 ** 9)
+    t = O.Time(p)
+    naive = 0.0
+    for _ in range(100000):
+        t.pushforward()
+        naive += 0.1
+    assert abs(t.t - 10000.0) < 1e-9
+    assert abs(naive - 10000.0) > 1e-9  # plain summation drifts
+    assert t.ite == 100000
+
+
+def test_kat_advection_bodyrotation_37_on_oracle():
+    """tests/test_models.py:17-26 (tracer_advection.py:10-48 set-up): the
+    oracle reaches tend = 10 in exactly 37 iterations."""
+    p = P(model="advection", tend=10)
+    model = O.Model(p)
+    mesh = model.mesh
+    omega = (4 * np.pi / 100) * np.ones(mesh.shape) * mesh.mskv
+    psi = omega * 0
+    mesh.poisson_vertices.solve(omega, psi)
+    O.perpgrad(mesh, psi, model.state.U)
+    x, y = mesh.xy("c")
+    q = model.state.q
+    q[:, :] = (np.round(x * 8) % 2 + np.round(y * 8) % 2) / 2
+    q *= mesh.msk
+    model.run()
+    assert model.time.ite == 37
