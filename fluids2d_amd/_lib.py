@@ -45,7 +45,12 @@ class PoissonOpts(ctypes.Structure):
                 ("tail_points", ctypes.c_int32),
                 ("tail_threads", ctypes.c_int32),
                 ("nu_coarse", ctypes.c_int32),
-                ("nu_tail", ctypes.c_int32)]
+                ("nu_tail", ctypes.c_int32),
+                ("solver", ctypes.c_int32),
+                ("fft_refine", ctypes.c_int32)]
+
+
+SOLVERS = {"auto": 0, "mg": 1, "fft": 2}
 
 
 _lib = None

@@ -1,0 +1,610 @@
+// f2d_fft.hip -- direct spectral solve of the fully wet, doubly periodic
+// Poisson/Helmholtz operator (see f2d_fft.hpp for the mathematics).
+//
+// Three passes over the data, each one kernel:
+//   rows:    two real rows packed as one complex row, length-nx FFT in LDS,
+//            unpacked to the two half spectra (kx = 0 .. nx/2), written
+//            column-major (each kx column contiguous in y)
+//   columns: length-ny FFT, division by the eigenvalue (and 1/(nx ny)),
+//            inverse FFT, all in LDS; written back in place
+//   rows:    the two half spectra of a row pair repacked into one complex
+//            row, inverse FFT, real/imaginary parts stored as the two rows
+//            together with their periodic halo images
+// The FFTs are mixed radix-4/2 Stockham passes (self-sorting, ping-pong
+// between two LDS buffers) with twiddles exp(-2 pi i t / N) from a table;
+// inverse transforms conjugate on the way in and out.
+#include <cmath>
+#include <vector>
+
+#include "f2d_fft.hpp"
+
+namespace f2d {
+
+template <class T> struct CplxOf;
+template <> struct CplxOf<double> { using type = double2; };
+template <> struct CplxOf<float> { using type = float2; };
+template <class T> using cplx = typename CplxOf<T>::type;
+
+template <class T>
+__device__ __forceinline__ cplx<T> mk(T a, T b) {
+  cplx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <class T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x + b.x, a.y + b.y);
+}
+template <class T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x - b.x, a.y - b.y);
+}
+template <class T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+struct FftArgs {
+  int nx, ny, lgx, lgy;
+  int nk;        // nx / 2 + 1 spectral columns
+  int NXP;       // padded row stride
+  int P;         // row pairs per workgroup (row passes)
+  int Q;         // spectral columns per workgroup (column pass)
+  double cx, cy, d;
+  int singular;
+  int pad;
+};
+
+// Forward DFTs of `nf` contiguous length-N sequences held in LDS buffer `a`
+// (`b` is scratch of the same size).  Returns the buffer with the result.
+template <class T>
+__device__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int N, int lgN, int nf,
+                            const cplx<T>* __restrict__ tw) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  int Ns = 1;
+  if (lgN & 1) {  // one radix-2 pass (Ns = 1: no twiddles)
+    const int nb = N >> 1;
+    for (int w = tid; w < nf * nb; w += nth) {
+      const int f = w >> (lgN - 1), j = w & (nb - 1);
+      const cplx<T>* s = a + f * N;
+      cplx<T>* o = b + f * N;
+      const cplx<T> a0 = s[j], a1 = s[j + nb];
+      o[2 * j] = cadd<T>(a0, a1);
+      o[2 * j + 1] = csub<T>(a0, a1);
+    }
+    __syncthreads();
+    cplx<T>* t = a;
+    a = b;
+    b = t;
+    Ns = 2;
+  }
+  while (Ns < N) {  // radix-4 passes
+    const int nb = N >> 2;
+    const int tstep = N / (4 * Ns);
+    for (int w = tid; w < nf * nb; w += nth) {
+      const int f = w >> (lgN - 2), j = w & (nb - 1);
+      const cplx<T>* s = a + f * N;
+      cplx<T>* o = b + f * N;
+      const int jm = j & (Ns - 1);
+      const int st = jm * tstep;
+      const cplx<T> a0 = s[j];
+      const cplx<T> a1 = cmul<T>(s[j + nb], tw[st]);
+      const cplx<T> a2 = cmul<T>(s[j + 2 * nb], tw[2 * st]);
+      const cplx<T> a3 = cmul<T>(s[j + 3 * nb], tw[3 * st]);
+      const cplx<T> b0 = cadd<T>(a0, a2), b1 = csub<T>(a0, a2);
+      const cplx<T> b2 = cadd<T>(a1, a3), b3 = csub<T>(a1, a3);
+      const int base = ((j - jm) << 2) + jm;
+      o[base] = cadd<T>(b0, b2);
+      o[base + Ns] = mk<T>(b1.x + b3.y, b1.y - b3.x);      // b1 - i b3
+      o[base + 2 * Ns] = csub<T>(b0, b2);
+      o[base + 3 * Ns] = mk<T>(b1.x - b3.y, b1.y + b3.x);  // b1 + i b3
+    }
+    __syncthreads();
+    cplx<T>* t = a;
+    a = b;
+    b = t;
+    Ns <<= 2;
+  }
+  return a;
+}
+
+// right-hand side at interior vertex k (padded index): b itself, or
+// centerstovertices (operators.py:126-133) of the cell field
+template <class TB>
+__device__ __forceinline__ double gval(const TB* __restrict__ b, int64_t k,
+                                       int NXP, int gmode) {
+  if (gmode == 0) return (double)b[k];
+  return 0.25 * ((((double)b[k - NXP - 1] + (double)b[k - 1]) +
+                  (double)b[k - NXP]) + (double)b[k]);
+}
+
+template <class T, class TB>
+__global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
+                                                      const TB* __restrict__ b,
+                                                      int gmode,
+                                                      cplx<T>* __restrict__ S,
+                                                      const cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nx = a.nx, P = a.P, lg = a.lgx;
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* b1 = b0 + P * nx;
+  const int y0 = blockIdx.x * 2 * P;
+  for (int w = threadIdx.x; w < P * nx; w += blockDim.x) {
+    const int p = w >> lg, n = w & (nx - 1);
+    const int64_t ka = (int64_t)(y0 + 2 * p + 3) * a.NXP + n + 3;
+    const double va = gval(b, ka, a.NXP, gmode);
+    const double vb = gval(b, ka + a.NXP, a.NXP, gmode);
+    b0[w] = mk<T>(T(va), T(vb));
+  }
+  __syncthreads();
+  const cplx<T>* r = fft_lds<T>(b0, b1, nx, lg, P, tw);
+  // half spectra of the two rows: Xa = (C + D) / 2, Xb = -i (C - D) / 2 with
+  // C = F(k), D = conj(F(nx - k))
+  const int n2 = a.nk * 2 * P;
+  for (int w = threadIdx.x; w < n2; w += blockDim.x) {
+    const int ab = w & 1, p = (w >> 1) % P, k = (w >> 1) / P;
+    const cplx<T> C = r[p * nx + k];
+    const cplx<T> D0 = r[p * nx + ((nx - k) & (nx - 1))];
+    const cplx<T> D = mk<T>(D0.x, -D0.y);
+    cplx<T> X;
+    if (ab == 0) {
+      X = mk<T>(T(0.5) * (C.x + D.x), T(0.5) * (C.y + D.y));
+    } else {
+      const cplx<T> u = csub<T>(C, D);
+      X = mk<T>(T(0.5) * u.y, T(-0.5) * u.x);
+    }
+    S[(int64_t)k * a.ny + y0 + 2 * p + ab] = X;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_cols(FftArgs a,
+                                                  cplx<T>* __restrict__ S,
+                                                  const cplx<T>* __restrict__ tw,
+                                                  const double* __restrict__ lamx,
+                                                  const double* __restrict__ lamy) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ny = a.ny, lg = a.lgy;
+  const int k0 = blockIdx.x * a.Q;
+  const int nq = min(a.Q, a.nk - k0);
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* b1 = b0 + a.Q * ny;
+  cplx<T>* col = S + (int64_t)k0 * ny;
+  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) b0[w] = col[w];
+  __syncthreads();
+  cplx<T>* r = fft_lds<T>(b0, b1, ny, lg, nq, tw);
+  cplx<T>* o = r == b0 ? b1 : b0;
+  const double inv = 1.0 / ((double)a.nx * (double)a.ny);
+  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+    const int q = w >> lg, ky = w & (ny - 1);
+    const double lam = lamx[k0 + q] + lamy[ky];
+    const T s = lam != 0.0 ? T(inv / lam) : T(0);
+    const cplx<T> v = r[w];
+    r[w] = mk<T>(v.x * s, -(v.y * s));  // conjugate: inverse by forward
+  }
+  __syncthreads();
+  const cplx<T>* z = fft_lds<T>(r, o, ny, lg, nq, tw);
+  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+    const cplx<T> v = z[w];
+    col[w] = mk<T>(v.x, -v.y);
+  }
+}
+
+// value v at interior point (y, n) and its periodic halo images
+template <class TX>
+__device__ __forceinline__ void put_images(TX* x, int y, int n, const FftArgs& a,
+                                           TX v) {
+  const int64_t P = a.NXP;
+  const int64_t k = (int64_t)(y + 3) * P + n + 3;
+  const int64_t dxs = n < 3 ? a.nx : (n >= a.nx - 3 ? -a.nx : 0);
+  const int64_t dys = y < 3 ? (int64_t)a.ny * P : (y >= a.ny - 3 ? -(int64_t)a.ny * P : 0);
+  x[k] = v;
+  if (dxs) x[k + dxs] = v;
+  if (dys) x[k + dys] = v;
+  if (dxs && dys) x[k + dxs + dys] = v;
+}
+
+// MODE 0: x = result (TX); 1: x = result and x64 = result; 2: x64 += result,
+// x = (TX) x64
+template <class T, class TX, int MODE>
+__global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
+                                                      const cplx<T>* __restrict__ S,
+                                                      const cplx<T>* __restrict__ tw,
+                                                      TX* __restrict__ x,
+                                                      double* __restrict__ x64) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nx = a.nx, P = a.P, lg = a.lgx, h = nx >> 1;
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* b1 = b0 + P * nx;
+  const int y0 = blockIdx.x * 2 * P;
+  // Xa -> b1, Xb -> b0 (k = 0 .. nx/2)
+  const int n2 = a.nk * 2 * P;
+  for (int w = threadIdx.x; w < n2; w += blockDim.x) {
+    const int ab = w & 1, p = (w >> 1) % P, k = (w >> 1) / P;
+    const cplx<T> X = S[(int64_t)k * a.ny + y0 + 2 * p + ab];
+    (ab == 0 ? b1 : b0)[p * nx + k] = X;
+  }
+  __syncthreads();
+  // conj(C) with C(k) = Xa(k) + i Xb(k) and C(nx - k) = conj(Xa(k)) +
+  // i conj(Xb(k)), written over b1 (each thread reads its own k first)
+  for (int w = threadIdx.x; w < P * (h + 1); w += blockDim.x) {
+    const int p = w / (h + 1), k = w - p * (h + 1);
+    const cplx<T> Xa = b1[p * nx + k], Xb = b0[p * nx + k];
+    b1[p * nx + k] = mk<T>(Xa.x - Xb.y, -(Xa.y + Xb.x));
+    if (k > 0 && k < h) b1[p * nx + nx - k] = mk<T>(Xa.x + Xb.y, -(Xb.x - Xa.y));
+  }
+  __syncthreads();
+  const cplx<T>* r = fft_lds<T>(b1, b0, nx, lg, P, tw);
+  // conj of the result: row a = Re, row b = -Im of r
+  for (int w = threadIdx.x; w < P * nx; w += blockDim.x) {
+    const int p = w >> lg, n = w & (nx - 1);
+    const int ya = y0 + 2 * p;
+    const cplx<T> v = r[w];
+    const double va = (double)v.x, vb = -(double)v.y;
+    if (MODE == 2) {
+      const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
+      const double na = x64[ka] + va, nb = x64[ka + a.NXP] + vb;
+      put_images<double>(x64, ya, n, a, na);
+      put_images<double>(x64, ya + 1, n, a, nb);
+      put_images<TX>(x, ya, n, a, TX(na));
+      put_images<TX>(x, ya + 1, n, a, TX(nb));
+    } else {
+      put_images<TX>(x, ya, n, a, TX(va));
+      put_images<TX>(x, ya + 1, n, a, TX(vb));
+      if (MODE == 1) {
+        put_images<double>(x64, ya, n, a, va);
+        put_images<double>(x64, ya + 1, n, a, vb);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double bsum256(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// partial sums of b over the interior (FP64)
+template <class TB>
+__global__ __launch_bounds__(256) void k_fft_bsum(FftArgs a, const TB* __restrict__ b,
+                                                  int gmode, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (idx < (int64_t)a.nx * a.ny) {
+    const int y = (int)(idx >> a.lgx), q = (int)(idx & (a.nx - 1));
+    v = gval(b, (int64_t)(y + 3) * a.NXP + q + 3, a.NXP, gmode);
+  }
+  v = bsum256(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fft_mean(const double* __restrict__ part,
+                                                  int nb, double n,
+                                                  double* __restrict__ mean) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) acc += part[i];
+  acc = bsum256(acc, sh);
+  if (threadIdx.x == 0) *mean = acc / n;
+}
+
+// r = (b - mb) - A x64 at the interior (FP64), partial sums of r^2, r, b^2,
+// b; mb (may be NULL: 0) is the mean of b for the singular operator, taken
+// out before the squares are summed (no cancellation in the projected norm)
+template <class TB>
+__global__ __launch_bounds__(256) void k_fft_res(FftArgs a, const TB* __restrict__ b,
+                                                 int gmode,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ r,
+                                                 double* __restrict__ part,
+                                                 const double* __restrict__ mb) {
+  __shared__ double sh[256];
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = (int64_t)a.nx * a.ny;
+  const double m0 = mb ? *mb : 0.0;
+  double rr = 0.0, rs = 0.0, bb = 0.0, bs = 0.0;
+  if (idx < n) {
+    const int y = (int)(idx >> a.lgx), q = (int)(idx & (a.nx - 1));
+    const int64_t P = a.NXP;
+    const int64_t k = (int64_t)(y + 3) * P + q + 3;
+    const double bv = gval(b, k, a.NXP, gmode) - m0;
+    const double Ax = a.cx * (x[k - 1] + x[k + 1]) + a.cy * (x[k - P] + x[k + P]) -
+                      a.d * x[k];
+    const double rv = bv - Ax;
+    if (r) r[k] = rv;
+    rr = rv * rv;
+    rs = rv;
+    bb = bv * bv;
+    bs = bv;
+  }
+  rr = bsum256(rr, sh);
+  rs = bsum256(rs, sh);
+  bb = bsum256(bb, sh);
+  bs = bsum256(bs, sh);
+  if (threadIdx.x == 0) {
+    const int nb = gridDim.x;
+    part[blockIdx.x] = rr;
+    part[nb + blockIdx.x] = rs;
+    part[2 * nb + blockIdx.x] = bb;
+    part[3 * nb + blockIdx.x] = bs;
+  }
+}
+
+// one block: relative residual of the (projected) system into the solve
+// statistics [iterations, relres, converged, refinements | failed, max, solves]
+__global__ __launch_bounds__(256) void k_fft_stats(const double* __restrict__ part,
+                                                   int nb, double n, int singular,
+                                                   double rtol, int nref,
+                                                   int measured,
+                                                   double* __restrict__ st) {
+  __shared__ double sh[256];
+  double v[4];
+  for (int c = 0; c < 4; c++) {
+    double acc = 0.0;
+    if (measured)
+      for (int i = threadIdx.x; i < nb; i += 256) acc += part[c * nb + i];
+    v[c] = bsum256(acc, sh);
+  }
+  if (threadIdx.x != 0) return;
+  double rel = 0.0, conv = 1.0;
+  if (measured) {
+    double rr = v[0], bb = v[2];
+    if (singular) {
+      rr -= v[1] * v[1] / n;
+      bb -= v[3] * v[3] / n;
+    }
+    rr = rr > 0.0 ? rr : 0.0;
+    rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+    conv = (rel <= rtol) ? 1.0 : 0.0;
+  }
+  st[0] = (double)(nref + 1);
+  st[1] = rel;
+  st[2] = conv;
+  st[3] = (double)nref;
+  st[4] += conv > 0.0 ? 0.0 : 1.0;
+  if ((double)nref > st[5]) st[5] = (double)nref;
+  st[6] += 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct FftSolver {
+  FftArgs a{};
+  int P64 = 1, Q64 = 1, P32 = 1, Q32 = 1;
+  size_t lds_r64 = 0, lds_c64 = 0, lds_r32 = 0, lds_c32 = 0;
+  double* lamx = nullptr;
+  double* lamy = nullptr;
+  double2* twx64 = nullptr;
+  double2* twy64 = nullptr;
+  float2* twx32 = nullptr;
+  float2* twy32 = nullptr;
+  double2* S64 = nullptr;
+  float2* S32 = nullptr;
+  double* x64 = nullptr;
+  double* r64 = nullptr;
+  double* part = nullptr;
+  double* part_b = nullptr;
+  double* mean_b = nullptr;
+  int nbres = 0;
+  std::vector<void*> allocs;
+};
+
+static int ilog2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) l++;
+  return l;
+}
+
+static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet) {
+  return xper && yper && pow2(nx) && pow2(ny) && nx >= 8 && ny >= 8 &&
+         nx <= 4096 && ny <= 4096 && nwet == nx * ny;
+}
+
+template <class P>
+static P* falloc(FftSolver* f, size_t bytes) {
+  void* p = nullptr;
+  F2D_HIP(hipMalloc(&p, bytes < 256 ? 256 : bytes));
+  F2D_HIP(hipMemset(p, 0, bytes < 256 ? 256 : bytes));
+  f->allocs.push_back(p);
+  return reinterpret_cast<P*>(p);
+}
+
+template <class C>
+static C* twiddles(FftSolver* f, int N) {
+  std::vector<C> t(N);
+  for (int i = 0; i < N; i++) {
+    // exp(-2 pi i t / N), angle reduced to the first octant by symmetry
+    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * i / N;
+    t[i].x = (decltype(t[i].x))std::cos(ang);
+    t[i].y = (decltype(t[i].y))std::sin(ang);
+  }
+  C* d = falloc<C>(f, sizeof(C) * N);
+  F2D_HIP(hipMemcpy(d, t.data(), sizeof(C) * N, hipMemcpyHostToDevice));
+  return d;
+}
+
+static const size_t LDS_CAP = 64 * 1024;
+
+template <class T>
+static void plan_blocks(const FftArgs& a, int& P, int& Q, size_t& lr, size_t& lc) {
+  const size_t c = sizeof(T) * 2;
+  P = (int)(LDS_CAP / (2 * a.nx * c));
+  if (P < 1) P = 1;
+  if (P > 8) P = 8;
+  while (P > 1 && (a.ny / 2) % P) P >>= 1;
+  Q = (int)(LDS_CAP / (2 * a.ny * c));
+  if (Q < 1) Q = 1;
+  if (Q > 8) Q = 8;
+  lr = 2 * (size_t)P * a.nx * c;
+  lc = 2 * (size_t)Q * a.ny * c;
+}
+
+template <class T>
+static void set_lds(size_t lr, size_t lc) {
+  if (lr > 64 * 1024) {
+    F2D_HIP(hipFuncSetAttribute((const void*)k_fft_rows_fwd<T, double>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
+    F2D_HIP(hipFuncSetAttribute((const void*)k_fft_rows_fwd<T, float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
+    F2D_HIP(hipFuncSetAttribute((const void*)k_fft_rows_inv<T, T, 0>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
+    F2D_HIP(hipFuncSetAttribute((const void*)k_fft_rows_inv<T, T, 1>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
+    F2D_HIP(hipFuncSetAttribute((const void*)k_fft_rows_inv<T, T, 2>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
+  }
+  if (lc > 64 * 1024)
+    F2D_HIP(hipFuncSetAttribute((const void*)k_fft_cols<T>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc));
+}
+
+FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
+                      double maindiag, bool f64, bool f32) {
+  FftSolver* f = new FftSolver();
+  try {
+    FftArgs& a = f->a;
+    a.nx = (int)nx;
+    a.ny = (int)ny;
+    a.lgx = ilog2(nx);
+    a.lgy = ilog2(ny);
+    a.nk = (int)(nx / 2 + 1);
+    a.NXP = (int)(nx + 6);
+    a.cx = dy / dx;
+    a.cy = dx / dy;
+    a.d = 2.0 * (a.cx + a.cy) + maindiag;
+    a.singular = maindiag == 0.0 ? 1 : 0;
+    const long double pi = 3.14159265358979323846264338327950288L;
+    std::vector<double> lx(a.nk), ly(ny);
+    for (int k = 0; k < a.nk; k++)
+      lx[k] = (double)(2.0L * a.cx * (std::cos(2.0L * pi * k / nx) - 1.0L));
+    for (int k = 0; k < ny; k++)
+      ly[k] = (double)(2.0L * a.cy * (std::cos(2.0L * pi * k / ny) - 1.0L)) - maindiag;
+    f->lamx = falloc<double>(f, sizeof(double) * a.nk);
+    f->lamy = falloc<double>(f, sizeof(double) * ny);
+    F2D_HIP(hipMemcpy(f->lamx, lx.data(), sizeof(double) * a.nk, hipMemcpyHostToDevice));
+    F2D_HIP(hipMemcpy(f->lamy, ly.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
+    const size_t nspec = (size_t)a.nk * ny;
+    const size_t npad = (size_t)(ny + 6) * (nx + 6);
+    f->nbres = (int)((nx * ny + 255) / 256);
+    f->part = falloc<double>(f, sizeof(double) * 4 * f->nbres);
+    f->part_b = falloc<double>(f, sizeof(double) * f->nbres);
+    f->mean_b = falloc<double>(f, sizeof(double));
+    if (f64) {
+      f->twx64 = twiddles<double2>(f, (int)nx);
+      f->twy64 = twiddles<double2>(f, (int)ny);
+      f->S64 = falloc<double2>(f, sizeof(double2) * nspec);
+      plan_blocks<double>(a, f->P64, f->Q64, f->lds_r64, f->lds_c64);
+      F2D_REQUIRE(f->lds_r64 <= 160 * 1024 && f->lds_c64 <= 160 * 1024,
+                  "FFT rows do not fit in LDS");
+      set_lds<double>(f->lds_r64, f->lds_c64);
+    }
+    if (f32) {
+      f->twx32 = twiddles<float2>(f, (int)nx);
+      f->twy32 = twiddles<float2>(f, (int)ny);
+      f->S32 = falloc<float2>(f, sizeof(float2) * nspec);
+      f->x64 = falloc<double>(f, sizeof(double) * npad);
+      f->r64 = falloc<double>(f, sizeof(double) * npad);
+      plan_blocks<float>(a, f->P32, f->Q32, f->lds_r32, f->lds_c32);
+      set_lds<float>(f->lds_r32, f->lds_c32);
+    }
+    F2D_HIP(hipDeviceSynchronize());
+  } catch (...) {
+    fft_destroy(f);
+    throw;
+  }
+  return f;
+}
+
+void fft_destroy(FftSolver* f) {
+  if (!f) return;
+  (void)hipDeviceSynchronize();
+  for (void* p : f->allocs) (void)hipFree(p);
+  delete f;
+}
+
+// one spectral solve: rows forward, columns, rows inverse
+template <class T, class TB, class TX, int MODE>
+static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
+                     hipStream_t s) {
+  FftArgs a = f->a;
+  const bool d = sizeof(T) == 8;
+  a.P = d ? f->P64 : f->P32;
+  a.Q = d ? f->Q64 : f->Q32;
+  const size_t lr = d ? f->lds_r64 : f->lds_r32, lc = d ? f->lds_c64 : f->lds_c32;
+  cplx<T>* S = d ? (cplx<T>*)f->S64 : (cplx<T>*)f->S32;
+  const cplx<T>* twx = d ? (const cplx<T>*)f->twx64 : (const cplx<T>*)f->twx32;
+  const cplx<T>* twy = d ? (const cplx<T>*)f->twy64 : (const cplx<T>*)f->twy32;
+  const int gr = a.ny / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  k_fft_rows_fwd<T, TB><<<gr, 256, lr, s>>>(a, b, gmode, S, twx);
+  k_fft_cols<T><<<gc, 256, lc, s>>>(a, S, twy, f->lamx, f->lamy);
+  k_fft_rows_inv<T, TX, MODE><<<gr, 256, lr, s>>>(a, S, twx, x, x64);
+}
+
+template <class TB>
+static void residual(FftSolver* f, const TB* b, int gmode, const double* x,
+                     double* r, hipStream_t s, const double* mb = nullptr) {
+  k_fft_res<TB><<<f->nbres, 256, 0, s>>>(f->a, b, gmode, x, r, f->part, mb);
+}
+
+// FP64 mean of b (singular operator), for the measured residual
+template <class TB>
+static const double* mean_of(FftSolver* f, const TB* b, int gmode, hipStream_t s) {
+  if (!f->a.singular) return nullptr;
+  k_fft_bsum<TB><<<f->nbres, 256, 0, s>>>(f->a, b, gmode, f->part_b);
+  k_fft_mean<<<1, 256, 0, s>>>(f->part_b, f->nbres, (double)f->a.nx * f->a.ny,
+                               f->mean_b);
+  return f->mean_b;
+}
+
+void fft_solve_f64(FftSolver* f, const double* b, int gmode, double* x,
+                   double rtol, bool measure, double* stats, hipStream_t s) {
+  F2D_REQUIRE(f->S64, "FFT solver built without FP64 buffers");
+  spectral<double, double, double, 0>(f, b, gmode, x, nullptr, s);
+  if (measure) residual<double>(f, b, gmode, x, nullptr, s, mean_of(f, b, gmode, s));
+  if (stats)
+    k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.nx * f->a.ny,
+                                  f->a.singular, rtol, 0, measure ? 1 : 0, stats);
+}
+
+void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
+                          int nref, double rtol, bool measure, double* stats,
+                          hipStream_t s) {
+  F2D_REQUIRE(f->S32, "FFT solver built without FP32 buffers");
+  if (nref <= 0) {
+    spectral<float, float, float, 0>(f, b, gmode, x, nullptr, s);
+    if (measure) {
+      // the FP32 result measured in FP64 needs an FP64 copy of it
+      spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s);
+      residual<float>(f, b, gmode, f->x64, nullptr, s, mean_of(f, b, gmode, s));
+    }
+  } else {
+    // the residual fed to the FP32 transforms must not carry the mean of b
+    // (singular operator): next to it the FP32 rounding would erase the
+    // residual itself
+    const double* mb = mean_of(f, b, gmode, s);
+    spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s);
+    for (int i = 0; i < nref; i++) {
+      residual<float>(f, b, gmode, f->x64, f->r64, s, mb);
+      spectral<float, double, float, 2>(f, f->r64, 0, x, f->x64, s);
+    }
+    if (measure) residual<float>(f, b, gmode, f->x64, nullptr, s, mb);
+  }
+  if (stats)
+    k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.nx * f->a.ny,
+                                  f->a.singular, rtol, nref, measure ? 1 : 0, stats);
+}
+
+}  // namespace f2d

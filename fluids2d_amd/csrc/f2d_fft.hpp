@@ -1,0 +1,40 @@
+// f2d_fft.hpp -- direct spectral solve of the doubly periodic 5-point
+// Poisson/Helmholtz operator (internal interface of the library).
+//
+// When every interior vertex is an unknown and both directions are periodic
+// (elliptic.py:114-195 with the y-periodic extension), the operator
+//   (A x)_P = cx (x_E + x_W) + cy (x_N + x_S) - (2 (cx + cy) + maindiag) x_P
+// is diagonalised by the 2-D DFT, eigenvalues
+//   lambda(kx, ky) = 2 cx (cos(2 pi kx / nx) - 1) + 2 cy (cos(2 pi ky / ny) - 1)
+//                    - maindiag,
+// so x = IDFT(DFT(b) / lambda), the k = 0 mode of the singular operator set
+// to zero (the zero-mean solution the iterative solver also returns).
+#pragma once
+#include "f2d_common.hpp"
+
+namespace f2d {
+
+struct FftSolver;
+
+// nx, ny powers of two in [8, 4096], all interior vertices unknown, doubly
+// periodic
+bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet);
+
+FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
+                      double maindiag, bool f64, bool f32);
+void fft_destroy(FftSolver* f);
+
+// x = A^-1 b on the padded (ny+6) x (nx+6) layout, halo filled.  gmode 0:
+// b given at the vertices; 1: b = centerstovertices(omega) of the cell
+// field passed as `b`.  FP64 throughout.  With `stats` (8 doubles, device)
+// the relative residual of the result is measured and recorded.
+void fft_solve_f64(FftSolver* f, const double* b, int gmode, double* x,
+                   double rtol, bool measure, double* stats, hipStream_t s);
+
+// FP32 transforms with FP64 iterative refinement: x64 = S32(b), then
+// `nref` times r = b - A x64 (FP64), x64 += S32(r); x = (float) x64.
+void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
+                          int nref, double rtol, bool measure, double* stats,
+                          hipStream_t s);
+
+}  // namespace f2d

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "f2d_common.hpp"
+#include "f2d_fft.hpp"
 
 namespace f2d {
 
@@ -288,42 +289,76 @@ __global__ __launch_bounds__(BS) void k_down(DownArgs<T> a) {
       if (!a.first && a.inner) c->x_valid = 1;
     }
   }
-  for (int k = t; k < W * W; k += BS) {
-    int ly = k / W, lx = k - ly * W;
-    int y = fy0 - R + ly, x = fx0 - R + lx;
-    T rv = T(0);
-    int8_t mv = 0;
-    if (lw(F, y, x)) {
-      int64_t g = li(F, y, x);
-      mv = F.m[g];
-      if (mv) {
-        rv = a.r_in[g];
-        if (CG) rv = a.first ? rv - shift : rv - alpha * (a.q[g] - mq);
-      }
+  // window loads: all issued before any use (no dependent chains)
+  {
+    constexpr int NLD = (W * W + BS - 1) / BS;
+    int64_t gi[NLD];
+    bool ok[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int k = t + i * BS;
+      const int ly = k / W, lx = k - ly * W;
+      int y = fy0 - R + ly, x = fx0 - R + lx;
+      ok[i] = k < W * W && lw(F, y, x);
+      gi[i] = ok[i] ? li(F, y, x) : F.origin;
     }
-    sr[k] = rv;
-    sm[k] = mv;
-    sz[k] = T(0);
+    int8_t mv[NLD];
+    T rv[NLD], qv[NLD];
+    const bool useq = CG && !a.first;
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      mv[i] = ok[i] ? F.m[gi[i]] : int8_t(0);
+      rv[i] = ok[i] ? a.r_in[gi[i]] : T(0);
+      qv[i] = (useq && ok[i]) ? a.q[gi[i]] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int k = t + i * BS;
+      if (k >= W * W) continue;
+      T v = T(0);
+      if (mv[i]) {
+        v = rv[i];
+        if (CG) v = a.first ? v - shift : v - alpha * (qv[i] - mq);
+      }
+      sr[k] = v;
+      sm[k] = mv[i];
+      sz[k] = T(0);
+    }
   }
   __syncthreads();
 
   double rr = 0.0;
   if (CG) {
-    for (int e = t; e < FC * FC; e += BS) {
-      int cy = e / FC, cx = e - cy * FC;
-      int y = fy0 + cy, x = fx0 + cx;
-      if (y >= F.ny || x >= F.nx) continue;
-      int64_t g = li(F, y, x);
-      int k = (cy + R) * W + (cx + R);
-      T rv = sr[k];
-      a.r_out[g] = rv;
+    constexpr int NC = (FC * FC + BS - 1) / BS;
+    int64_t gi[NC];
+    bool ok[NC];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      const int e = t + i * BS;
+      const int cy = e / FC, cx = e - cy * FC;
+      const int y = fy0 + cy, x = fx0 + cx;
+      ok[i] = e < FC * FC && y < F.ny && x < F.nx;
+      gi[i] = ok[i] ? li(F, y, x) : F.origin;
+    }
+    T pv[NC], xv[NC];
+    const bool upd = !zero_x && !a.first;
+    const bool xold = upd && !(a.inner && a.iter == 1);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      pv[i] = (upd && ok[i]) ? a.p[gi[i]] : T(0);
+      xv[i] = (xold && ok[i]) ? a.x[gi[i]] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      if (!ok[i]) continue;
+      const int e = t + i * BS;
+      const int cy = e / FC, cx = e - cy * FC;
+      const int k = (cy + R) * W + (cx + R);
+      const T rv = sr[k];
+      a.r_out[gi[i]] = rv;
       if (sm[k]) {
-        if (zero_x) {
-          a.x[g] = T(0);
-        } else if (!a.first) {
-          T pv = a.p[g];
-          a.x[g] = (a.inner && a.iter == 1) ? alpha * pv : a.x[g] + alpha * pv;
-        }
+        if (zero_x) a.x[gi[i]] = T(0);
+        else if (upd) a.x[gi[i]] = xold ? xv[i] + alpha * pv[i] : alpha * pv[i];
         rr += (double)rv * (double)rv;
       }
     }
@@ -429,32 +464,53 @@ __global__ __launch_bounds__(BS) void k_up(UpArgs<T> a) {
     if (conv) return;
   }
   const int fy0 = blockIdx.y * FC, fx0 = blockIdx.x * FC;
-  for (int k = t; k < W * W; k += BS) {
-    int ly = k / W, lx = k - ly * W;
-    int y = fy0 - R + ly, x = fx0 - R + lx;
-    T rv = T(0), zv = T(0);
-    int8_t mv = 0;
-    if (lw(F, y, x)) {
-      int64_t g = li(F, y, x);
-      mv = F.m[g];
-      if (mv) {
-        rv = a.r[g];
-        zv = a.zpre[g];
-      }
-    }
-    sr[k] = rv;
-    sx[k] = zv;
-    sm[k] = mv;
-  }
   const int cyb = fy0 / 2 - NU - 1, cxb = fx0 / 2 - NU - 1;
-  if (COARSE) {
-    const LevelDev& C = a.C;
-    for (int k = t; k < CW * CW; k += BS) {
-      int ly = k / CW, lx = k - ly * CW;
+  {
+    constexpr int NLD = (W * W + BS - 1) / BS;
+    constexpr int NCL = COARSE ? (CW * CW + BS - 1) / BS : 0;
+    int64_t gi[NLD];
+    bool ok[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int k = t + i * BS;
+      const int ly = k / W, lx = k - ly * W;
+      int y = fy0 - R + ly, x = fx0 - R + lx;
+      ok[i] = k < W * W && lw(F, y, x);
+      gi[i] = ok[i] ? li(F, y, x) : F.origin;
+    }
+    int64_t ci[NCL > 0 ? NCL : 1];
+    bool cok[NCL > 0 ? NCL : 1];
+#pragma unroll
+    for (int i = 0; i < NCL; i++) {
+      const LevelDev& C = a.C;
+      const int k = t + i * BS;
+      const int ly = k / CW, lx = k - ly * CW;
       int y = cyb + ly, x = cxb + lx;
-      T v = T(0);
-      if (lw(C, y, x)) v = a.ec[li(C, y, x)];
-      sc[k] = v;
+      cok[i] = k < CW * CW && lw(C, y, x);
+      ci[i] = cok[i] ? li(C, y, x) : C.origin;
+    }
+    int8_t mv[NLD];
+    T rv[NLD], zv[NLD], cv[NCL > 0 ? NCL : 1];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      mv[i] = ok[i] ? F.m[gi[i]] : int8_t(0);
+      rv[i] = ok[i] ? a.r[gi[i]] : T(0);
+      zv[i] = ok[i] ? a.zpre[gi[i]] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NCL; i++) cv[i] = cok[i] ? a.ec[ci[i]] : T(0);
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int k = t + i * BS;
+      if (k >= W * W) continue;
+      sr[k] = mv[i] ? rv[i] : T(0);
+      sx[k] = mv[i] ? zv[i] : T(0);
+      sm[k] = mv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NCL; i++) {
+      const int k = t + i * BS;
+      if (k < CW * CW) sc[k] = cv[i];
     }
   }
   __syncthreads();
@@ -541,17 +597,6 @@ struct PArgs {
 };
 
 template <class T>
-__device__ __forceinline__ T pnew_at(const PArgs<T>& a, int y, int x, T beta,
-                                     T mz) {
-  if (!lw(a.L, y, x)) return T(0);
-  int64_t k = li(a.L, y, x);
-  if (!a.L.m[k]) return T(0);
-  T v = a.z[k] - mz;
-  if (!a.first) v = v + beta * a.p_old[k];
-  return v;
-}
-
-template <class T>
 __global__ __launch_bounds__(BS) void k_pupdate(PArgs<T> a) {
   __shared__ double sh[BS];
   if (stop(a.ctl, a.outer)) return;
@@ -577,22 +622,55 @@ __global__ __launch_bounds__(BS) void k_pupdate(PArgs<T> a) {
   const T beta = T(bet), mz = T(mzd);
   int y0, x0;
   tile_origin<FC0>(y0, x0);
+  // p on the tile and its one-point ring, in LDS
+  constexpr int W = FC0 + 2, NLD = (W * W + BS - 1) / BS;
+  __shared__ T sp[W * W];
+  {
+    int64_t gi[NLD];
+    bool ok[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int k = threadIdx.x + i * BS;
+      const int ly = k / W, lx = k - ly * W;
+      int y = y0 - 1 + ly, x = x0 - 1 + lx;
+      ok[i] = k < W * W && lw(L, y, x);
+      gi[i] = ok[i] ? li(L, y, x) : L.origin;
+    }
+    int8_t mv[NLD];
+    T zv[NLD], pv[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      mv[i] = ok[i] ? L.m[gi[i]] : int8_t(0);
+      zv[i] = ok[i] ? a.z[gi[i]] : T(0);
+      pv[i] = (ok[i] && !a.first) ? a.p_old[gi[i]] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+      const int k = threadIdx.x + i * BS;
+      if (k >= W * W) continue;
+      T v = zv[i] - mz;
+      if (!a.first) v = v + beta * pv[i];
+      sp[k] = mv[i] ? v : T(0);
+    }
+  }
+  __syncthreads();
   double pq = 0.0, qs = 0.0;
   for (int e = threadIdx.x; e < FC0 * FC0; e += BS) {
-    int y = y0 + e / FC0, x = x0 + e % FC0;
+    const int cy = e / FC0, cx = e - cy * FC0;
+    const int y = y0 + cy, x = x0 + cx;
     if (y >= L.ny || x >= L.nx) continue;
-    int64_t k = li(L, y, x);
-    T pc = T(0), qv = T(0);
-    if (L.m[k]) {
-      pc = pnew_at(a, y, x, beta, mz);
-      T w = pnew_at(a, y, x - 1, beta, mz), e2 = pnew_at(a, y, x + 1, beta, mz);
-      T s = pnew_at(a, y - 1, x, beta, mz), n = pnew_at(a, y + 1, x, beta, mz);
-      qv = T(L.d) * pc - T(L.cx) * (w + e2) - T(L.cy) * (s + n);
+    const int64_t g = li(L, y, x);
+    const int k = (cy + 1) * W + cx + 1;
+    const T pc = sp[k];
+    T qv = T(0);
+    if (L.m[g]) {
+      qv = T(L.d) * pc - T(L.cx) * (sp[k - 1] + sp[k + 1]) -
+           T(L.cy) * (sp[k - W] + sp[k + W]);
       pq += (double)pc * (double)qv;
       qs += (double)qv;
     }
-    a.p[k] = pc;
-    a.q[k] = qv;
+    a.p[g] = pc;
+    a.q[g] = qv;
   }
   double b0 = block_sum<BS>(pq, sh);
   double b1 = block_sum<BS>(qs, sh);
@@ -1429,6 +1507,7 @@ struct f2d_poisson {
   int nu_coarse = 2, nu_tail = 2;  // smoothing sweeps: multi-block coarse
                                    // levels, single-workgroup tail
   double inner_rtol = 1e-4;
+  f2d::FftSolver* fft = nullptr;  // direct spectral solve, when it applies
   std::vector<void*> allocs;
   double* reg(int r) { return part + (int64_t)r * region; }
 };
@@ -2213,10 +2292,20 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
         h->e32 = (float*)dalloc(h, N * sizeof(float));
       }
     }
+    const bool fft_ok = fft_applicable(h->ny, h->nx, h->xper, h->yper, nwet);
+    F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT || fft_ok,
+                "FFT solver requested but the operator is not fully wet, "
+                "doubly periodic with power-of-two sizes <= 4096");
+    F2D_REQUIRE(h->opts.solver >= 0 && h->opts.solver <= 2, "opts.solver unknown");
+    if (fft_ok && h->opts.solver != F2D_SOLVER_MG)
+      h->fft = fft_create(h->ny, h->nx, g->dx, g->dy, maindiag,
+                          h->opts.mode == F2D_POISSON_F64,
+                          h->opts.mode != F2D_POISSON_F64);
     F2D_HIP(hipDeviceSynchronize());
     *out = h;
   });
   if (rc != 0 && h) {
+    if (h->fft) fft_destroy(h->fft);
     for (void* p : h->allocs) (void)hipFree(p);
     delete h;
   }
@@ -2227,6 +2316,7 @@ int f2d_poisson_destroy(f2d_poisson* h) {
   return guarded([&] {
     if (!h) return;
     F2D_HIP(hipDeviceSynchronize());
+    if (h->fft) fft_destroy(h->fft);
     for (void* p : h->allocs) F2D_HIP(hipFree(p));
     delete h;
   });
@@ -2243,10 +2333,23 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
     hipStream_t s = S(stream);
     bool sync = iters != nullptr || relres != nullptr;
     const int mode = h->opts.mode;
-    if (mode == F2D_POISSON_F64)
+    if (h->fft) {
+      Ctl* c = mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
+      if (mode == F2D_POISSON_F64) {
+        fft_solve_f64(h->fft, (const double*)b, gmode, (double*)x, rtol, sync,
+                      c->stats, s);
+      } else {
+        const int nref = mode == F2D_POISSON_F32
+                             ? 0
+                             : (h->opts.fft_refine > 0 ? h->opts.fft_refine : 1);
+        fft_solve_f32_refine(h->fft, (const float*)b, gmode, (float*)x, nref, rtol,
+                             sync, c->stats, s);
+      }
+    } else if (mode == F2D_POISSON_F64) {
       do_solve<double>(h, (const double*)b, x, gmode, rtol, maxit, sync, s);
-    else
+    } else {
       do_solve<float>(h, (const float*)b, x, gmode, rtol, maxit, sync, s);
+    }
     if (sync) {
       Ctl* c = mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
       double st[8];

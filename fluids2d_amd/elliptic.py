@@ -1,10 +1,12 @@
 """Poisson2D: masked Poisson/Helmholtz solve on the vertex grid
-(elliptic.py:71-203), by the library's multigrid-preconditioned CG.
+(elliptic.py:71-203), by the library's multigrid-preconditioned CG, or by its
+direct FFT solve when the operator is fully wet and doubly periodic.
 
 The reference factorises the masked 5-point operator with SuperLU once per
 ``mesh.finalize()`` and solves with it at every stage.  Here the operator is
-matrix-free; ``create`` builds the multigrid hierarchy from the mask and every
-solve is warm-started from the previous solution held in ``x``.
+matrix-free; ``create`` builds the multigrid hierarchy from the mask (and the
+spectral tables when they apply) and every iterative solve is warm-started
+from the previous solution held in ``x``.
 """
 import ctypes
 
@@ -36,6 +38,7 @@ class Poisson2D:
         opts = _lib.PoissonOpts()
         opts.mode = _lib.POISSON_F64 if self.precision == "fp64" else _lib.POISSON_F32_REFINE
         opts.nu = int(getattr(p, "poisson_nu", 2) or 2)
+        opts.solver = _lib.SOLVERS[getattr(p, "poisson_solver", "auto") or "auto"]
         rt = getattr(p, "poisson_rtol", None)
         mi = getattr(p, "poisson_maxit", None)
         self.rtol = DEFAULT_RTOL[self.precision] if rt is None else float(rt)

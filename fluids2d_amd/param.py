@@ -12,6 +12,9 @@ Fields added for the MI355X path (declared in the constructor, so that
 * ``poisson_maxit`` iteration cap of the Poisson solves (None: 100 in FP64,
                     4 refinements in FP32)
 * ``poisson_nu``    red-black Gauss-Seidel sweeps before/after each level
+* ``poisson_solver`` "auto" (direct FFT solve for the fully wet doubly
+                    periodic operator with power-of-two sizes, multigrid CG
+                    otherwise), "mg" or "fft"
 * ``device``        torch device index of the GPU
 """
 _models = ["euler", "eulerpsi", "advection", "vectoradv",
@@ -84,6 +87,7 @@ class Param:
         self.poisson_rtol = None
         self.poisson_maxit = None
         self.poisson_nu = 2
+        self.poisson_solver = "auto"
         self.device = 0
 
         self.__parameters__ = get_parameters(self)

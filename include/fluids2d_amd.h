@@ -178,7 +178,17 @@ typedef struct f2d_poisson_opts {
   int32_t tail_threads;  /* workgroup size of the tail kernel (256/512/1024) */
   int32_t nu_coarse;     /* sweeps on the multi-block coarse levels (0: nu) */
   int32_t nu_tail;       /* sweeps on the tail levels (0: nu)              */
+  int32_t solver;        /* F2D_SOLVER_*: 0 auto, 1 multigrid CG, 2 FFT      */
+  int32_t fft_refine;    /* FFT in FP32 modes: FP64 refinements (0: 1)     */
 } f2d_poisson_opts;
+
+/* Solver selection.  AUTO takes the direct spectral (FFT) solve when every
+ * interior vertex is an unknown, both directions are periodic and nx, ny
+ * are powers of two in [8, 4096] (the operator is then diagonalised by the
+ * 2-D DFT), and the multigrid-preconditioned CG otherwise. */
+#define F2D_SOLVER_AUTO 0
+#define F2D_SOLVER_MG 1
+#define F2D_SOLVER_FFT 2
 
 /* unknowns_host: int8, padded shape, 1 at unknowns (elliptic.py:102-111:
  * mskv with the periodic halos cleared).  opts may be NULL (defaults). */

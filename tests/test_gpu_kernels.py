@@ -227,29 +227,35 @@ def test_point_operators_bitexact(dev, L, xper, yper):
 # ---------------------------------------------------------------------------
 # Poisson / Helmholtz
 # ---------------------------------------------------------------------------
-def _poisson_case(dev, L, p, m, maindiag, mode, nu=2, seed=0):
+def _poisson_case(dev, L, p, m, maindiag, mode, nu=2, seed=0, solver="mg", c2v=False):
     import torch
     mesh = O.Mesh(p, msk=m)
     unk = O.unknown_mask(p, mesh.mskv)
     rng = np.random.default_rng(seed)
     b = rng.standard_normal(m.shape) * mesh.mskv
+    if c2v:  # the fused form takes a cell field; the oracle its vertex average
+        w = rng.standard_normal(m.shape) * mesh.msk
+        O.fill(p, w)
+        b = np.zeros(m.shape)
+        O.centerstovertices(mesh, w, b)
     x_ref = np.zeros(m.shape)
     ref = O.Poisson2D(p, mesh, "v", maindiag=maindiag)
     ref.solve(b, x_ref)
     g = L.make_grid(p)
     opts = L.PoissonOpts()
     opts.mode, opts.nu = mode, nu
+    opts.solver = L.SOLVERS[solver]
     h = ctypes.c_void_p()
     L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
            maindiag, ctypes.byref(opts), ctypes.byref(h))
     dt = torch.float64 if mode == L.POISSON_F64 else torch.float32
-    db = T(b, dev).to(dt)
+    db = T(w if c2v else b, dev).to(dt)
     dx = torch.zeros(m.shape, dtype=dt, device=dev)
     it, rr = ctypes.c_int(), ctypes.c_double()
     rtol = 1e-13 if mode == L.POISSON_F64 else (1e-10 if mode == L.POISSON_F32_REFINE else 1e-5)
     maxit = 200 if mode != L.POISSON_F32_REFINE else 6
-    L.call("f2d_poisson_solve", h, L.ptr(db), L.ptr(dx), rtol, maxit, ctypes.byref(it),
-           ctypes.byref(rr), L.stream_ptr())
+    L.call("f2d_poisson_solve_c2v" if c2v else "f2d_poisson_solve", h, L.ptr(db), L.ptr(dx),
+           rtol, maxit, ctypes.byref(it), ctypes.byref(rr), L.stream_ptr())
     x = N(dx).astype(np.float64)
     L.call("f2d_poisson_destroy", h)
     err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
@@ -287,6 +293,62 @@ def test_poisson_periodic_vs_fft(dev, L):
     xf = O.fft_poisson_periodic(b[3:-3, 3:-3], 1 / nx, 1 / ny)
     e2 = np.linalg.norm(x[3:-3, 3:-3] - xf) / np.linalg.norm(xf)
     assert e2 < 1e-10, e2
+
+
+@pytest.mark.parametrize("ny,nx,Ly,maindiag,c2v", [
+    (128, 128, 1.3, 0.0, False), (64, 256, 0.7, 0.0, False), (256, 64, 3.0, 0.37, False),
+    (8, 16, 0.2, 0.0, False), (512, 512, 8.0, 0.0, True), (32, 1024, 0.9, 1.5, True)])
+def test_poisson_fft_f64_vs_superlu(dev, L, ny, nx, Ly, maindiag, c2v):
+    """Direct spectral solve of the fully wet doubly periodic operator (and
+    its Helmholtz form, dx != dy) against SuperLU on the same matrix."""
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True,
+          Lx=nx / 64.0, Ly=Ly)
+    m = O.default_mask(ny, nx, 3, True, True)
+    err, it, rr, x, xr, unk = _poisson_case(dev, L, p, m, maindiag, L.POISSON_F64,
+                                            solver="fft", c2v=c2v)
+    print(f"\nfft {ny}x{nx} md={maindiag} c2v={c2v}: relres={rr:.2e} err={err:.2e}")
+    assert rr <= 1e-12 and err <= 1e-11
+    # halo images are the periodic copies (the reference fills them after solving)
+    xx = x.copy()
+    O.fill(p, xx)
+    assert np.array_equal(xx, x)
+
+
+@pytest.mark.parametrize("mode,ny,nx", [("refine", 512, 512), ("refine", 64, 128),
+                                        ("f32", 256, 256)])
+def test_poisson_fft_f32_modes(dev, L, mode, ny, nx):
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True,
+          Lx=nx / 256.0, Ly=ny / 256.0)
+    m = O.default_mask(ny, nx, 3, True, True)
+    md = L.POISSON_F32_REFINE if mode == "refine" else L.POISSON_F32
+    err, it, rr, x, xr, unk = _poisson_case(dev, L, p, m, 0.0, md, solver="fft")
+    print(f"\nfft {mode} {ny}x{nx}: relres={rr:.2e} err={err:.2e}")
+    if mode == "refine":
+        assert rr <= 1e-10 and err < 1e-6
+    else:
+        assert err < 1e-5
+
+
+def test_poisson_fft_matches_mg(dev, L):
+    p = P(model="eulerpsi", nx=256, ny=256, xperiodic=True, yperiodic=True)
+    m = O.default_mask(256, 256, 3, True, True)
+    e1, _, _, x1, _, _ = _poisson_case(dev, L, p, m, 0.0, L.POISSON_F64, solver="fft", seed=4)
+    e2, _, _, x2, _, _ = _poisson_case(dev, L, p, m, 0.0, L.POISSON_F64, solver="mg", seed=4)
+    assert np.linalg.norm(x1 - x2) / np.linalg.norm(x2) < 1e-11
+
+
+def test_poisson_fft_refuses_masked(dev, L):
+    p = P(model="eulerpsi", nx=64, ny=64, xperiodic=True, yperiodic=True)
+    m = random_mask(64, 64, 3, True, True, nisl=2)
+    mesh = O.Mesh(p, msk=m)
+    unk = O.unknown_mask(p, mesh.mskv)
+    g = L.make_grid(p)
+    opts = L.PoissonOpts()
+    opts.solver = L.SOLVERS["fft"]
+    h = ctypes.c_void_p()
+    with pytest.raises(L.F2DError, match="FFT solver requested"):
+        L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+               0.0, ctypes.byref(opts), ctypes.byref(h))
 
 
 @pytest.mark.parametrize("mode,ny,nx,xper,yper,islands", [
