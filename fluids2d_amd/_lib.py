@@ -24,6 +24,7 @@ EXPORTS = [
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
     "f2d_poisson_solve_c2v", "f2d_poisson_stats", "f2d_poisson_reset_stats",
     "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
+    "f2d_poisson_method",
 ]
 
 
@@ -91,6 +92,7 @@ def lib():
         "f2d_poisson_create": (i32, [vp, gp, d, ctypes.POINTER(PoissonOpts),
                                      ctypes.POINTER(vp)]),
         "f2d_poisson_destroy": (i32, [vp]),
+        "f2d_poisson_method": (i32, [vp]),
         "f2d_poisson_solve": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
                                     ctypes.POINTER(d), vp]),
         "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),

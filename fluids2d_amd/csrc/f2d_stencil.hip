@@ -174,56 +174,115 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
     const int by = blockIdx.x / a.tiles_x, bx = blockIdx.x - by * a.tiles_x;
     const int j0 = 3 + by * TY, i0 = 3 + bx * TX;
     const int NYP = a.ny + 6, NXP = a.nx + 6;
-    for (int k = t; k < (TY + 6) * LW; k += 256) {
-      int r = k / LW, c = k - r * LW;
-      int gj = j0 - 3 + r, gi = i0 - 3 + c;
-      sq[k] = (gj < NYP && gi < NXP) ? a.q[(int64_t)gj * P + gi] : T(0);
+    // Every global load of the tile is issued here, before any is used, so
+    // that their latencies overlap (no load waits on another load).
+    constexpr int NQ = ((TY + 6) * LW + 255) / 256;
+    constexpr int NFX = (TY * (TX + 1) + 255) / 256;
+    constexpr int NFY = ((TY + 1) * TX + 255) / 256;
+    constexpr int NCL = (TY * TX + 255) / 256;
+    T qv[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+      const int k = t + i * 256;
+      const int r = k / LW, c = k - r * LW;
+      const int gj = j0 - 3 + r, gi = i0 - 3 + c;
+      const bool ok = k < (TY + 6) * LW && gj < NYP && gi < NXP;
+      qv[i] = ok ? a.q[(int64_t)gj * P + gi] : T(0);
+    }
+    int8_t ox[NFX], oy[NFY];
+    T ux[NFX], uy[NFY];
+#pragma unroll
+    for (int i = 0; i < NFX; i++) {
+      const int f = t + i * 256;
+      const int r = f / (TX + 1), c = f - r * (TX + 1);
+      const int gj = j0 + r, gi = i0 + c;
+      const bool ok = f < TY * (TX + 1) && gj < 3 + a.ny && gi <= 3 + a.nx;
+      const int64_t k = ok ? (int64_t)gj * P + gi : 0;
+      ox[i] = ok ? a.ocx[k] : int8_t(0);
+      ux[i] = ok ? a.Ux[k] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NFY; i++) {
+      const int f = t + i * 256;
+      const int r = f / TX, c = f - r * TX;
+      const int gj = j0 + r, gi = i0 + c;
+      const bool ok = f < (TY + 1) * TX && gj <= 3 + a.ny && gi < 3 + a.nx;
+      const int64_t k = ok ? (int64_t)gj * P + gi : 0;
+      oy[i] = ok ? a.ocy[k] : int8_t(0);
+      uy[i] = ok ? a.Uy[k] : T(0);
+    }
+    // the stage's earlier tendencies, as many as its coefficients need
+    double cf[3] = {0.0, 0.0, 0.0};
+    const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, *a.dt, cf);
+    int8_t mv[NCL];
+    T d0[NCL], d1[NCL];
+#pragma unroll
+    for (int i = 0; i < NCL; i++) {
+      const int e = t + i * 256;
+      const int r = e / TX, c = e - r * TX;
+      const int gj = j0 + r, gi = i0 + c;
+      const bool ok = e < TY * TX && gj < 3 + a.ny && gi < 3 + a.nx;
+      const int64_t k = ok ? (int64_t)gj * P + gi : 0;
+      mv[i] = ok ? a.msk[k] : int8_t(0);
+      d0[i] = (ok && nc > 1) ? a.ds0[k] : T(0);
+      d1[i] = (ok && nc > 2) ? a.ds1[k] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+      const int k = t + i * 256;
+      if (k < (TY + 6) * LW) sq[k] = qv[i];
     }
     __syncthreads();
     // x faces: rows j0..j0+TY-1, faces i0..i0+TX
-    for (int f = t; f < TY * (TX + 1); f += 256) {
-      int r = f / (TX + 1), c = f - r * (TX + 1);
-      int gj = j0 + r, gi = i0 + c;
+#pragma unroll
+    for (int i = 0; i < NFX; i++) {
+      const int f = t + i * 256;
+      if (f >= TY * (TX + 1)) continue;
+      const int r = f / (TX + 1), c = f - r * (TX + 1);
       T v = T(0);
-      if (gj < 3 + a.ny && gi <= 3 + a.nx) {
-        int64_t k = (int64_t)gj * P + gi;
-        int o = a.ocx[k];
-        if (o > 0) {
-          T u = a.Ux[k];
-          const T* s = sq + (r + 3) * LW + c;  // s[0] = q[gi-3]
-          v = flux_by_order<T, M>(o, u, s[0], s[1], s[2], s[3], s[4], s[5]) * u;
-        }
+      const int o = ox[i];
+      if (o > 0) {
+        const T u = ux[i];
+        const T* sp = sq + (r + 3) * LW + c;  // sp[0] = q[gi-3]
+        v = flux_by_order<T, M>(o, u, sp[0], sp[1], sp[2], sp[3], sp[4], sp[5]) * u;
       }
       sfx[f] = v;
     }
     // y faces: faces j0..j0+TY, columns i0..i0+TX-1
-    for (int f = t; f < (TY + 1) * TX; f += 256) {
-      int r = f / TX, c = f - r * TX;
-      int gj = j0 + r, gi = i0 + c;
+#pragma unroll
+    for (int i = 0; i < NFY; i++) {
+      const int f = t + i * 256;
+      if (f >= (TY + 1) * TX) continue;
+      const int r = f / TX, c = f - r * TX;
       T v = T(0);
-      if (gj <= 3 + a.ny && gi < 3 + a.nx) {
-        int64_t k = (int64_t)gj * P + gi;
-        int o = a.ocy[k];
-        if (o > 0) {
-          T u = a.Uy[k];
-          const T* s = sq + r * LW + (c + 3);  // s[0] = q[gj-3]
-          v = flux_by_order<T, M>(o, u, s[0], s[LW], s[2 * LW], s[3 * LW],
-                                  s[4 * LW], s[5 * LW]) *
-              u;
-        }
+      const int o = oy[i];
+      if (o > 0) {
+        const T u = uy[i];
+        const T* sp = sq + r * LW + (c + 3);  // sp[0] = q[gj-3]
+        v = flux_by_order<T, M>(o, u, sp[0], sp[LW], sp[2 * LW], sp[3 * LW],
+                                sp[4 * LW], sp[5 * LW]) *
+            u;
       }
       sfy[f] = v;
     }
     __syncthreads();
-    for (int e = t; e < TY * TX; e += 256) {
-      int r = e / TX, c = e - r * TX;
-      int gj = j0 + r, gi = i0 + c;
-      if (gj >= 3 + a.ny || gi >= 3 + a.nx) continue;
-      int64_t k = (int64_t)gj * P + gi;
-      T dfx = sfx[r * (TX + 1) + c + 1] - sfx[r * (TX + 1) + c];
-      T dfy = sfy[(r + 1) * TX + c] - sfy[r * TX + c];
-      T ds = ((-dfx) - dfy) * T(a.msk[k]);
-      stage_update<T>(a, k, ds, sq[(r + 3) * LW + c + 3]);
+#pragma unroll
+    for (int i = 0; i < NCL; i++) {
+      const int e = t + i * 256;
+      const int r = e / TX, c = e - r * TX;
+      const int gj = j0 + r, gi = i0 + c;
+      if (e >= TY * TX || gj >= 3 + a.ny || gi >= 3 + a.nx) continue;
+      const int64_t k = (int64_t)gj * P + gi;
+      const T dfx = sfx[r * (TX + 1) + c + 1] - sfx[r * (TX + 1) + c];
+      const T dfy = sfy[(r + 1) * TX + c] - sfy[r * TX + c];
+      const T ds = ((-dfx) - dfy) * T(mv[i]);
+      if (a.dso) a.dso[k] = ds;
+      if (nc > 0) {
+        const T x0 = (nc == 1) ? ds : d0[i];
+        const T x1 = (nc == 2) ? ds : (nc > 2 ? d1[i] : T(0));
+        const T x2 = (nc == 3) ? ds : T(0);
+        a.qo[k] = sq[(r + 3) * LW + c + 3] + rk_combine<T>(nc, cf, x0, x1, x2);
+      }
     }
     return;
   }

@@ -56,7 +56,8 @@ def profile_dominant(model, reps=20):
         k = 4
         per_solve = {0: nref * (k + 1), 1: nref * k, 2: nref * k}
     rows = {}
-    for kind in (0, 1, 2):
+    kinds = (0, 1, 2) if pois.method == "mg" else ()
+    for kind in kinds:
         us, by = pois.bench_kernel(kind, reps)
         rows[KINDS[kind]] = dict(us=us, bytes=by, per_step=3 * per_solve[kind])
     us, by = transport_timing(model, reps)

@@ -205,6 +205,14 @@ int f2d_poisson_destroy(f2d_poisson* h);
  * residual; giving either synchronises `stream`. */
 int f2d_poisson_solve(f2d_poisson* h, const void* b, void* x, double rtol,
                       int maxit, int* iters, double* relres, void* stream);
+/* The method a handle solves with, F2D_SOLVER_MG or F2D_SOLVER_FFT (-1 for
+ * NULL); no counterpart in the reference, whose SuperLU factor is its only
+ * method (elliptic.py:78).  With FFT the solve is direct: `maxit` is unused,
+ * *iters receives 1 + the FP64 refinements, *relres the measured true
+ * relative residual, and the solve counts as converged when that residual
+ * is below 1e-6 (`rtol` is not its acceptance test: a direct FP64 solve
+ * reaches eps times the condition number, about 1e-11 at 1024^2). */
+int f2d_poisson_method(f2d_poisson* h);
 /* Fused eulerpsi diagnostic solve (equations.py:80-82): b = c2v(omega)*mskv
  * computed on the fly from the cell field omega, then solve as above. */
 int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
@@ -223,7 +231,8 @@ const double* f2d_poisson_stats(f2d_poisson* h);
 /* Zero the counters [4..6] (no counterpart in the reference). */
 int f2d_poisson_reset_stats(f2d_poisson* h, void* stream);
 /* Copy the 8 statistics above to host memory, synchronising `stream`
- * (no counterpart in the reference). */
+ * (no counterpart in the reference).  FFT solves made without iters/relres
+ * measure nothing: they only add to the solve count [6]. */
 int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream);
 /* Measurement hook (no counterpart in the reference): launch one fine-level
  * kernel of the CG iteration `reps` times on the handle's current vectors --
