@@ -129,6 +129,22 @@ def accuracy(nx, seed, nsteps):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b)), dt
 
 
+def traffic_of(kernel, nx, prec):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 counter summary (profiles/traffic_<workload>.txt, JSON: FETCH_SIZE +
+    WRITE_SIZE per dispatch, scaled by the same counters on a copy kernel of
+    known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"traffic_eulerpsi{nx}_{prec}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    for k, v in t.get("kernels", {}).items():
+        if kernel.startswith(k):
+            return v
+    return None
+
+
 def cpu_baseline(nx, seed, nsteps):
     """The oracle (C loops + SuperLU, the reference's algorithm) on the host,
     single thread, a few steps of the same workload after factorisation."""
@@ -166,6 +182,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain elementwise copy of MIB MiB three times "
+                         "(a kernel of known traffic, to scale rocprofv3 counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU (domain-decomposed) "
@@ -173,6 +192,15 @@ def main():
         sys.exit(2)
     import torch
     from fluids2d_amd import roofline as rl
+
+    if a.copy_calibration:
+        n = a.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     nx = a.nx
     res = {}
@@ -195,6 +223,7 @@ def main():
         acc_err, acc_dt = accuracy(nx, a.seed, a.accuracy_steps)
 
     head = res.get("fp32") or res.get("fp64")
+    head_prec = "fp32" if "fp32" in res else "fp64"
     cpu = None
     if not a.no_cpu_baseline:
         cps, el, tf = cpu_baseline(nx, a.seed, a.cpu_steps)
@@ -224,8 +253,12 @@ def main():
                    "nx": nx, "ny": nx},
         "roofline": {"bound": "hbm", "achieved": prof["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": prof["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": prof.get("traffic"), "kernel": prof["kernel"],
-                     "bytes_per_launch": prof["bytes"], "avg_launch_us": prof["us"]},
+                     "traffic": traffic_of(prof["kernel"], nx, head_prec),
+                     "kernel": prof["kernel"],
+                     "bytes_per_launch": prof["bytes"], "avg_launch_us": prof["us"],
+                     "kernels": {k: {"us": round(v["us"], 2), "per_step": v["per_step"],
+                                     "gbs": round(v["gbs"], 1)}
+                                 for k, v in prof["all"].items()}},
         "cpu_baseline": cpu,
     }
     if "fp64" in res:

@@ -45,6 +45,14 @@ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
   return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Row workgroups are mapped XCD-aware: the hardware places workgroup b on
+// XCD b % 8, so workgroup b takes row block (b % 8) * (G / 8) + b / 8 and
+// each XCD handles one contiguous run of rows.  The column-major writes of
+// neighbouring rows then meet in the same L2 and leave it as whole lines.
+__device__ __forceinline__ int row_block(int b, int G) {
+  return (G & 7) ? b : (b & 7) * (G >> 3) + (b >> 3);
+}
+
 struct FftArgs {
   int nx, ny, lgx, lgy;
   int nk;        // nx / 2 + 1 spectral columns
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
   cplx<T>* b1 = b0 + P * nx;
   cplx<T>* tl = b1 + P * nx;  // twiddles, staged in LDS
   for (int w = threadIdx.x; w < nx; w += blockDim.x) tl[w] = tw[w];
-  const int y0 = blockIdx.x * 2 * P;
+  const int y0 = row_block(blockIdx.x, gridDim.x) * 2 * P;
   double mb = 0.0;
   if (x64 && bpin) {
     double acc = 0.0;
@@ -274,7 +282,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
   cplx<T>* b1 = b0 + P * nx;
   cplx<T>* tl = b1 + P * nx;
   for (int w = threadIdx.x; w < nx; w += blockDim.x) tl[w] = tw[w];
-  const int y0 = blockIdx.x * 2 * P;
+  const int y0 = row_block(blockIdx.x, gridDim.x) * 2 * P;
   // Xa -> b1, Xb -> b0 (k = 0 .. nx/2)
   const int n2 = a.nk * 2 * P;
   for (int w = threadIdx.x; w < n2; w += blockDim.x) {
@@ -446,6 +454,7 @@ struct FftSolver {
   double* part_b = nullptr;
   double* mean_b = nullptr;
   double* rowsum = nullptr;  // per row workgroup FP64 sums of b
+  double* bench_buf = nullptr;  // padded scratch for kernel timing
   int nbres = 0;
   std::vector<void*> allocs;
 };
@@ -659,6 +668,81 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
   if (stats && measure)
     k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.nx * f->a.ny,
                                   f->a.singular, rtol, nref, measure ? 1 : 0, stats);
+}
+
+}  // namespace f2d
+
+namespace f2d {
+
+// Mean launch time (us, HIP events on `s`) of one pass of the spectral solve
+// on scratch data, and its algorithmic bytes: kind 0 rows forward (1: the
+// refinement's residual form, FP32 only), 2 columns, 3 rows inverse (4: the
+// refinement's accumulating form, FP32 only).
+double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
+                 double* bytes) {
+  F2D_REQUIRE(f64 ? f->S64 != nullptr : f->S32 != nullptr,
+              "FFT solver has no buffers of this precision");
+  F2D_REQUIRE(f64 ? (kind == 0 || kind == 2 || kind == 3) : (kind >= 0 && kind <= 4),
+              "FFT bench kind");
+  const size_t npad = (size_t)(f->a.ny + 6) * (f->a.nx + 6);
+  if (!f->bench_buf) f->bench_buf = falloc<double>(f, sizeof(double) * npad);
+  FftArgs a = f->a;
+  a.P = f64 ? f->P64 : f->P32;
+  a.Q = f64 ? f->Q64 : f->Q32;
+  const size_t lr = f64 ? f->lds_r64 : f->lds_r32, lc = f64 ? f->lds_c64 : f->lds_c32;
+  const int gr = a.ny / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  const double N = (double)a.nx * a.ny, NS = (double)a.nk * a.ny;
+  const double w = f64 ? 8.0 : 4.0;  // real word
+  hipEvent_t e0, e1;
+  F2D_HIP(hipEventCreate(&e0));
+  F2D_HIP(hipEventCreate(&e1));
+  auto launch = [&]() {
+    if (f64) {
+      double* buf = f->bench_buf;
+      if (kind == 0)
+        k_fft_rows_fwd<double, double><<<gr, 256, lr, s>>>(a, buf, 0, f->S64, f->twx64,
+                                                           nullptr, nullptr, nullptr, gr);
+      else if (kind == 2)
+        k_fft_cols<double><<<gc, 256, lc, s>>>(a, f->S64, f->twy64, f->lamx, f->lamy);
+      else
+        k_fft_rows_inv<double, double, 0><<<gr, 256, lr, s>>>(a, f->S64, f->twx64, buf,
+                                                              nullptr);
+    } else {
+      float* buf = reinterpret_cast<float*>(f->bench_buf);
+      if (kind == 0 || kind == 1)
+        k_fft_rows_fwd<float, float><<<gr, 256, lr, s>>>(
+            a, buf, 0, f->S32, f->twx32, nullptr, kind == 1 ? f->x64 : nullptr,
+            nullptr, gr);
+      else if (kind == 2)
+        k_fft_cols<float><<<gc, 256, lc, s>>>(a, f->S32, f->twy32, f->lamx, f->lamy);
+      else if (kind == 3)
+        k_fft_rows_inv<float, float, 0><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
+                                                            nullptr);
+      else
+        k_fft_rows_inv<float, float, 2><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
+                                                            f->x64);
+    }
+  };
+  launch();
+  F2D_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) launch();
+  F2D_HIP(hipEventRecord(e1, s));
+  F2D_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  F2D_HIP(hipEventElapsedTime(&ms, e0, e1));
+  F2D_HIP(hipEventDestroy(e0));
+  F2D_HIP(hipEventDestroy(e1));
+  const double spec = NS * 2.0 * w;  // one pass over the half spectrum
+  double by = 0.0;
+  switch (kind) {
+    case 0: by = N * w + spec; break;
+    case 1: by = N * w + N * 8.0 + spec; break;
+    case 2: by = 2.0 * spec; break;
+    case 3: by = spec + N * w; break;
+    default: by = spec + N * 8.0 * 2.0 + N * w; break;
+  }
+  *bytes = by;
+  return 1e3 * (double)ms / reps;
 }
 
 }  // namespace f2d

@@ -37,4 +37,8 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
                           int nref, double rtol, bool measure, double* stats,
                           hipStream_t s);
 
+// timing hook (see f2d_poisson_bench_kernel)
+double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
+                 double* bytes);
+
 }  // namespace f2d

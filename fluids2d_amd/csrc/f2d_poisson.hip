@@ -2197,9 +2197,15 @@ int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,
                              double* bytes, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(h && us && bytes, "NULL argument");
-    F2D_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0 (down+CG), 1 (up), 2 (p/q)");
+    F2D_REQUIRE(kind >= 0 && kind <= 7, "kind must be in 0 .. 7");
     F2D_REQUIRE(reps > 0, "reps must be > 0");
     hipStream_t s = S(stream);
+    if (kind >= 3) {
+      F2D_REQUIRE(h->fft, "FFT kernels timed on a handle without the FFT solve");
+      *us = fft_bench(h->fft, h->opts.mode == F2D_POISSON_F64, kind - 3, reps, s, bytes);
+      return;
+    }
+    F2D_REQUIRE(!h->fft || h->p64.L > 0 || h->p32.L > 0, "no multigrid hierarchy");
     if (h->opts.mode == F2D_POISSON_F64)
       *us = bench_kernel<double>(h, h->p64, kind, reps, s, bytes);
     else

@@ -40,26 +40,42 @@ def transport_timing(model, reps=20):
     return us, byts
 
 
+FFT_KINDS = {3: "k_fft_rows_fwd (row FFTs of the right-hand side)",
+             4: "k_fft_rows_fwd (row FFTs of the FP64 refinement residual)",
+             5: "k_fft_cols (column FFT, eigenvalue division, inverse FFT)",
+             6: "k_fft_rows_inv (inverse row FFTs, halo images)",
+             7: "k_fft_rows_inv (inverse row FFTs accumulating the refinement)"}
+
+
 def profile_dominant(model, reps=20):
-    """Time the fine-level multigrid-CG kernels and the transport stage; the
-    dominant one is the largest per-step total (mean launch time x launches
-    per step, the launches counted from the last solves)."""
+    """Time the kernels of one RK3 step live (HIP events on the stream they
+    run on) and pick the dominant one: the largest mean launch time x
+    launches per step.  Launch counts come from the solver's schedule (FFT:
+    one solve per stage, one FP64 refinement in FP32 mode; MG: the
+    iterations of the last solves)."""
     pois = model.mesh.poisson_vertices
-    st = pois.read_stats()
     prec = model.param.precision
-    if prec == "fp64":
-        iters = max(st["max_iters"], 1)
-        per_solve = {0: iters + 1, 1: iters, 2: iters}
-    else:
-        nref = max(st["max_iters"], 1)
-        inner = model.mesh.poisson_vertices  # inner iterations per refinement
-        k = 4
-        per_solve = {0: nref * (k + 1), 1: nref * k, 2: nref * k}
     rows = {}
-    kinds = (0, 1, 2) if pois.method == "mg" else ()
-    for kind in kinds:
-        us, by = pois.bench_kernel(kind, reps)
-        rows[KINDS[kind]] = dict(us=us, bytes=by, per_step=3 * per_solve[kind])
+    if pois.method == "fft":
+        if prec == "fp64":
+            per = {3: 3, 5: 3, 6: 3}
+        else:
+            per = {3: 3, 4: 3, 5: 6, 6: 3, 7: 3}
+        for kind, n in per.items():
+            us, by = pois.bench_kernel(kind, reps)
+            rows[FFT_KINDS[kind]] = dict(us=us, bytes=by, per_step=n)
+    else:
+        st = pois.read_stats()
+        if prec == "fp64":
+            iters = max(st["max_iters"], 1)
+            per_solve = {0: iters + 1, 1: iters, 2: iters}
+        else:
+            nref = max(st["max_iters"], 1)
+            k = 4
+            per_solve = {0: nref * (k + 1), 1: nref * k, 2: nref * k}
+        for kind in (0, 1, 2):
+            us, by = pois.bench_kernel(kind, reps)
+            rows[KINDS[kind]] = dict(us=us, bytes=by, per_step=3 * per_solve[kind])
     us, by = transport_timing(model, reps)
     rows["k_transport (fused WENO5 flux + div + RK update)"] = dict(us=us, bytes=by,
                                                                      per_step=3)
