@@ -238,7 +238,7 @@ int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream);
  * kernel of the CG iteration `reps` times on the handle's current vectors --
  * kind 0 the update/pre-smooth/restrict pass, 1 the prolongate/post-smooth
  * pass, 2 the search-direction/operator pass -- with a scratch control block
- * and scratch outputs (the caller's solution is not touched); kinds 3..7
+ * and scratch outputs, not touching the caller's solution.  Kinds 3..7
  * time the passes of the FFT solve instead (3 rows forward, 4 rows forward
  * forming the FP64 refinement residual, 5 columns, 6 rows inverse, 7 rows
  * inverse accumulating the refinement; 4 and 7 in FP32 modes only).  *us receives
