@@ -24,7 +24,8 @@ EXPORTS = [
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
     "f2d_poisson_solve_c2v", "f2d_poisson_stats", "f2d_poisson_reset_stats",
     "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
-    "f2d_poisson_method",
+    "f2d_poisson_method", "f2d_coriolis", "f2d_addgrad", "f2d_sharp", "f2d_vorticity",
+    "f2d_ke_finish", "f2d_pressure", "f2d_div", "f2d_c2v_scaled", "f2d_v2c_scaled",
 ]
 
 
@@ -93,6 +94,15 @@ def lib():
                                      ctypes.POINTER(vp)]),
         "f2d_poisson_destroy": (i32, [vp]),
         "f2d_poisson_method": (i32, [vp]),
+        "f2d_coriolis": (i32, [i32, gp, vp, vp, vp, vp, d, vp]),
+        "f2d_addgrad": (i32, [i32, gp, vp, vp, vp, vp, vp, vp]),
+        "f2d_sharp": (i32, [i32, gp, vp, vp, vp, vp, vp]),
+        "f2d_vorticity": (i32, [i32, gp, vp, vp, vp, vp, vp]),
+        "f2d_ke_finish": (i32, [i32, gp, vp, vp, vp]),
+        "f2d_pressure": (i32, [i32, gp, vp, vp, d, d, vp]),
+        "f2d_div": (i32, [i32, gp, vp, vp, vp, vp, vp]),
+        "f2d_c2v_scaled": (i32, [i32, gp, vp, d, vp, vp, i32, vp]),
+        "f2d_v2c_scaled": (i32, [i32, gp, vp, d, vp, vp, vp, vp]),
         "f2d_poisson_solve": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
                                     ctypes.POINTER(d), vp]),
         "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),

@@ -1,0 +1,142 @@
+"""Shallow-water device models: rsw (equations.py:139-157) and qgrsw, the
+RSW tendency projected onto the QG manifold (equations.py:106-136), operator
+by operator on the device in the reference's order.
+
+The RK stage is the reference's: rhs into the stage's tendency, `addto` on
+every prognostic array (u.x, u.y, h), then diag.
+"""
+from . import _lib
+from . import operators as op
+from .equations import _Base
+from .states import allocate_device, leaves
+
+
+def _ops(name, *args):
+    _lib.call(name, *args, _lib.stream_ptr())
+
+
+class _SW(_Base):
+    prognostic = ("u", "h")
+
+    def __init__(self, param, mesh):
+        super().__init__(param, mesh)
+        self._ds_last = None
+
+    def _grid(self):
+        return self.mesh.grid_ref()
+
+    def _dt(self, t):
+        return _lib.dtype_code(t.dtype)
+
+    def _vortexforce_coriolis(self, s, ds):
+        m, p = self.mesh, self.param
+        op.vortexforce(ds.u.x, s.U.y, s.omega, m.d_ov.y, m.yshift, m.xshift, +1,
+                       p.vortexforce)
+        op.vortexforce(ds.u.y, s.U.x, s.omega, m.d_ov.x, m.xshift, m.yshift, -1,
+                       p.vortexforce)
+        f = p.f0 * m.area * 0.25
+        _ops("f2d_coriolis", self._dt(s.h), self._grid(), _lib.ptr(s.U.x), _lib.ptr(s.U.y),
+             _lib.ptr(ds.u.x), _lib.ptr(ds.u.y), float(f))
+
+    def _divflux(self, s, dh):
+        """divflux (operators.py:21-29): face fluxes into s.flx, then div."""
+        m, p = self.mesh, self.param
+        op.compflux(s.flx.x, s.U.x, s.h, m.d_oc.x, m.xshift, p.compflux)
+        op.compflux(s.flx.y, s.U.y, s.h, m.d_oc.y, m.yshift, p.compflux)
+        _ops("f2d_div", self._dt(s.h), self._grid(), _lib.ptr(s.flx.x), _lib.ptr(s.flx.y),
+             _lib.ptr(m.d_msk), _lib.ptr(dh))
+
+    def _sharp_vorticity(self, s):
+        m = self.mesh
+        g = self._grid()
+        dt = self._dt(s.h)
+        _ops("f2d_sharp", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y), _lib.ptr(s.U.x),
+             _lib.ptr(s.U.y))
+        _ops("f2d_vorticity", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y),
+             _lib.ptr(m.d_slipcoef), _lib.ptr(s.omega))
+
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+        """One stage of the reference's RK (integrators.py:92-124): rhs,
+        addto on u.x, u.y and h, diag."""
+        p = self.param
+        if ds_out is None:
+            if self._ds_last is None:
+                self._ds_last = allocate_device(self.prognostic, self.mesh.shape,
+                                                s.h.dtype, s.h.device)
+            ds = self._ds_last
+        else:
+            ds = ds_out
+        self.rhs(s, ds)
+        ys = leaves(s.u) + [s.h]
+        dsl = leaves(ds.u) + [ds.h]
+        prev = [leaves(x.u) + [x.h] for x in ds_prev]
+        for n, y in enumerate(ys):
+            op.rk_addto(y, [pv[n] for pv in prev] + [dsl[n]], p.integrator, k, dt)
+        self.diag(s)
+        if dt_next is not None:
+            op.cfl_dt(s.U, p.cfl, p.dtmax, dt_next)
+        return s, alt
+
+
+class RSW(_SW):
+    """Rotating shallow water (equations.py:139-157)."""
+
+    def rhs(self, s, ds):
+        m = self.mesh
+        self._vortexforce_coriolis(s, ds)
+        dt, g = self._dt(s.h), self._grid()
+        for phi in (s.ke, s.p):  # addgrad(ke), addgrad(p)
+            _ops("f2d_addgrad", dt, g, _lib.ptr(phi), _lib.ptr(m.d_mskx),
+                 _lib.ptr(m.d_msky), _lib.ptr(ds.u.x), _lib.ptr(ds.u.y))
+        self._divflux(s, ds.h)
+        for a in (ds.u.x, ds.u.y, ds.h):
+            op.fill(m, a)
+
+    def diag(self, s):
+        m, p = self.mesh, self.param
+        self._sharp_vorticity(s)
+        # compute_kinetic_energy (operators.py:80-101)
+        s.ke.zero_()
+        op.innerproduct(s.ke, s.U.x, s.u.x, m.d_ok.x, m.xshift, p.innerproduct)
+        op.innerproduct(s.ke, s.U.y, s.u.y, m.d_ok.y, m.yshift, p.innerproduct)
+        dt, g = self._dt(s.h), self._grid()
+        _ops("f2d_ke_finish", dt, g, _lib.ptr(s.ke), _lib.ptr(m.d_msk))
+        # compute_pressure (operators.py:110-111), hb = 0
+        _ops("f2d_pressure", dt, g, _lib.ptr(s.h), _lib.ptr(s.p), float(p.g / m.area),
+             0.0)
+        op.fill(m, s.omega)
+        op.fill(m, s.ke)
+
+
+class QGRSW(_SW):
+    """RSW tendency projected onto the QG manifold (equations.py:106-136,
+    Thiry et al. 2024)."""
+
+    def rhs(self, s, ds):
+        m, p = self.mesh, self.param
+        self._vortexforce_coriolis(s, ds)
+        self._divflux(s, ds.h)
+        dt, g = self._dt(s.h), self._grid()
+        # qg_projection(mesh, ds.u, ds.h, s.pv, s.psi, anomaly=True),
+        # operators.py:176-183
+        _ops("f2d_vorticity", dt, g, _lib.ptr(ds.u.x), _lib.ptr(ds.u.y),
+             _lib.ptr(m.d_slipcoef), _lib.ptr(s.pv))
+        _ops("f2d_c2v_scaled", dt, g, _lib.ptr(ds.h), float(-p.f0 / p.H), _lib.ptr(s.pv),
+             _lib.ptr(m.d_mskv), 1)
+        m.qg_helmholtz.solve(s.pv, s.psi)
+        _ops("f2d_v2c_scaled", dt, g, _lib.ptr(s.psi), float(p.f0 * m.area / p.g),
+             _lib.ptr(m.d_mskv), _lib.ptr(m.d_msk), _lib.ptr(ds.h))
+        op.perpgrad(m, s.psi, ds.u, contravariant=False)
+        for a in (ds.u.x, ds.u.y, ds.h):
+            op.fill(m, a)
+
+    def diag(self, s):
+        self._sharp_vorticity(s)
+        op.fill(self.mesh, s.omega)
+
+
+MODELS = {"rsw": RSW, "qgrsw": QGRSW}
+
+
+def get_device_model(param, mesh):
+    return MODELS[param.model](param, mesh)

@@ -157,6 +157,47 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
 int f2d_set_scalar(double* dst, double value, void* stream);
 
 /* ------------------------------------------------------------------------
+ * C-grid point operators of the shallow-water models (operators.py), each
+ * writing exactly the entries the reference's numpy slices assign (the
+ * others keep their values), masks applied by multiplication, sums left to
+ * right.  Padded arrays of the grid; dtype F2D_F64 or F2D_F32.
+ * ------------------------------------------------------------------------ */
+/* addcoriolis, operators.py:32-39: dux += f*(4 Uy), duy -= f*(4 Ux),
+ * f = f0*area*0.25 */
+int f2d_coriolis(int dtype, const f2d_grid* g, const void* Ux, const void* Uy,
+                 void* dux, void* duy, double f, void* stream);
+/* addgrad, operators.py:49-56: du -= grad(phi) * (mskx, msky) */
+int f2d_addgrad(int dtype, const f2d_grid* g, const void* phi,
+                const int8_t* mskx, const int8_t* msky, void* dux, void* duy,
+                void* stream);
+/* sharp, operators.py:59-64: U = u * (1/dx**2, 1/dy**2) */
+int f2d_sharp(int dtype, const f2d_grid* g, const void* ux, const void* uy,
+              void* Ux, void* Uy, void* stream);
+/* compute_vorticity, operators.py:67-77 (slipcoef: float64 array) */
+int f2d_vorticity(int dtype, const f2d_grid* g, const void* ux, const void* uy,
+                  const double* slipcoef, void* omega, void* stream);
+/* the last statement of compute_kinetic_energy, operators.py:99-101:
+ * ke *= msk * 0.5 (the inner products are f2d_innerproduct) */
+int f2d_ke_finish(int dtype, const f2d_grid* g, void* ke, const int8_t* msk,
+                  void* stream);
+/* compute_pressure, operators.py:110-111: p = (g/area) * (h + hb) */
+int f2d_pressure(int dtype, const f2d_grid* g, const void* h, void* p,
+                 double g_over_area, double hb, void* stream);
+/* div, operators.py:104-107: delta = -(div F) * msk (last row/column of
+ * the x/y parts as numpy leaves them) */
+int f2d_div(int dtype, const f2d_grid* g, const void* Fx, const void* Fy,
+            const int8_t* msk, void* delta, void* stream);
+/* centerstovertices of c*scale, operators.py:126-133 (addto as the
+ * reference's flag), as add_stretching calls it, operators.py:194-201 */
+int f2d_c2v_scaled(int dtype, const f2d_grid* g, const void* c, double scale,
+                   void* v, const int8_t* mskv, int addto, void* stream);
+/* verticestocenters of vh*scale, operators.py:136-141, as
+ * thickness_from_psi calls it, operators.py:204-211 (cells with no wet
+ * vertex get (1/0)*0 = NaN before the mask, as in numpy) */
+int f2d_v2c_scaled(int dtype, const f2d_grid* g, const void* vh, double scale,
+                   const int8_t* mskv, const int8_t* msk, void* h, void* stream);
+
+/* ------------------------------------------------------------------------
  * Masked Poisson / Helmholtz on the vertex grid (elliptic.py:71-203).
  * Solves A x = b at the wet points of `unknowns_host` where A is the
  * masked 5-point operator of get_Laplacian_sparse ("v", Dirichlet):

@@ -1,0 +1,160 @@
+"""Shallow-water models on the GPU against the oracle (FP64, identical
+seeded inputs): rsw (equations.py:139-157) and qgrsw (equations.py:106-136).
+
+rsw is composed of bit-exact kernels (flat WENO loops, numpy-exact point
+operators, RK addto), so its prognostic fields must match the oracle to the
+last bit.  qgrsw adds a masked Helmholtz solve per stage (multigrid CG or the
+FFT solve against SuperLU, ~1e-13), so it is held to 1e-10 relative L2.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import P, assert_same, random_mask
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ["model", "nx", "ny", "Lx", "Ly", "xperiodic", "yperiodic", "halowidth",
+        "noslip", "f0", "beta", "g", "H", "tend", "dt", "maxite", "dtmax",
+        "integrator", "cfl", "compflux", "vortexforce", "innerproduct", "maxorder"]
+
+
+def _param(**kw):
+    import fluids2d_amd as f2d
+    import fluids2d_amd.param as fp
+    fp.QUIET = True
+    p = f2d.Param()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _bumps(mesh, seed, n=6, amp=0.05):
+    x, y = mesh.xy("c")
+    rng = np.random.default_rng(seed)
+    h = np.ones(mesh.shape)
+    Lx, Ly = mesh.nx * mesh.dx, mesh.ny * mesh.dy
+    for _ in range(n):
+        x0, y0 = rng.uniform(0, Lx), rng.uniform(0, Ly)
+        r = rng.uniform(0.03, 0.1) * min(Lx, Ly)
+        h += amp * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * r * r))
+    return h * mesh.area * mesh.msk
+
+
+def _run_pair(p, msk, nsteps, seed, perturb_u=False):
+    import fluids2d_amd as f2d
+    model = f2d.Model(p)
+    model.mesh.msk[...] = msk
+    model.mesh.finalize()
+    model.integrator.model.mesh = model.mesh
+    h0 = _bumps(model.mesh, seed)
+    s = model.state
+    s.h[...] = h0
+    if perturb_u:
+        rng = np.random.default_rng(seed + 1)
+        s.u.x[...] = 1e-4 * rng.standard_normal(s.u.x.shape) * model.mesh.mskx
+        s.u.y[...] = 1e-4 * rng.standard_normal(s.u.y.shape) * model.mesh.msky
+    u0x, u0y = s.u.x.copy(), s.u.y.copy()
+    model.integrator.diag(s)
+    p.maxite = nsteps
+    model.run()
+    ref = O.Model(P(**{k: getattr(p, k) for k in KEYS}), msk=msk.copy())
+    ref.state.h[...] = h0
+    ref.state.u.x[...] = u0x
+    ref.state.u.y[...] = u0y
+    ref.diag(ref.state)
+    ref.run(nsteps)
+    assert model.time.ite == ref.time.ite == nsteps
+    return model.state, ref.state
+
+
+def clean_mask(m):
+    """Dry every wet cell none of whose four vertices is wet (repeatedly):
+    at such cells verticestocenters (operators.py:136-141) divides by zero
+    and the reference's qgrsw fills the whole domain with NaN."""
+    m = m.copy()
+    while True:
+        _, _, mv = O.set_masks(m)
+        c = np.zeros(m.shape, dtype=int)
+        c[:-1, :-1] = mv[:-1, :-1] + mv[1:, :-1] + mv[:-1, 1:] + mv[1:, 1:]
+        bad = (m == 1) & (c == 0)
+        bad[-1, :] = False
+        bad[:, -1] = False
+        if not bad.any():
+            return m
+        m[bad] = 0
+
+
+def _rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.mark.parametrize("nx,ny,xper,nisl,noslip", [
+    (64, 32, True, 3, None), (48, 48, False, 2, None), (40, 32, True, 2, True)])
+def test_rsw_bitexact(dev, nx, ny, xper, nisl, noslip):
+    p = _param(model="rsw", nx=nx, ny=ny, Lx=nx / ny, Ly=1.0, xperiodic=xper,
+               yperiodic=False, noslip=noslip, f0=1.0, g=1.0, H=1.0, tend=1e9)
+    msk = random_mask(ny, nx, 11, xper, False, nisl=nisl)
+    s, r = _run_pair(p, msk, 30, seed=3, perturb_u=True)
+    assert_same(s.h, r.h, "h")
+    assert_same(s.u.x, r.u.x, "u.x")
+    assert_same(s.u.y, r.u.y, "u.y")
+    assert_same(s.omega, r.omega, "omega")
+    assert_same(s.ke, r.ke, "ke")
+
+
+@pytest.mark.parametrize("nx,ny,xper,yper,nisl", [
+    (48, 48, False, False, 3), (64, 64, True, True, 0), (64, 32, True, False, 2)])
+def test_qgrsw_parity(dev, nx, ny, xper, yper, nisl):
+    # a fixed step with f0 dt = 0.05: the CFL step of model.py:84-87 sees only
+    # the (small) advective velocity and would make f0 dt >> 1, where RK3 is
+    # unstable for the rotation term and any rounding difference explodes
+    p = _param(model="qgrsw", nx=nx, ny=ny, Lx=nx / ny, Ly=1.0, xperiodic=xper,
+               yperiodic=yper, tend=1e9, dt=0.005)
+    msk = clean_mask(random_mask(ny, nx, 7, xper, yper, nisl=nisl)) if nisl else \
+        O.default_mask(ny, nx, 3, xper, yper)
+    s, r = _run_pair(p, msk, 20, seed=5, perturb_u=True)
+    for name in ("h",):
+        e = _rel(np.nan_to_num(getattr(s, name)), np.nan_to_num(getattr(r, name)))
+        print(f"\nqgrsw {nx}x{ny} {name}: rel L2 {e:.2e}")
+        assert e <= 1e-10, (name, e)
+    for c in ("x", "y"):
+        e = _rel(getattr(s.u, c), getattr(r.u, c))
+        assert e <= 1e-10, ("u." + c, e)
+    # NaN halo cells of h (cells with no wet vertex, operators.py:136-141)
+    # appear where the reference's do
+    assert np.array_equal(np.isnan(s.h), np.isnan(r.h))
+
+
+def test_qgrsw_nan_cells_like_reference(dev):
+    """On a mask with wet cells that touch no wet vertex, one qgrsw tendency
+    has its NaN (1/0 * 0 of verticestocenters) exactly where the reference's
+    has, and agrees elsewhere."""
+    import fluids2d_amd as f2d
+    import fluids2d_amd.states as st
+    p = _param(model="qgrsw", nx=48, ny=48, tend=1e9, dt=0.005)
+    msk = random_mask(48, 48, 7, False, False, nisl=3)
+    model = f2d.Model(p)
+    model.mesh.msk[...] = msk
+    model.mesh.finalize()
+    ref = O.Model(P(**{k: getattr(p, k) for k in KEYS}), msk=msk.copy())
+    rng = np.random.default_rng(1)
+    h0 = (1 + 0.05 * rng.standard_normal(msk.shape)) * model.mesh.area * model.mesh.msk
+    ux = 1e-4 * rng.standard_normal(msk.shape) * model.mesh.mskx
+    uy = 1e-4 * rng.standard_normal(msk.shape) * model.mesh.msky
+    for s in (model.state, ref.state):
+        s.h[...] = h0
+        s.u.x[...] = ux
+        s.u.y[...] = uy
+    model.integrator.diag(model.state)
+    ref.diag(ref.state)
+    ds_ref = O.allocate(("u", "h"), msk.shape)
+    ref.rhs(ref.state, ds_ref)
+    ds = st.Prognostic(p, msk.shape)
+    model.integrator.rhs(model.state, ds)
+    wet = msk == 1
+    assert np.isnan(ds_ref.h[wet]).sum() > 0
+    assert np.array_equal(np.isnan(ds.h), np.isnan(ds_ref.h))
+    ok = ~np.isnan(ds_ref.h)
+    assert _rel(ds.h[ok], ds_ref.h[ok]) < 1e-11
