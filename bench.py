@@ -129,6 +129,45 @@ def accuracy(nx, seed, nsteps):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b)), dt
 
 
+def adjoint_cost(n=512, steps=200, checkpoint=20, seed=2024, reps=2):
+    """Forward + adjoint time over forward time for the gradient of a
+    final-time misfit with respect to the initial vorticity: eulerpsi n^2
+    doubly periodic FP32, `steps` RK3 steps at a fixed CFL-0.5 step,
+    checkpoints every `checkpoint` steps (fluids2d_amd.adjoint)."""
+    import torch
+    from fluids2d_amd.adjoint import EulerAdjoint, EulerGradient
+    p = make_param(n, "fp32")
+    w0 = np.zeros((n + 6, n + 6))
+    w0[3:-3, 3:-3] = random_phase_vorticity(n, seed) / (n * n)
+    fill_periodic(w0)
+    wt = np.zeros_like(w0)
+    wt[3:-3, 3:-3] = (random_phase_vorticity(n, seed) +
+                      0.3 * random_phase_vorticity(n, seed + 1)) / (n * n)
+    fill_periodic(wt)
+    probe = EulerAdjoint(p, 1.0, 1)
+    dev = probe.mesh.device
+    w0 = torch.from_numpy(w0).to(dev)
+    wt = torch.from_numpy(wt).to(dev).float()
+    runner = EulerAdjoint(p, probe.cfl_dt(w0, 0.5), steps, checkpoint)
+    runner.forward(w0)  # warm-up (Poisson set-up, allocator)
+    torch.cuda.synchronize()
+    tf = []
+    tb = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        runner.forward(w0)
+        torch.cuda.synchronize()
+        tf.append(time.perf_counter() - t0)
+        x = w0.clone().float().requires_grad_(True)
+        t0 = time.perf_counter()
+        wT = EulerGradient.apply(x, runner)
+        L = 0.5 * ((wT - wt)[3:-3, 3:-3] ** 2).sum()
+        L.backward()
+        torch.cuda.synchronize()
+        tb.append(time.perf_counter() - t0)
+    return min(tb) / min(tf), min(tf), min(tb)
+
+
 def traffic_of(kernel, nx, prec):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary (profiles/traffic_<workload>.txt, JSON: FETCH_SIZE +
@@ -182,6 +221,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+adjoint cost measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB three times "
                          "(a kernel of known traffic, to scale rocprofv3 counters)")
@@ -271,6 +312,13 @@ def main():
         out["accuracy_steps"] = a.accuracy_steps
     out["poisson"] = {k: v["stats"] for k, v in res.items()}
     out["adjoint_cost_ratio"] = None
+    if not a.no_adjoint:
+        ratio, tfw, tfb = adjoint_cost()
+        out["adjoint_cost_ratio"] = ratio
+        out["adjoint"] = {"workload": "eulerpsi 512x512 doubly periodic FP32, 200 RK3 steps, "
+                                      "checkpoints every 20, gradient of a final misfit "
+                                      "w.r.t. the initial vorticity",
+                          "forward_s": tfw, "forward_plus_adjoint_s": tfb}
     print(json.dumps(out))
 
 

@@ -198,6 +198,26 @@ int f2d_v2c_scaled(int dtype, const f2d_grid* g, const void* vh, double scale,
                    const int8_t* mskv, const int8_t* msk, void* h, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Adjoint of the eulerpsi RK stage on a fully wet, doubly periodic grid (no
+ * counterpart in the reference, which has no derivatives): the stage
+ * evaluates R(s) = -div(U r(s)) with U = perpgrad(A^-1 c2v(s)) (weno.py:346-
+ * 364, operators.py:126-149, elliptic.py:80-87); these entry points build
+ * R'(s)^T lam.  Inputs need their periodic halos filled; outputs are written
+ * on the interior of padded arrays.
+ * ------------------------------------------------------------------------ */
+/* Transport part (compflux/div of weno.py:346-364, operators.py:21-29,
+ * reversed): lq = (dT/dq)^T lam, lUx/lUy = (dT/dU)^T lam.  work: 12 padded
+ * arrays of scratch. */
+int f2d_adj_transport(int dtype, const f2d_grid* g, const void* q, const void* Ux,
+                      const void* Uy, const void* lam, void* lUx, void* lUy,
+                      void* work, void* lq, void* stream);
+/* perpgrad^T (operators.py:144-149 reversed, contravariant factors) */
+int f2d_adj_perpgrad(int dtype, const f2d_grid* g, const void* lUx, const void* lUy,
+                     void* lpsi, void* stream);
+/* lq += c2v^T lb (operators.py:126-133 reversed) */
+int f2d_adj_c2v(int dtype, const f2d_grid* g, const void* lb, void* lq, void* stream);
+
+/* ------------------------------------------------------------------------
  * Masked Poisson / Helmholtz on the vertex grid (elliptic.py:71-203).
  * Solves A x = b at the wet points of `unknowns_host` where A is the
  * masked 5-point operator of get_Laplacian_sparse ("v", Dirichlet):

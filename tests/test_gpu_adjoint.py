@@ -1,0 +1,119 @@
+"""Hand-written adjoint of the eulerpsi RK3/WENO step (fluids2d_amd.adjoint)
+against an FP64 torch-autograd gradient of the same discretisation
+(tests/torch_euler.py) and against central finite differences of the loss
+L = 1/2 sum (omega_T - omega_truth)^2 over the interior cells."""
+import numpy as np
+import pytest
+import torch
+
+import torch_euler as TE
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n, precision, nsteps, checkpoint, seed=7, dt=None):
+    import bench
+    import fluids2d_amd.param as fp
+    from fluids2d_amd import Param
+    from fluids2d_amd.adjoint import EulerAdjoint
+    fp.QUIET = True
+    p = Param()
+    p.model = "eulerpsi"
+    p.nx = p.ny = n
+    p.xperiodic = p.yperiodic = True
+    p.precision = precision
+    w0 = np.zeros((n + 6, n + 6))
+    w0[3:-3, 3:-3] = bench.random_phase_vorticity(n, seed) / (n * n)
+    bench.fill_periodic(w0)
+    wt = np.zeros_like(w0)
+    wt[3:-3, 3:-3] = (bench.random_phase_vorticity(n, seed) +
+                      0.3 * bench.random_phase_vorticity(n, seed + 1)) / (n * n)
+    bench.fill_periodic(wt)
+    probe = EulerAdjoint(p, 1.0, 1)
+    dev = probe.mesh.device
+    if dt is None:
+        dt = probe.cfl_dt(torch.from_numpy(w0).to(dev), 0.5)
+    runner = EulerAdjoint(p, dt, nsteps, checkpoint)
+    return runner, torch.from_numpy(w0).to(dev), torch.from_numpy(wt).to(dev), dt
+
+
+def _loss_and_grad(runner, w0, wt):
+    from fluids2d_amd.adjoint import EulerGradient
+    x = w0.clone().to(runner.dtype).requires_grad_(True)
+    wT = EulerGradient.apply(x, runner)
+    d = (wT - wt.to(runner.dtype))[3:-3, 3:-3]
+    L = 0.5 * (d.double() ** 2).sum()
+    L.backward()
+    return float(L.detach()), x.grad.double()
+
+
+def test_forward_matches_torch_restatement(dev):
+    runner, w0, wt, dt = _setup(64, "fp64", 5, 5)
+    wT = runner.forward(w0)
+    h = 1.0 / 64
+    ref = TE.run(w0[3:-3, 3:-3].cpu(), 5, dt, h, h)
+    e = float(torch.linalg.norm(wT[3:-3, 3:-3].cpu() - ref) / torch.linalg.norm(ref))
+    assert e <= 1e-12, e
+
+
+def test_adjoint_matches_torch_autograd(dev):
+    runner, w0, wt, dt = _setup(64, "fp64", 6, 4)
+    L, g = _loss_and_grad(runner, w0, wt)
+    h = 1.0 / 64
+    x = w0[3:-3, 3:-3].cpu().clone().requires_grad_(True)
+    wT = TE.run(x, 6, dt, h, h)
+    Lr = 0.5 * ((wT - wt[3:-3, 3:-3].cpu()) ** 2).sum()
+    Lr.backward()
+    gr = x.grad
+    e = float(torch.linalg.norm(g[3:-3, 3:-3].cpu() - gr) / torch.linalg.norm(gr))
+    print(f"\nadjoint vs torch autograd: rel L2 {e:.2e} (L {L:.6e} vs {float(Lr):.6e})")
+    assert abs(L - float(Lr)) <= 1e-10 * abs(float(Lr))
+    assert e <= 1e-9, e
+    # halo entries of omega0 do not enter the map (they are refilled)
+    assert float(g[:3].abs().max()) == 0.0 and float(g[:, :3].abs().max()) == 0.0
+
+
+def test_adjoint_finite_differences(dev):
+    """Central differences along 8 seeded unit directions.  The map is only
+    piecewise smooth (the |beta1 - beta3| of WENO-Z and the upwind switch at
+    U = 0 have kinks), so the step must be small: eps = 3e-9 relative to the
+    field, with the loss increment formed from the difference of the two
+    final states, (w+ - w-).(w+ + w- - 2 truth) / 2, to keep FP64 rounding
+    below the comparison."""
+    runner, w0, wt, dt = _setup(128, "fp64", 20, 5)
+    L, g = _loss_and_grad(runner, w0, wt)
+    rng = np.random.default_rng(3)
+    scale = float(w0[3:-3, 3:-3].abs().max())
+    errs = []
+    tt = wt[3:-3, 3:-3]
+    for k in range(8):
+        d = torch.zeros_like(w0)
+        d[3:-3, 3:-3] = torch.from_numpy(rng.standard_normal((128, 128))).to(w0.device)
+        d /= torch.linalg.norm(d)
+        eps = 3e-9 * scale * 128
+        wp = runner.forward(w0 + eps * d)[3:-3, 3:-3]
+        wm = runner.forward(w0 - eps * d)[3:-3, 3:-3]
+        fd = 0.5 * float(((wp - wm) * ((wp - tt) + (wm - tt))).sum()) / (2 * eps)
+        ad = float((g * d).sum())
+        errs.append(abs(ad - fd) / abs(fd))
+        print(f"direction {k}: adjoint {ad:.12e} FD {fd:.12e} rel {errs[-1]:.1e}")
+    assert max(errs) <= 2e-6, errs
+    assert float(np.median(errs)) <= 1e-6, errs
+
+
+def test_checkpoint_interval_does_not_change_gradient(dev):
+    r1, w0, wt, dt = _setup(64, "fp64", 9, 1)
+    r2, _, _, _ = _setup(64, "fp64", 9, 4, dt=dt)
+    _, g1 = _loss_and_grad(r1, w0, wt)
+    _, g2 = _loss_and_grad(r2, w0, wt)
+    assert torch.equal(g1, g2)
+
+
+def test_fp32_adjoint_close_to_fp64(dev):
+    r64, w0, wt, dt = _setup(128, "fp64", 20, 5)
+    r32, _, _, _ = _setup(128, "fp32", 20, 5, dt=dt)
+    _, g64 = _loss_and_grad(r64, w0, wt)
+    _, g32 = _loss_and_grad(r32, w0, wt)
+    e = float(torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64))
+    print(f"\nFP32 vs FP64 gradient: rel L2 {e:.2e}")
+    assert e <= 1e-3, e
