@@ -20,7 +20,7 @@ EXPORTS = [
     "f2d_set_masks", "f2d_set_order", "f2d_slipcoef", "f2d_gindex",
     "f2d_compflux", "f2d_vortexforce", "f2d_innerproduct",
     "f2d_transport_stage", "f2d_rk_addto", "f2d_c2v", "f2d_perpgrad",
-    "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar",
+    "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar", "f2d_copy_f64",
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
     "f2d_poisson_solve_c2v", "f2d_poisson_stats", "f2d_poisson_reset_stats",
     "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
@@ -50,7 +50,8 @@ class PoissonOpts(ctypes.Structure):
                 ("nu_coarse", ctypes.c_int32),
                 ("nu_tail", ctypes.c_int32),
                 ("solver", ctypes.c_int32),
-                ("fft_refine", ctypes.c_int32)]
+                ("fft_refine", ctypes.c_int32),
+                ("fft_warm", ctypes.c_int32)]
 
 
 SOLVERS = {"auto": 0, "mg": 1, "fft": 2}
@@ -91,6 +92,7 @@ def lib():
         "f2d_fill_halo": (i32, [i32, gp, vp, vp]),
         "f2d_cfl_dt": (i32, [i32, vp, vp, i64, d, d, vp, vp]),
         "f2d_set_scalar": (i32, [vp, d, vp]),
+        "f2d_copy_f64": (i32, [vp, vp, i32, vp]),
         "f2d_poisson_create": (i32, [vp, gp, d, ctypes.POINTER(PoissonOpts),
                                      ctypes.POINTER(vp)]),
         "f2d_poisson_destroy": (i32, [vp]),

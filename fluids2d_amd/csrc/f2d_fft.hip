@@ -151,10 +151,29 @@ __device__ __forceinline__ double rval(const FftArgs& a, const TB* __restrict__ 
   return (gval(b, k, a.NXP, gmode) - mb) - Ax;
 }
 
-// Rows forward.  Without x64: the rows of b (or of c2v(b)); with bpart
-// non-NULL each workgroup also writes the FP64 sum of its values (the mean
-// of b for the singular operator).  With x64: the rows of the residual
-// (b - mb) - A x64, mb = (sum of the nbp partials bpin) / (nx ny).
+// right-hand side of a warm-started refinement: (g - mb) - A x32 at
+// interior point (y, n), g the right-hand side there, x32 a caller's FP32
+// initial guess read with periodic (wrapped) interior indices, so that its
+// halo is not read
+__device__ __forceinline__ double rval32(const FftArgs& a, double g,
+                                         const float* __restrict__ x, int y, int n,
+                                         double mb) {
+  const int64_t P = a.NXP;
+  const int64_t r = (int64_t)(y + 3) * P + 3;
+  const int64_t rs = (int64_t)(y == 0 ? a.ny - 1 : y - 1) + 3;
+  const int64_t rn = (int64_t)(y == a.ny - 1 ? 0 : y + 1) + 3;
+  const int nw = n == 0 ? a.nx - 1 : n - 1, ne = n == a.nx - 1 ? 0 : n + 1;
+  const double Ax = a.cx * ((double)x[r + nw] + (double)x[r + ne]) +
+                    a.cy * ((double)x[rs * P + n + 3] + (double)x[rn * P + n + 3]) -
+                    a.d * (double)x[r + n];
+  return (g - mb) - Ax;
+}
+
+// Rows forward.  Without x64/x32: the rows of b (or of c2v(b)); with bpart
+// non-NULL each workgroup also writes the FP64 sum of its values of b (the
+// mean of b for the singular operator).  With x64 (or x32): the rows of
+// the residual (b - mb) - A x, mb = (sum of the nbp partials bpin) / (nx ny);
+// bpart, if given, still receives the sums of b.
 template <class T, class TB>
 __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
                                                       const TB* __restrict__ b,
@@ -164,7 +183,8 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
                                                       double* __restrict__ bpart,
                                                       const double* __restrict__ x64,
                                                       const double* __restrict__ bpin,
-                                                      int nbp) {
+                                                      int nbp,
+                                                      const float* __restrict__ x32) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double sh[256];
   const int nx = a.nx, P = a.P, lg = a.lgx;
@@ -174,7 +194,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
   for (int w = threadIdx.x; w < nx; w += blockDim.x) tl[w] = tw[w];
   const int y0 = row_block(blockIdx.x, gridDim.x) * 2 * P;
   double mb = 0.0;
-  if (x64 && bpin) {
+  if ((x64 || x32) && bpin) {
     double acc = 0.0;
     for (int i = threadIdx.x; i < nbp; i += blockDim.x) acc += bpin[i];
     mb = bsum256(acc, sh) / ((double)a.nx * (double)a.ny);
@@ -184,7 +204,12 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
     const int p = w >> lg, n = w & (nx - 1);
     const int64_t ka = (int64_t)(y0 + 2 * p + 3) * a.NXP + n + 3;
     double va, vb;
-    if (x64) {
+    if (x32) {
+      const double ga = gval(b, ka, a.NXP, gmode), gb = gval(b, ka + a.NXP, a.NXP, gmode);
+      sum += ga + gb;
+      va = rval32(a, ga, x32, y0 + 2 * p, n, mb);
+      vb = rval32(a, gb, x32, y0 + 2 * p + 1, n, mb);
+    } else if (x64) {
       va = rval(a, b, gmode, x64, ka, mb);
       vb = rval(a, b, gmode, x64, ka + a.NXP, mb);
     } else {
@@ -269,7 +294,9 @@ __device__ __forceinline__ void put_images(TX* x, int y, int n, const FftArgs& a
 }
 
 // MODE 0: x = result (TX); 1: x = result and x64 = result; 2: x64 += result,
-// x = (TX) x64
+// x = (TX) x64; 3: x += result in FP64 (x = (TX)(x + result)), also into x64
+// when given (each interior point is read and written by one thread; the
+// halo images written are never read)
 template <class T, class TX, int MODE>
 __global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
                                                       const cplx<T>* __restrict__ S,
@@ -307,7 +334,16 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
     const int ya = y0 + 2 * p;
     const cplx<T> v = r[w];
     const double va = (double)v.x, vb = -(double)v.y;
-    if (MODE == 2) {
+    if (MODE == 3) {
+      const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
+      const double na = (double)x[ka] + va, nb = (double)x[ka + a.NXP] + vb;
+      if (x64) {
+        put_images<double>(x64, ya, n, a, na);
+        put_images<double>(x64, ya + 1, n, a, nb);
+      }
+      put_images<TX>(x, ya, n, a, TX(na));
+      put_images<TX>(x, ya + 1, n, a, TX(nb));
+    } else if (MODE == 2) {
       const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
       const double na = x64[ka] + va, nb = x64[ka + a.NXP] + vb;
       put_images<double>(x64, ya, n, a, na);
@@ -453,7 +489,11 @@ struct FftSolver {
   double* part = nullptr;
   double* part_b = nullptr;
   double* mean_b = nullptr;
-  double* rowsum = nullptr;  // per row workgroup FP64 sums of b
+  // per row-workgroup FP64 sums of b, double-buffered: a warm-started
+  // solve reads the previous call's while it writes its own
+  double* rowsum[2] = {nullptr, nullptr};
+  int rs_cur = 0;
+  bool solved = false;  // a warm start needs an earlier solve of this handle
   double* bench_buf = nullptr;  // padded scratch for kernel timing
   int nbres = 0;
   std::vector<void*> allocs;
@@ -561,7 +601,8 @@ FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
     f->part = falloc<double>(f, sizeof(double) * 4 * f->nbres);
     f->part_b = falloc<double>(f, sizeof(double) * f->nbres);
     f->mean_b = falloc<double>(f, sizeof(double));
-    f->rowsum = falloc<double>(f, sizeof(double) * (ny / 2 + 1));
+    f->rowsum[0] = falloc<double>(f, sizeof(double) * (ny / 2 + 1));
+    f->rowsum[1] = falloc<double>(f, sizeof(double) * (ny / 2 + 1));
     if (f64) {
       f->twx64 = twiddles<double2>(f, (int)nx);
       f->twy64 = twiddles<double2>(f, (int)ny);
@@ -594,13 +635,17 @@ void fft_destroy(FftSolver* f) {
   delete f;
 }
 
-// one spectral solve: rows forward, columns, rows inverse.  rowsum: write
-// the per-workgroup sums of b; xres: transform the residual b - A xres
-// instead of b (with the mean of b from the sums written before)
+// one spectral solve: rows forward, columns, rows inverse.  rs_out: write
+// the per-workgroup sums of b there; xres (FP64) or xres32 (a caller's FP32
+// guess): transform the residual b - A x instead of b, with the mean of b
+// taken from the sums rs_in (singular operator only; any constant does,
+// since the solve drops the mean, and the closer to the mean of b the less
+// FP32 rounding of the residual)
 template <class T, class TB, class TX, int MODE>
 static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
-                     hipStream_t s, bool rowsum = false,
-                     const double* xres = nullptr) {
+                     hipStream_t s, double* rs_out = nullptr,
+                     const double* xres = nullptr, const float* xres32 = nullptr,
+                     const double* rs_in = nullptr) {
   FftArgs a = f->a;
   const bool d = sizeof(T) == 8;
   a.P = d ? f->P64 : f->P32;
@@ -610,10 +655,9 @@ static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
   const cplx<T>* twx = d ? (const cplx<T>*)f->twx64 : (const cplx<T>*)f->twx32;
   const cplx<T>* twy = d ? (const cplx<T>*)f->twy64 : (const cplx<T>*)f->twy32;
   const int gr = a.ny / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
-  const double* bpin = (xres && a.singular) ? f->rowsum : nullptr;
-  k_fft_rows_fwd<T, TB><<<gr, 256, lr, s>>>(a, b, gmode, S, twx,
-                                            rowsum ? f->rowsum : nullptr, xres,
-                                            bpin, gr);
+  const double* bpin = ((xres || xres32) && a.singular) ? rs_in : nullptr;
+  k_fft_rows_fwd<T, TB><<<gr, 256, lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
+                                            xres32);
   k_fft_cols<T><<<gc, 256, lc, s>>>(a, S, twy, f->lamx, f->lamy);
   k_fft_rows_inv<T, TX, MODE><<<gr, 256, lr, s>>>(a, S, twx, x, x64);
 }
@@ -645,8 +689,8 @@ void fft_solve_f64(FftSolver* f, const double* b, int gmode, double* x,
 }
 
 void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
-                          int nref, double rtol, bool measure, double* stats,
-                          hipStream_t s) {
+                          int nref, bool warm, double rtol, bool measure,
+                          double* stats, hipStream_t s) {
   F2D_REQUIRE(f->S32, "FFT solver built without FP32 buffers");
   if (nref <= 0) {
     spectral<float, float, float, 0>(f, b, gmode, x, nullptr, s);
@@ -655,16 +699,33 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
       spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s);
       residual<float>(f, b, gmode, f->x64, nullptr, s, mean_of(f, b, gmode, s));
     }
-  } else {
+  } else if (!warm || !f->solved) {
     // The residual fed to the FP32 transforms is formed in FP64 inside the
     // row pass, and must not carry the mean of b (singular operator): next
     // to it the FP32 rounding would erase the residual itself.  The first
     // row pass records the FP64 row sums the later ones take the mean from.
-    spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s, true);
+    double* rs = f->rowsum[f->rs_cur];
+    spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s, rs);
     for (int i = 0; i < nref; i++)
-      spectral<float, float, float, 2>(f, b, gmode, x, f->x64, s, false, f->x64);
+      spectral<float, float, float, 2>(f, b, gmode, x, f->x64, s, nullptr, f->x64, nullptr,
+                                       rs);
+    if (measure) residual<float>(f, b, gmode, f->x64, nullptr, s, mean_of(f, b, gmode, s));
+  } else {
+    // Warm start: the first refinement starts from the caller's x.  Its
+    // row pass takes the mean from the previous call's row sums (a stale
+    // mean is harmless, see spectral) and records this b's sums for the
+    // later passes and the next call.
+    const bool keep64 = measure || nref > 1;
+    const int cur = f->rs_cur;
+    spectral<float, float, float, 3>(f, b, gmode, x, keep64 ? f->x64 : nullptr, s,
+                                     f->rowsum[1 - cur], nullptr, x, f->rowsum[cur]);
+    f->rs_cur = 1 - cur;
+    for (int i = 1; i < nref; i++)
+      spectral<float, float, float, 2>(f, b, gmode, x, f->x64, s, nullptr, f->x64, nullptr,
+                                       f->rowsum[f->rs_cur]);
     if (measure) residual<float>(f, b, gmode, f->x64, nullptr, s, mean_of(f, b, gmode, s));
   }
+  f->solved = true;
   if (stats && measure)
     k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.nx * f->a.ny,
                                   f->a.singular, rtol, nref, measure ? 1 : 0, stats);
@@ -677,12 +738,14 @@ namespace f2d {
 // Mean launch time (us, HIP events on `s`) of one pass of the spectral solve
 // on scratch data, and its algorithmic bytes: kind 0 rows forward (1: the
 // refinement's residual form, FP32 only), 2 columns, 3 rows inverse (4: the
-// refinement's accumulating form, FP32 only).
+// refinement's accumulating form, FP32 only); FP32 warm start: 5 rows
+// forward of the residual of an FP32 guess with the row sums of b, 6 rows
+// inverse accumulating onto the FP32 guess.
 double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
                  double* bytes) {
   F2D_REQUIRE(f64 ? f->S64 != nullptr : f->S32 != nullptr,
               "FFT solver has no buffers of this precision");
-  F2D_REQUIRE(f64 ? (kind == 0 || kind == 2 || kind == 3) : (kind >= 0 && kind <= 4),
+  F2D_REQUIRE(f64 ? (kind == 0 || kind == 2 || kind == 3) : (kind >= 0 && kind <= 6),
               "FFT bench kind");
   const size_t npad = (size_t)(f->a.ny + 6) * (f->a.nx + 6);
   if (!f->bench_buf) f->bench_buf = falloc<double>(f, sizeof(double) * npad);
@@ -701,7 +764,8 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
       double* buf = f->bench_buf;
       if (kind == 0)
         k_fft_rows_fwd<double, double><<<gr, 256, lr, s>>>(a, buf, 0, f->S64, f->twx64,
-                                                           nullptr, nullptr, nullptr, gr);
+                                                           nullptr, nullptr, nullptr, gr,
+                                                           nullptr);
       else if (kind == 2)
         k_fft_cols<double><<<gc, 256, lc, s>>>(a, f->S64, f->twy64, f->lamx, f->lamy);
       else
@@ -712,15 +776,22 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
       if (kind == 0 || kind == 1)
         k_fft_rows_fwd<float, float><<<gr, 256, lr, s>>>(
             a, buf, 0, f->S32, f->twx32, nullptr, kind == 1 ? f->x64 : nullptr,
-            nullptr, gr);
+            nullptr, gr, nullptr);
       else if (kind == 2)
         k_fft_cols<float><<<gc, 256, lc, s>>>(a, f->S32, f->twy32, f->lamx, f->lamy);
       else if (kind == 3)
         k_fft_rows_inv<float, float, 0><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
                                                             nullptr);
-      else
+      else if (kind == 4)
         k_fft_rows_inv<float, float, 2><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
                                                             f->x64);
+      else if (kind == 5)
+        k_fft_rows_fwd<float, float><<<gr, 256, lr, s>>>(
+            a, buf, 0, f->S32, f->twx32, f->rowsum[1 - f->rs_cur], nullptr,
+            a.singular ? f->rowsum[f->rs_cur] : nullptr, gr, buf);
+      else
+        k_fft_rows_inv<float, float, 3><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
+                                                            nullptr);
     }
   };
   launch();
@@ -739,7 +810,9 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
     case 1: by = N * w + N * 8.0 + spec; break;
     case 2: by = 2.0 * spec; break;
     case 3: by = spec + N * w; break;
-    default: by = spec + N * 8.0 * 2.0 + N * w; break;
+    case 4: by = spec + N * 8.0 * 2.0 + N * w; break;
+    case 5: by = 2.0 * N * w + spec; break;
+    default: by = spec + 2.0 * N * w; break;
   }
   *bytes = by;
   return 1e3 * (double)ms / reps;

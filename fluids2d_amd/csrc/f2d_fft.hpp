@@ -33,9 +33,12 @@ void fft_solve_f64(FftSolver* f, const double* b, int gmode, double* x,
 
 // FP32 transforms with FP64 iterative refinement: x64 = S32(b), then
 // `nref` times r = b - A x64 (FP64), x64 += S32(r); x = (float) x64.
+// warm (from the handle's second solve on): no cold solve, the refinement
+// starts from the caller's x (its interior; the halo is not read):
+// x64 = x + S32(b - A x), then nref - 1 further refinements.
 void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
-                          int nref, double rtol, bool measure, double* stats,
-                          hipStream_t s);
+                          int nref, bool warm, double rtol, bool measure,
+                          double* stats, hipStream_t s);
 
 // timing hook (see f2d_poisson_bench_kernel)
 double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,

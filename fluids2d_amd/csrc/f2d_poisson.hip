@@ -2197,7 +2197,7 @@ int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,
                              double* bytes, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(h && us && bytes, "NULL argument");
-    F2D_REQUIRE(kind >= 0 && kind <= 7, "kind must be in 0 .. 7");
+    F2D_REQUIRE(kind >= 0 && kind <= 9, "kind must be in 0 .. 9");
     F2D_REQUIRE(reps > 0, "reps must be > 0");
     hipStream_t s = S(stream);
     if (kind >= 3) {
@@ -2349,7 +2349,8 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
         const int nref = mode == F2D_POISSON_F32
                              ? 0
                              : (h->opts.fft_refine > 0 ? h->opts.fft_refine : 1);
-        fft_solve_f32_refine(h->fft, (const float*)b, gmode, (float*)x, nref, rtol,
+        fft_solve_f32_refine(h->fft, (const float*)b, gmode, (float*)x, nref,
+                             h->opts.fft_warm != 0, rtol,
                              sync, c->stats, s);
       }
       if (!sync) h->fft_unmeasured++;

@@ -527,6 +527,11 @@ __global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ Ux,
 }
 
 __global__ void k_set_scalar(double* d, double v) { *d = v; }
+__global__ void k_copy_f64(const double* __restrict__ src, double* __restrict__ dst,
+                           int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
 
 // Per-device scratch of the grid reductions of this file: partial maxima
 // written by one kernel and read by the one-block finaliser that follows it
@@ -770,6 +775,13 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
 
 int f2d_set_scalar(double* dst, double value, void* stream) {
   return guarded([&] { k_set_scalar<<<1, 1, 0, S(stream)>>>(dst, value); });
+}
+
+int f2d_copy_f64(double* dst, const double* src, int n, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(n >= 0 && (n == 0 || (dst && src)), "bad copy arguments");
+    if (n > 0) k_copy_f64<<<(n + 255) / 256, 256, 0, S(stream)>>>(src, dst, n);
+  });
 }
 
 }  // extern "C"

@@ -24,7 +24,8 @@ import torch
 from . import _lib
 from . import operators as op
 from .equations import get_device_model
-from .states import (Prognostic, allocate_device, download, get_specs, upload)
+from .states import (Prognostic, allocate_device, download, get_specs, leaves,
+                     upload)
 
 rkdocs = {
     "rk3": "Strongly Stably Preserving RK3 (Shu 2003)",
@@ -64,6 +65,11 @@ class RKIntegrator:
         self.__doc__ = (self.__doc__ or "") + f"\n{rkdocs[param.integrator]}"
         self.rhs = self._host_rhs
         self.diag = self._host_diag
+        self._graphs = {}
+        self._graph_solves = []
+        self._glog = None
+        self._eager_steps = 0
+        self._graph_mesh = None
 
     # -- host API (reference semantics) ----------------------------------------
     def _host_rhs(self, s, ds):
@@ -142,27 +148,102 @@ class RKIntegrator:
         self.advance()
         time.pushforward()
 
-    def device_steps(self, nsteps, stats=False):
-        """nsteps steps with dt kept on the device (no host round trip).
-        Returns the per-step dt as a device tensor."""
-        dts = torch.zeros(nsteps, dtype=torch.float64, device=self.d_dt.device)
+    def _select_dt(self):
+        """The dt of the next step into d_dt, on the device (Model.set_dt,
+        model.py:71-87)."""
         p = self.param
-        for n in range(nsteps):
-            if p.dt > 0:
-                op.set_scalar(self.d_dt, p.dt)
-            elif p.model == "rsw":
-                c = (p.g * p.H) ** 0.5
-                op.set_scalar(self.d_dt, min(p.cfl / (c / self.mesh.dx + c / self.mesh.dy),
-                                             p.dtmax))
-            else:
-                if not self._dtn_valid:
-                    op.cfl_dt(self.model.cfl_velocity(self.dstate), p.cfl, p.dtmax,
-                              self.d_dtn)
-                self.d_dt, self.d_dtn = self.d_dtn, self.d_dt
+        if p.dt > 0:
+            op.set_scalar(self.d_dt, p.dt)
+        elif p.model == "rsw":
+            c = (p.g * p.H) ** 0.5
+            op.set_scalar(self.d_dt, min(p.cfl / (c / self.mesh.dx + c / self.mesh.dy),
+                                         p.dtmax))
+        else:
+            if not self._dtn_valid:
+                op.cfl_dt(self.model.cfl_velocity(self.dstate), p.cfl, p.dtmax, self.d_dtn)
+            self.d_dt, self.d_dtn = self.d_dtn, self.d_dt
+
+    def device_steps(self, nsteps, stats=False, graph=None):
+        """nsteps steps with dt kept on the device (no host round trip).
+        Returns the per-step dt as a device tensor.
+
+        graph (default ``param.hipgraph``): after two eager steps, runs of
+        ``graph_steps`` steps are captured once into a HIP graph and
+        replayed; the launches are the eager ones, so results are
+        identical."""
+        dts = torch.zeros(nsteps, dtype=torch.float64, device=self.d_dt.device)
+        use = getattr(self.param, "hipgraph", True) if graph is None else graph
+        use = use and not stats
+        G = self.graph_steps
+        n = 0
+        while n < nsteps:
+            if use and nsteps - n >= G and self._eager_steps >= 2 and \
+                    (self._dtn_valid or not self._wants_dtn()):
+                g = self._graph()
+                g.replay()
+                for e, k in self._graph_solves:
+                    e.graph_replayed(k, 0)
+                _lib.call("f2d_copy_f64", _lib.ptr(dts[n:n + G]), _lib.ptr(self._glog), G,
+                          _lib.stream_ptr())
+                n += G
+                continue
+            self._select_dt()
             dts[n:n + 1].copy_(self.d_dt)
             self.advance(stats=stats)
+            self._eager_steps += 1
+            n += 1
         return dts
 
+    # -- HIP graphs -----------------------------------------------------------------
+    graph_steps = 4  # steps per captured graph; even, so ping-pong buffers return
+
+    def _graph_key(self):
+        if self._graph_mesh != self.mesh.version:
+            # a re-finalised mesh has new solvers and masks: drop old graphs
+            self._graphs = {}
+            self._graph_mesh = self.mesh.version
+        return (self.mesh.version, id(self.model),
+                tuple(t.data_ptr() for t in leaves(self.dstate)),
+                tuple(t.data_ptr() for t in leaves(self.dalt)),
+                self.d_dt.data_ptr(), self.d_dtn.data_ptr())
+
+    def _solvers(self):
+        m = self.mesh
+        out = []
+        for name in ("poisson_vertices", "qg_helmholtz"):
+            e = m.__dict__.get(name)
+            if e is not None:
+                out.append(e)
+        return out
+
+    def _graph(self):
+        """The captured graph of graph_steps steps for the current buffer
+        assignment (captured on first use)."""
+        key = self._graph_key()
+        g = self._graphs.get(key)
+        if g is not None:
+            return g
+        G = self.graph_steps
+        if self._glog is None:
+            self._glog = torch.zeros(G, dtype=torch.float64, device=self.d_dt.device)
+        sols = self._solvers()
+        before = [e.calls for e in sols]
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for i in range(G):
+                self._select_dt()
+                _lib.call("f2d_copy_f64", _lib.ptr(self._glog[i:i + 1]), _lib.ptr(self.d_dt), 1,
+                          _lib.stream_ptr())
+                self.advance(stats=False)
+        if self._graph_key() != key:
+            raise RuntimeError("graph capture did not return the state buffers to "
+                               "their roles (odd number of buffer swaps)")
+        self._graph_solves = [(e, e.calls - c0) for e, c0 in zip(sols, before)]
+        for e, k in self._graph_solves:
+            e.graph_replayed(0, k)
+        self._graphs[key] = g
+        return g
 
 def _one(x):
     """The single prognostic array of a one-field prognostic namedtuple."""

@@ -48,6 +48,9 @@ class Mesh:
 
     # -- reference API ----------------------------------------------------
     def finalize(self):
+        # bumped on every finalize: captured step graphs hold the solvers
+        # and mask arrays of one version
+        self.version = getattr(self, "version", 0) + 1
         self._check_mask()
         self.set_masks()
         self.slipcoef = self._slipcoef()

@@ -15,6 +15,10 @@ Fields added for the MI355X path (declared in the constructor, so that
 * ``poisson_solver`` "auto" (direct FFT solve for the fully wet doubly
                     periodic operator with power-of-two sizes, multigrid CG
                     otherwise), "mg" or "fft"
+* ``poisson_warm``  FP32 direct (FFT) solves: the FP64-residual refinement
+                    starts from the solution held in x (the previous stage's
+                    streamfunction) instead of a cold FP32 solve
+* ``hipgraph``      replay runs of device steps from captured HIP graphs
 * ``device``        torch device index of the GPU
 """
 _models = ["euler", "eulerpsi", "advection", "vectoradv",
@@ -88,6 +92,8 @@ class Param:
         self.poisson_maxit = None
         self.poisson_nu = 2
         self.poisson_solver = "auto"
+        self.poisson_warm = True
+        self.hipgraph = True
         self.device = 0
 
         self.__parameters__ = get_parameters(self)

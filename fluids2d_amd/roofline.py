@@ -44,21 +44,27 @@ FFT_KINDS = {3: "k_fft_rows_fwd (row FFTs of the right-hand side)",
              4: "k_fft_rows_fwd (row FFTs of the FP64 refinement residual)",
              5: "k_fft_cols (column FFT, eigenvalue division, inverse FFT)",
              6: "k_fft_rows_inv (inverse row FFTs, halo images)",
-             7: "k_fft_rows_inv (inverse row FFTs accumulating the refinement)"}
+             7: "k_fft_rows_inv (inverse row FFTs accumulating the refinement)",
+             8: "k_fft_rows_fwd (row FFTs of the FP64 residual of the previous solution)",
+             9: "k_fft_rows_inv (inverse row FFTs accumulating onto the previous solution)"}
 
 
 def profile_dominant(model, reps=20):
     """Time the kernels of one RK3 step live (HIP events on the stream they
     run on) and pick the dominant one: the largest mean launch time x
     launches per step.  Launch counts come from the solver's schedule (FFT:
-    one solve per stage, one FP64 refinement in FP32 mode; MG: the
-    iterations of the last solves)."""
+    one solve per stage; in FP32 one FP64-residual refinement, from the
+    previous solution or after a cold FP32 solve; MG: the iterations of the
+    last solves)."""
     pois = model.mesh.poisson_vertices
     prec = model.param.precision
     rows = {}
     if pois.method == "fft":
         if prec == "fp64":
             per = {3: 3, 5: 3, 6: 3}
+        elif getattr(model.param, "poisson_warm", True):
+            # one FP64-residual refinement from the previous solution
+            per = {8: 3, 5: 3, 9: 3}
         else:
             per = {3: 3, 4: 3, 5: 6, 6: 3, 7: 3}
         for kind, n in per.items():

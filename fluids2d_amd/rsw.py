@@ -21,6 +21,7 @@ class _SW(_Base):
     def __init__(self, param, mesh):
         super().__init__(param, mesh)
         self._ds_last = None
+        self._stats = True  # host-checked solves unless a stage says otherwise
 
     def _grid(self):
         return self.mesh.grid_ref()
@@ -66,7 +67,11 @@ class _SW(_Base):
             ds = self._ds_last
         else:
             ds = ds_out
-        self.rhs(s, ds)
+        self._stats = stats
+        try:
+            self.rhs(s, ds)
+        finally:
+            self._stats = True
         ys = leaves(s.u) + [s.h]
         dsl = leaves(ds.u) + [ds.h]
         prev = [leaves(x.u) + [x.h] for x in ds_prev]
@@ -123,7 +128,7 @@ class QGRSW(_SW):
              _lib.ptr(m.d_slipcoef), _lib.ptr(s.pv))
         _ops("f2d_c2v_scaled", dt, g, _lib.ptr(ds.h), float(-p.f0 / p.H), _lib.ptr(s.pv),
              _lib.ptr(m.d_mskv), 1)
-        m.qg_helmholtz.solve(s.pv, s.psi)
+        m.qg_helmholtz.solve(s.pv, s.psi, stats=self._stats)
         _ops("f2d_v2c_scaled", dt, g, _lib.ptr(s.psi), float(p.f0 * m.area / p.g),
              _lib.ptr(m.d_mskv), _lib.ptr(m.d_msk), _lib.ptr(ds.h))
         op.perpgrad(m, s.psi, ds.u, contravariant=False)

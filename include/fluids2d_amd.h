@@ -155,6 +155,10 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
 /* dt = value (device scalar): the fixed-dt and rsw branches of Model.set_dt,
  * model.py:71-82 */
 int f2d_set_scalar(double* dst, double value, void* stream);
+/* dst[0:n] = src[0:n] on the device, as a kernel (a graph-capturable copy
+ * of device scalars such as the step's dt into a log): the device form of
+ * the per-step dt the host reads in Model.set_dt, model.py:71-87 */
+int f2d_copy_f64(double* dst, const double* src, int n, void* stream);
 
 /* ------------------------------------------------------------------------
  * C-grid point operators of the shallow-water models (operators.py), each
@@ -241,6 +245,10 @@ typedef struct f2d_poisson_opts {
   int32_t nu_tail;       /* sweeps on the tail levels (0: nu)              */
   int32_t solver;        /* F2D_SOLVER_*: 0 auto, 1 multigrid CG, 2 FFT      */
   int32_t fft_refine;    /* FFT in FP32 modes: FP64 refinements (0: 1)     */
+  int32_t fft_warm;      /* FFT in F32_REFINE: 1 = from the handle's second
+                            solve on, the first refinement starts from x
+                            (its interior) instead of a cold FP32 solve;
+                            0 = cold (default)                             */
 } f2d_poisson_opts;
 
 /* Solver selection.  AUTO takes the direct spectral (FFT) solve when every
@@ -299,10 +307,12 @@ int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream);
  * kernel of the CG iteration `reps` times on the handle's current vectors --
  * kind 0 the update/pre-smooth/restrict pass, 1 the prolongate/post-smooth
  * pass, 2 the search-direction/operator pass -- with a scratch control block
- * and scratch outputs, not touching the caller's solution.  Kinds 3..7
+ * and scratch outputs, not touching the caller's solution.  Kinds 3..9
  * time the passes of the FFT solve instead (3 rows forward, 4 rows forward
  * forming the FP64 refinement residual, 5 columns, 6 rows inverse, 7 rows
- * inverse accumulating the refinement; 4 and 7 in FP32 modes only).  *us receives
+ * inverse accumulating the refinement, 8 rows forward of the warm start's
+ * residual of an FP32 guess, 9 rows inverse accumulating onto the FP32
+ * guess; 4, 7, 8 and 9 in FP32 modes only).  *us receives
  * the mean launch time from HIP events on `stream`, *bytes the algorithmic
  * bytes of one launch (each array read once, written once).  Synchronises. */
 int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,

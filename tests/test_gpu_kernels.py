@@ -329,6 +329,68 @@ def test_poisson_fft_f32_modes(dev, L, mode, ny, nx):
         assert err < 1e-5
 
 
+@pytest.mark.parametrize("ny,nx,maindiag,nref,bmean", [
+    (256, 256, 0.0, 1, 0.0), (128, 512, 0.0, 1, 3.0), (256, 128, 0.4, 1, 0.0),
+    (512, 512, 0.0, 2, 0.5)])
+def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean):
+    """Warm-started FP32 refinement (opts.fft_warm): from the handle's
+    second solve on, the first refinement starts from the x passed in.
+    Solve b1 (cold), then b2 = b1 + 1% noise from the first solution (halo
+    of x spoiled: only its interior may be read), then b3 unrelated to b2,
+    each against SuperLU on the same b; b may carry a mean, which the
+    singular operator drops."""
+    import torch
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True,
+          Lx=nx / 256.0, Ly=ny / 256.0)
+    m = O.default_mask(ny, nx, 3, True, True)
+    mesh = O.Mesh(p, msk=m)
+    unk = O.unknown_mask(p, mesh.mskv)
+    rng = np.random.default_rng(9)
+    g = L.make_grid(p)
+    opts = L.PoissonOpts()
+    opts.mode, opts.solver, opts.fft_refine, opts.fft_warm = \
+        L.POISSON_F32_REFINE, L.SOLVERS["fft"], nref, 1
+    h = ctypes.c_void_p()
+    L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+           maindiag, ctypes.byref(opts), ctypes.byref(h))
+    ref = O.Poisson2D(p, mesh, "v", maindiag=maindiag)
+    dx = torch.zeros(m.shape, dtype=torch.float32, device=dev)
+    b1 = (rng.standard_normal(m.shape) + bmean) * mesh.mskv
+    b2 = (b1 + 0.01 * rng.standard_normal(m.shape)) * mesh.mskv
+    b3 = (rng.standard_normal(m.shape) + bmean) * mesh.mskv
+    errs = []
+    for k, b in enumerate((b1, b2, b3)):
+        if k == 1:
+            dx[:3, :] = 1e6
+            dx[:, -3:] = -1e6
+        x_ref = np.zeros(m.shape)
+        ref.solve(b, x_ref)
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        L.call("f2d_poisson_solve", h, L.ptr(T(b, dev).float()), L.ptr(dx), 1e-10, 6,
+               ctypes.byref(it), ctypes.byref(rr), L.stream_ptr())
+        x = N(dx).astype(np.float64)
+        xi, xr = x[3:-3, 3:-3], x_ref[3:-3, 3:-3]
+        if maindiag == 0.0:  # the singular operator's solution is up to a constant
+            xi, xr = xi - xi.mean(), xr - xr.mean()
+        err = np.linalg.norm(xi - xr) / np.linalg.norm(xr)
+        errs.append((err, rr.value))
+        print(f"\nwarm {ny}x{nx} md={maindiag} nref={nref} solve {k}: relres={rr.value:.2e} "
+              f"err={err:.2e}")
+        # halo images of the periodic solution
+        assert np.array_equal(x[:3, 3:-3], x[-6:-3, 3:-3])
+        assert np.array_equal(x[3:-3, :3], x[3:-3, -6:-3])
+    L.call("f2d_poisson_destroy", h)
+    # k=0: cold FP32 solve + nref FP64-residual refinements; k=1: from the
+    # nearby solution the refinement's FP32 error is 1% of a cold solve's;
+    # k=2: from an unrelated solution one refinement is an FP32 solve (its
+    # FP64 residual is that of FP32 rounding of the high wavenumbers)
+    for err, rr in errs:
+        assert err < 2e-6, errs
+    assert errs[0][1] < 1e-9, errs
+    assert errs[1][1] < 3e-7, errs
+    assert errs[2][1] < (1e-9 if nref > 1 else 1e-4), errs
+
+
 def test_poisson_fft_matches_mg(dev, L):
     p = P(model="eulerpsi", nx=256, ny=256, xperiodic=True, yperiodic=True)
     m = O.default_mask(256, 256, 3, True, True)
