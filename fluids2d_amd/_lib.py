@@ -26,7 +26,7 @@ EXPORTS = [
     "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
     "f2d_poisson_method", "f2d_coriolis", "f2d_addgrad", "f2d_sharp", "f2d_vorticity",
     "f2d_ke_finish", "f2d_pressure", "f2d_div", "f2d_c2v_scaled", "f2d_v2c_scaled",
-    "f2d_adj_transport", "f2d_adj_perpgrad", "f2d_adj_c2v",
+    "f2d_adj_stage", "f2d_adj_combine",
 ]
 
 
@@ -106,9 +106,9 @@ def lib():
         "f2d_div": (i32, [i32, gp, vp, vp, vp, vp, vp]),
         "f2d_c2v_scaled": (i32, [i32, gp, vp, d, vp, vp, i32, vp]),
         "f2d_v2c_scaled": (i32, [i32, gp, vp, d, vp, vp, vp, vp]),
-        "f2d_adj_transport": (i32, [i32, gp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-        "f2d_adj_perpgrad": (i32, [i32, gp, vp, vp, vp, vp]),
-        "f2d_adj_c2v": (i32, [i32, gp, vp, vp, vp]),
+        "f2d_adj_stage": (i32, [i32, gp, vp, vp, vp, vp, vp, vp, d, d, d, vp, vp, vp, vp,
+                                vp, vp]),
+        "f2d_adj_combine": (i32, [i32, gp, vp, vp, vp, vp, vp]),
         "f2d_poisson_solve": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
                                     ctypes.POINTER(d), vp]),
         "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),

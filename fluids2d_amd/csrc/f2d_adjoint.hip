@@ -8,21 +8,25 @@
 // fully wet periodic grid).  Given the adjoint lam of R(s), this file
 // computes R'(s)^T lam = T_q^T lam + U'^T T_U^T lam:
 //
-//   k_adj_faces   per face f: lamF = lam[f] - lam[f-1] (the two cells the
-//                 face bounds), lamU[f] = lamF r_f, and the six stencil
-//                 weights W[m][f] = U_f lamF dr_f/dq_{f+m-3}
-//   k_adj_gather  per cell c: sum over the faces whose stencils hold c of
-//                 W[m][c-(m-3)] in x and y (a gather: deterministic, no atomics)
-//   k_adj_perp    perpgrad^T of lamU
+//   k_adj_faces        per face f: lamF = lam[f] - lam[f-1] (the two cells
+//                      the face bounds), lamU[f] = lamF r_f, and the six
+//                      stencil weights W[m][f] = U_f lamF dr_f/dq_{f+m-3};
+//                      lam is formed on the fly from the RK combination of
+//                      the stage outputs' adjoints
+//   k_adj_gather_perp  per cell c: sum over the faces whose stencils hold c
+//                      of W[m][c-(m-3)] in x and y (a gather: deterministic,
+//                      no atomics), and perpgrad^T of lamU
 //   (A^-1 is self-adjoint: the same solve as the forward one)
-//   k_adj_c2v     c2v^T, added to the gathered term
+//   k_adj_combine      c2v^T of the solve, plus the gathered term, plus the
+//                      adjoint carried from the next stage
 //
 // dr/dq of WENO-Z5 follows the chain rule through the candidate values,
 // the smoothness indicators beta_k, tau5 = |beta1 - beta3| and the weights
 // (sign(beta1 - beta3) at the kink, 0 when equal); the upwind selection is
 // piecewise constant in U.  All arrays use the padded (ny+6) x (nx+6)
-// layout; outputs are written on the interior, inputs need their periodic
-// halos filled.
+// layout; outputs are written on the interior; adjoint inputs are read on
+// the interior (periodic neighbours wrapped), the forward state q and U
+// need their periodic halos filled.
 #include "f2d_common.hpp"
 
 namespace f2d {
@@ -85,12 +89,30 @@ __device__ __forceinline__ int wrapi(int v, int n) {
   return v < 0 ? v + n : (v >= n ? v - n : v);
 }
 
-// x faces and y faces at interior point (y, x): lamU and the stencil weights
+// lam of one RK stage's tendency: the combination c0 l0 + c1 l1 + c2 l2 of
+// the adjoints of the stage outputs (an absent term has a NULL array)
+template <class T>
+struct LamComb {
+  const T* l0;
+  const T* l1;
+  const T* l2;
+  T c0, c1, c2;
+  __device__ __forceinline__ T at(int64_t k) const {
+    T v = c0 * l0[k];
+    if (l1) v = v + c1 * l1[k];
+    if (l2) v = v + c2 * l2[k];
+    return v;
+  }
+};
+
+// x faces and y faces at interior point (y, x): lamU and the stencil weights.
+// lam is read at interior points only (periodic neighbours wrapped); q, Ux,
+// Uy need their periodic halos filled.
 template <class T>
 __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs a, const T* __restrict__ q,
                                                    const T* __restrict__ Ux,
                                                    const T* __restrict__ Uy,
-                                                   const T* __restrict__ lam,
+                                                   LamComb<T> lam,
                                                    T* __restrict__ lUx,
                                                    T* __restrict__ lUy,
                                                    T* __restrict__ Wx,
@@ -101,9 +123,12 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs a, const T* __restric
   const int y = (int)(idx / a.nx), x = (int)(idx - (int64_t)y * a.nx);
   const int64_t P = a.NXP;
   const int64_t k = (int64_t)(y + 3) * P + x + 3;
+  const T lc = lam.at(k);
+  const T lw = lam.at((int64_t)(y + 3) * P + wrapi(x - 1, a.nx) + 3);
+  const T ls = lam.at((int64_t)(wrapi(y - 1, a.ny) + 3) * P + x + 3);
   // x face between cells (y, x-1) and (y, x)
   {
-    const T lf = lam[k] - lam[k - 1];
+    const T lf = lc - lw;
     const T u = Ux[k];
     T xs[5], dr[5];
     const bool pos = u > T(0);
@@ -127,7 +152,7 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs a, const T* __restric
   }
   // y face between cells (y-1, x) and (y, x)
   {
-    const T lf = lam[k] - lam[k - P];
+    const T lf = lc - ls;
     const T u = Uy[k];
     T xs[5], dr[5];
     const bool pos = u > T(0);
@@ -149,15 +174,24 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs a, const T* __restric
   }
 }
 
-// lq[c] = sum_m Wx[m][c - (m-3)] + Wy[m][c - (m-3) rows], periodic
+// lq[c] = sum_m Wx[m][c - (m-3)] + Wy[m][c - (m-3) rows] (periodic), and
+// perpgrad^T of lamU: Ux[j,i] = sx (psi[j,i] - psi[j+1,i]), Uy[j,i] =
+// sy (psi[j,i+1] - psi[j,i])  =>  lpsi[j,i] = sx (lUx[j,i] - lUx[j-1,i]) +
+// sy (lUy[j,i-1] - lUy[j,i]) (periodic)
 template <class T>
-__global__ __launch_bounds__(256) void k_adj_gather(AdjArgs a, const T* __restrict__ Wx,
-                                                    const T* __restrict__ Wy,
-                                                    T* __restrict__ lq, int64_t plane) {
+__global__ __launch_bounds__(256) void k_adj_gather_perp(AdjArgs a, const T* __restrict__ Wx,
+                                                         const T* __restrict__ Wy,
+                                                         const T* __restrict__ lUx,
+                                                         const T* __restrict__ lUy,
+                                                         double sx, double sy,
+                                                         T* __restrict__ lq,
+                                                         T* __restrict__ lpsi,
+                                                         int64_t plane) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)a.ny * a.nx) return;
   const int y = (int)(idx / a.nx), x = (int)(idx - (int64_t)y * a.nx);
   const int64_t P = a.NXP;
+  const int64_t k = (int64_t)(y + 3) * P + x + 3;
   T acc = T(0);
 #pragma unroll
   for (int m = 0; m < 6; m++) {
@@ -169,31 +203,18 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjArgs a, const T* __restri
     const int yf = wrapi(y - (m - 3), a.ny);
     acc += Wy[m * plane + (int64_t)(yf + 3) * P + x + 3];
   }
-  lq[(int64_t)(y + 3) * P + x + 3] = acc;
-}
-
-// perpgrad^T: Ux[j,i] = sx (psi[j,i] - psi[j+1,i]), Uy[j,i] = sy (psi[j,i+1]
-// - psi[j,i])  =>  lpsi[j,i] = sx (lUx[j,i] - lUx[j-1,i]) + sy (lUy[j,i-1]
-// - lUy[j,i]) (periodic)
-template <class T>
-__global__ __launch_bounds__(256) void k_adj_perp(AdjArgs a, const T* __restrict__ lUx,
-                                                  const T* __restrict__ lUy, double sx,
-                                                  double sy, T* __restrict__ lpsi) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)a.ny * a.nx) return;
-  const int y = (int)(idx / a.nx), x = (int)(idx - (int64_t)y * a.nx);
-  const int64_t P = a.NXP;
-  const int64_t k = (int64_t)(y + 3) * P + x + 3;
+  lq[k] = acc;
   const int64_t km = (int64_t)(wrapi(y - 1, a.ny) + 3) * P + x + 3;
   const int64_t kw = (int64_t)(y + 3) * P + wrapi(x - 1, a.nx) + 3;
   lpsi[k] = T(sx) * (lUx[k] - lUx[km]) + T(sy) * (lUy[kw] - lUy[k]);
 }
 
-// c2v^T added to lq: lq[j,i] += 0.25 (lb[j,i] + lb[j+1,i] + lb[j,i+1] +
-// lb[j+1,i+1]) (periodic)
+// out = base + lq + c2v^T lb, c2v^T lb [j,i] = 0.25 (lb[j,i] + lb[j+1,i] +
+// lb[j,i+1] + lb[j+1,i+1]) (periodic); interior only (out may be base)
 template <class T>
-__global__ __launch_bounds__(256) void k_adj_c2v(AdjArgs a, const T* __restrict__ lb,
-                                                 T* __restrict__ lq) {
+__global__ __launch_bounds__(256) void k_adj_combine(AdjArgs a, const T* base,
+                                                     const T* __restrict__ lq,
+                                                     const T* __restrict__ lb, T* out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)a.ny * a.nx) return;
   const int y = (int)(idx / a.nx), x = (int)(idx - (int64_t)y * a.nx);
@@ -203,7 +224,7 @@ __global__ __launch_bounds__(256) void k_adj_c2v(AdjArgs a, const T* __restrict_
   const T s = ((lb[k] + lb[(int64_t)(yp + 3) * P + x + 3]) +
                lb[(int64_t)(y + 3) * P + xp + 3]) +
               lb[(int64_t)(yp + 3) * P + xp + 3];
-  lq[k] = lq[k] + T(0.25) * s;
+  out[k] = base[k] + (lq[k] + T(0.25) * s);
 }
 
 }  // namespace f2d
@@ -212,64 +233,56 @@ using namespace f2d;
 
 extern "C" {
 
-int f2d_adj_transport(int dtype, const f2d_grid* g, const void* q, const void* Ux,
-                      const void* Uy, const void* lam, void* lUx, void* lUy,
-                      void* work, void* lq, void* stream) {
+int f2d_adj_stage(int dtype, const f2d_grid* g, const void* q, const void* Ux,
+                  const void* Uy, const void* lam0, const void* lam1, const void* lam2,
+                  double c0, double c1, double c2, void* lUx, void* lUy, void* work,
+                  void* lq, void* lpsi, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(g && g->halowidth == 3 && g->xperiodic && g->yperiodic,
                 "adjoint: doubly periodic grid with halo 3 only");
-    F2D_REQUIRE(q && Ux && Uy && lam && lUx && lUy && work && lq, "NULL argument");
+    F2D_REQUIRE(q && Ux && Uy && lam0 && lUx && lUy && work && lq && lpsi,
+                "NULL argument");
+    F2D_REQUIRE(lam1 || !lam2, "lam2 given without lam1");
     AdjArgs a{(int)g->ny, (int)g->nx, (int)g->nx + 6};
     const int64_t plane = (g->ny + 6) * (g->nx + 6);
     const int64_t n = g->ny * g->nx;
-    hipStream_t s = S(stream);
-    if (dtype == F2D_F64) {
-      double* W = (double*)work;
-      k_adj_faces<double><<<grid1d(n, 256), 256, 0, s>>>(
-          a, (const double*)q, (const double*)Ux, (const double*)Uy, (const double*)lam,
-          (double*)lUx, (double*)lUy, W, W + 6 * plane, plane);
-      k_adj_gather<double><<<grid1d(n, 256), 256, 0, s>>>(a, W, W + 6 * plane,
-                                                          (double*)lq, plane);
-    } else if (dtype == F2D_F32) {
-      float* W = (float*)work;
-      k_adj_faces<float><<<grid1d(n, 256), 256, 0, s>>>(
-          a, (const float*)q, (const float*)Ux, (const float*)Uy, (const float*)lam,
-          (float*)lUx, (float*)lUy, W, W + 6 * plane, plane);
-      k_adj_gather<float><<<grid1d(n, 256), 256, 0, s>>>(a, W, W + 6 * plane,
-                                                         (float*)lq, plane);
-    } else {
-      throw Error{"dtype must be F2D_F64 or F2D_F32"};
-    }
-  });
-}
-
-int f2d_adj_perpgrad(int dtype, const f2d_grid* g, const void* lUx, const void* lUy,
-                     void* lpsi, void* stream) {
-  return guarded([&] {
-    F2D_REQUIRE(g && g->xperiodic && g->yperiodic, "adjoint: doubly periodic only");
-    AdjArgs a{(int)g->ny, (int)g->nx, (int)g->nx + 6};
-    const int64_t n = g->ny * g->nx;
     const double sx = 1.0 / (g->dy * g->dy), sy = 1.0 / (g->dx * g->dx);
+    hipStream_t s = S(stream);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      T* W = (T*)work;
+      LamComb<T> lc{(const T*)lam0, (const T*)lam1, (const T*)lam2, T(c0), T(c1), T(c2)};
+      k_adj_faces<T><<<grid1d(n, 256), 256, 0, s>>>(a, (const T*)q, (const T*)Ux,
+                                                    (const T*)Uy, lc, (T*)lUx, (T*)lUy, W,
+                                                    W + 6 * plane, plane);
+      k_adj_gather_perp<T><<<grid1d(n, 256), 256, 0, s>>>(a, W, W + 6 * plane,
+                                                          (const T*)lUx, (const T*)lUy, sx,
+                                                          sy, (T*)lq, (T*)lpsi, plane);
+    };
     if (dtype == F2D_F64)
-      k_adj_perp<double><<<grid1d(n, 256), 256, 0, S(stream)>>>(
-          a, (const double*)lUx, (const double*)lUy, sx, sy, (double*)lpsi);
+      run(double());
+    else if (dtype == F2D_F32)
+      run(float());
     else
-      k_adj_perp<float><<<grid1d(n, 256), 256, 0, S(stream)>>>(
-          a, (const float*)lUx, (const float*)lUy, sx, sy, (float*)lpsi);
+      throw Error{"dtype must be F2D_F64 or F2D_F32"};
   });
 }
 
-int f2d_adj_c2v(int dtype, const f2d_grid* g, const void* lb, void* lq, void* stream) {
+int f2d_adj_combine(int dtype, const f2d_grid* g, const void* base, const void* lq,
+                    const void* lb, void* out, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(g && g->xperiodic && g->yperiodic, "adjoint: doubly periodic only");
+    F2D_REQUIRE(base && lq && lb && out, "NULL argument");
     AdjArgs a{(int)g->ny, (int)g->nx, (int)g->nx + 6};
     const int64_t n = g->ny * g->nx;
     if (dtype == F2D_F64)
-      k_adj_c2v<double><<<grid1d(n, 256), 256, 0, S(stream)>>>(a, (const double*)lb,
-                                                                (double*)lq);
+      k_adj_combine<double><<<grid1d(n, 256), 256, 0, S(stream)>>>(
+          a, (const double*)base, (const double*)lq, (const double*)lb, (double*)out);
+    else if (dtype == F2D_F32)
+      k_adj_combine<float><<<grid1d(n, 256), 256, 0, S(stream)>>>(
+          a, (const float*)base, (const float*)lq, (const float*)lb, (float*)out);
     else
-      k_adj_c2v<float><<<grid1d(n, 256), 256, 0, S(stream)>>>(a, (const float*)lb,
-                                                               (float*)lq);
+      throw Error{"dtype must be F2D_F64 or F2D_F32"};
   });
 }
 

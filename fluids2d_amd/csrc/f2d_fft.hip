@@ -302,8 +302,14 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
                                                       const cplx<T>* __restrict__ S,
                                                       const cplx<T>* __restrict__ tw,
                                                       TX* __restrict__ x,
-                                                      double* __restrict__ x64) {
+                                                      double* __restrict__ x64,
+                                                      const double* __restrict__ rs_src =
+                                                          nullptr,
+                                                      double* __restrict__ rs_dst = nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // the row sums of this call's b become the next call's (one per workgroup;
+  // the forward row pass has the same grid)
+  if (rs_dst && threadIdx.x == 0) rs_dst[blockIdx.x] = rs_src[blockIdx.x];
   const int nx = a.nx, P = a.P, lg = a.lgx, h = nx >> 1;
   cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* b1 = b0 + P * nx;
@@ -489,10 +495,9 @@ struct FftSolver {
   double* part = nullptr;
   double* part_b = nullptr;
   double* mean_b = nullptr;
-  // per row-workgroup FP64 sums of b, double-buffered: a warm-started
-  // solve reads the previous call's while it writes its own
+  // per row-workgroup FP64 sums of b: [1] this call's, [0] the previous
+  // call's (a warm-started solve reads [0] while it writes [1])
   double* rowsum[2] = {nullptr, nullptr};
-  int rs_cur = 0;
   bool solved = false;  // a warm start needs an earlier solve of this handle
   double* bench_buf = nullptr;  // padded scratch for kernel timing
   int nbres = 0;
@@ -645,7 +650,7 @@ template <class T, class TB, class TX, int MODE>
 static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
                      hipStream_t s, double* rs_out = nullptr,
                      const double* xres = nullptr, const float* xres32 = nullptr,
-                     const double* rs_in = nullptr) {
+                     const double* rs_in = nullptr, bool keep_sums = false) {
   FftArgs a = f->a;
   const bool d = sizeof(T) == 8;
   a.P = d ? f->P64 : f->P32;
@@ -659,7 +664,9 @@ static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
   k_fft_rows_fwd<T, TB><<<gr, 256, lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
                                             xres32);
   k_fft_cols<T><<<gc, 256, lc, s>>>(a, S, twy, f->lamx, f->lamy);
-  k_fft_rows_inv<T, TX, MODE><<<gr, 256, lr, s>>>(a, S, twx, x, x64);
+  k_fft_rows_inv<T, TX, MODE><<<gr, 256, lr, s>>>(a, S, twx, x, x64,
+                                                  keep_sums ? f->rowsum[1] : nullptr,
+                                                  keep_sums ? f->rowsum[0] : nullptr);
 }
 
 template <class TB>
@@ -703,26 +710,27 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
     // The residual fed to the FP32 transforms is formed in FP64 inside the
     // row pass, and must not carry the mean of b (singular operator): next
     // to it the FP32 rounding would erase the residual itself.  The first
-    // row pass records the FP64 row sums the later ones take the mean from.
-    double* rs = f->rowsum[f->rs_cur];
-    spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s, rs);
+    // row pass records the FP64 row sums (rowsum[1]) the later ones take the
+    // mean from; the last pass keeps them for the next call (rowsum[0]).
+    double* rs = f->rowsum[1];
+    spectral<float, float, float, 1>(f, b, gmode, x, f->x64, s, rs, nullptr, nullptr,
+                                     nullptr, nref == 0);
     for (int i = 0; i < nref; i++)
       spectral<float, float, float, 2>(f, b, gmode, x, f->x64, s, nullptr, f->x64, nullptr,
-                                       rs);
+                                       rs, i == nref - 1);
     if (measure) residual<float>(f, b, gmode, f->x64, nullptr, s, mean_of(f, b, gmode, s));
   } else {
     // Warm start: the first refinement starts from the caller's x.  Its
-    // row pass takes the mean from the previous call's row sums (a stale
-    // mean is harmless, see spectral) and records this b's sums for the
-    // later passes and the next call.
+    // row pass takes the mean from the previous call's row sums (rowsum[0];
+    // a stale mean is harmless, see spectral) and records this b's sums
+    // (rowsum[1]) for the later passes; the last pass keeps them for the
+    // next call.  Fixed buffers: a captured graph replays the same.
     const bool keep64 = measure || nref > 1;
-    const int cur = f->rs_cur;
     spectral<float, float, float, 3>(f, b, gmode, x, keep64 ? f->x64 : nullptr, s,
-                                     f->rowsum[1 - cur], nullptr, x, f->rowsum[cur]);
-    f->rs_cur = 1 - cur;
+                                     f->rowsum[1], nullptr, x, f->rowsum[0], nref == 1);
     for (int i = 1; i < nref; i++)
       spectral<float, float, float, 2>(f, b, gmode, x, f->x64, s, nullptr, f->x64, nullptr,
-                                       f->rowsum[f->rs_cur]);
+                                       f->rowsum[1], i == nref - 1);
     if (measure) residual<float>(f, b, gmode, f->x64, nullptr, s, mean_of(f, b, gmode, s));
   }
   f->solved = true;
@@ -787,8 +795,8 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
                                                             f->x64);
       else if (kind == 5)
         k_fft_rows_fwd<float, float><<<gr, 256, lr, s>>>(
-            a, buf, 0, f->S32, f->twx32, f->rowsum[1 - f->rs_cur], nullptr,
-            a.singular ? f->rowsum[f->rs_cur] : nullptr, gr, buf);
+            a, buf, 0, f->S32, f->twx32, f->rowsum[1], nullptr,
+            a.singular ? f->rowsum[0] : nullptr, gr, buf);
       else
         k_fft_rows_inv<float, float, 3><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
                                                             nullptr);

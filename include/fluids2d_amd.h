@@ -206,20 +206,26 @@ int f2d_v2c_scaled(int dtype, const f2d_grid* g, const void* vh, double scale,
  * counterpart in the reference, which has no derivatives): the stage
  * evaluates R(s) = -div(U r(s)) with U = perpgrad(A^-1 c2v(s)) (weno.py:346-
  * 364, operators.py:126-149, elliptic.py:80-87); these entry points build
- * R'(s)^T lam.  Inputs need their periodic halos filled; outputs are written
- * on the interior of padded arrays.
+ * R'(s)^T lam in two calls around the solve: f2d_adj_stage (transport and
+ * perpgrad reversed), the Poisson solve of lpsi (A^-1 is self-adjoint), then
+ * f2d_adj_combine (c2v reversed).  Adjoint arrays are read on the interior
+ * (periodic neighbours wrapped) and written on the interior of padded
+ * arrays; the forward state q, Ux, Uy need their periodic halos filled.
  * ------------------------------------------------------------------------ */
 /* Transport part (compflux/div of weno.py:346-364, operators.py:21-29,
- * reversed): lq = (dT/dq)^T lam, lUx/lUy = (dT/dU)^T lam.  work: 12 padded
- * arrays of scratch. */
-int f2d_adj_transport(int dtype, const f2d_grid* g, const void* q, const void* Ux,
-                      const void* Uy, const void* lam, void* lUx, void* lUy,
-                      void* work, void* lq, void* stream);
-/* perpgrad^T (operators.py:144-149 reversed, contravariant factors) */
-int f2d_adj_perpgrad(int dtype, const f2d_grid* g, const void* lUx, const void* lUy,
-                     void* lpsi, void* stream);
-/* lq += c2v^T lb (operators.py:126-133 reversed) */
-int f2d_adj_c2v(int dtype, const f2d_grid* g, const void* lb, void* lq, void* stream);
+ * reversed) and perpgrad^T (operators.py:144-149 reversed, contravariant
+ * factors) for the stage adjoint lam = c0 lam0 + c1 lam1 + c2 lam2 (lam1,
+ * lam2 may be NULL), the RK combination of integrators.py:92-107 reversed:
+ * lq = (dT/dq)^T lam, lUx/lUy = (dT/dU)^T lam, lpsi = perpgrad^T (lUx, lUy).
+ * work: 12 padded arrays of scratch. */
+int f2d_adj_stage(int dtype, const f2d_grid* g, const void* q, const void* Ux,
+                  const void* Uy, const void* lam0, const void* lam1, const void* lam2,
+                  double c0, double c1, double c2, void* lUx, void* lUy, void* work,
+                  void* lq, void* lpsi, void* stream);
+/* out = base + lq + c2v^T lb (operators.py:126-133 reversed); out may be
+ * base or any other array (interior written). */
+int f2d_adj_combine(int dtype, const f2d_grid* g, const void* base, const void* lq,
+                    const void* lb, void* out, void* stream);
 
 /* ------------------------------------------------------------------------
  * Masked Poisson / Helmholtz on the vertex grid (elliptic.py:71-203).
