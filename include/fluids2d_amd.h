@@ -222,8 +222,8 @@ int f2d_adj_stage(int dtype, const f2d_grid* g, const void* q, const void* Ux,
                   const void* Uy, const void* lam0, const void* lam1, const void* lam2,
                   double c0, double c1, double c2, void* lUx, void* lUy, void* work,
                   void* lq, void* lpsi, void* stream);
-/* out = base + lq + c2v^T lb (operators.py:126-133 reversed); out may be
- * base or any other array (interior written). */
+/* out = base + lq + c2v^T lb, c2v of operators.py:126-133 reversed.  out
+ * may be base or any other array (interior written). */
 int f2d_adj_combine(int dtype, const f2d_grid* g, const void* base, const void* lq,
                     const void* lb, void* out, void* stream);
 
