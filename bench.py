@@ -168,6 +168,141 @@ def adjoint_cost(n=512, steps=200, checkpoint=20, seed=2024, reps=2):
     return min(tb) / min(tf), min(tf), min(tb)
 
 
+ENSEMBLE_SIZES = (128, 256, 512, 1024)
+
+
+def basin_mask(n, seed, ny=None):
+    """Closed basin: Gaussian-filtered white noise above a level, walls on
+    the border ring, largest wet component kept; then wet cells that touch
+    no wet vertex are dried (repeatedly), since the reference's
+    verticestocenters divides by zero there (operators.py:136-141)."""
+    from scipy import ndimage
+    ny = n if ny is None else ny
+    rng = np.random.default_rng(seed)
+    sm = ndimage.gaussian_filter(rng.standard_normal((ny, n)), sigma=max(n, ny) / 16.0)
+    wet = sm > -0.3 * sm.std()
+    wet[0, :] = wet[-1, :] = False
+    wet[:, 0] = wet[:, -1] = False
+    while True:
+        lab, nl = ndimage.label(wet)
+        if nl > 1:
+            sizes = ndimage.sum(wet, lab, range(1, nl + 1))
+            wet = lab == (1 + int(np.argmax(sizes)))
+        # vertex (j, i) is wet iff the 4 cells around it are
+        v = np.zeros((ny + 1, n + 1), dtype=bool)
+        v[1:-1, 1:-1] = wet[1:, 1:] & wet[:-1, 1:] & wet[1:, :-1] & wet[:-1, :-1]
+        touch = v[:-1, :-1] | v[1:, :-1] | v[:-1, 1:] | v[1:, 1:]
+        bad = wet & ~touch
+        if not bad.any():
+            break
+        wet &= ~bad
+    m = np.zeros((ny + 6, n + 6), dtype=np.int8)
+    m[3:-3, 3:-3] = wet
+    return m
+
+
+def ensemble_member(n, seed, precision="fp32", model="qgrsw", dtmax=0.05):
+    """One synthetic member: `model` on an n x n closed basin (Lx = Ly = 1,
+    f0 = 10, g = H = 1), h = H + seeded Gaussian bumps, balanced by the QG
+    projection (geos_adj.py:37-49).  dtmax keeps f0 dt = 0.5: the CFL step of
+    model.py:84-87 sees only the slow advective velocity."""
+    import fluids2d_amd as f2d
+    import fluids2d_amd.param as fp
+    from fluids2d_amd.rsw import qg_projection
+    fp.QUIET = True
+    p = f2d.Param()
+    p.model = model
+    p.nx = p.ny = n
+    p.precision = precision
+    p.dtmax = dtmax
+    p.tend = 1e9
+    p.maxite = 10 ** 9
+    m = f2d.Model(p)
+    m.mesh.msk[...] = basin_mask(n, seed)
+    m.mesh.finalize()
+    m.integrator.model.mesh = m.mesh
+    rng = np.random.default_rng(seed + 7)
+    x, y = m.mesh.xy("c")
+    bumps = np.zeros(m.mesh.shape)
+    for _ in range(4):
+        x0, y0 = rng.uniform(0.25, 0.75, 2)
+        r = rng.uniform(0.03, 0.08)
+        bumps += rng.choice([-1, 1]) * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * r * r))
+    if model in ("qgrsw", "rsw"):
+        m.state.h[...] = (p.H + 0.05 * bumps) * m.mesh.msk * m.mesh.area
+        qg_projection(m)
+    else:  # eulerpsi / qg: vorticity or PV bumps (vortex.py:19-26 convention)
+        field = "pv" if model == "qg" else "omega"
+        getattr(m.state, field)[...] = bumps * m.mesh.msk * m.mesh.area
+        m.integrator.diag(m.state)
+    return m
+
+
+def ensemble_sizes(count, seed):
+    rng = np.random.default_rng(seed)
+    return [int(v) for v in rng.choice(ENSEMBLE_SIZES, size=count)]
+
+
+def ensemble_bench(count=64, steps=200, seed=99, precision="fp32", tune=True, log=None):
+    """The batched-ensemble workload: `count` qgrsw members with sizes drawn
+    by seed from ENSEMBLE_SIZES (squared), closed smoothed-noise basins,
+    FP32 + refinement, `steps` steps replayed from one captured graph.
+    Multigrid schedules per size class and the stream count are picked by
+    the autotuner first."""
+    import torch
+    from fluids2d_amd import autotune
+    from fluids2d_amd.ensemble import Ensemble
+    sizes = ensemble_sizes(count, seed)
+    t0 = time.perf_counter()
+    tuned = {}
+    members = []
+    for k, n in enumerate(sizes):
+        m = ensemble_member(n, seed * 1000 + k, precision)
+        if tune:
+            if n not in tuned:
+                tuned[n] = autotune.tune_poisson(m.mesh, maindiag=m.mesh.qg_helmholtz.maindiag)
+                if log:
+                    log(f"ensemble: tuned {n}^2 multigrid schedule {tuned[n]}")
+            m.param.poisson_opts = tuned[n]
+            m.mesh.finalize()
+            m.integrator.model.mesh = m.mesh
+        m.integrator.begin(m.state)
+        members.append(m)
+        if log and (k + 1) % 8 == 0:
+            log(f"ensemble: {k + 1}/{count} members set up "
+                f"({time.perf_counter() - t0:.1f} s)")
+    setup = time.perf_counter() - t0
+    ens = Ensemble(members, nstreams=8)
+    ens.steps(2)
+    if log:
+        log("ensemble: 2 eager steps done")
+    streams = ens.tune_streams() if tune else {}
+    if log:
+        log(f"ensemble: stream counts timed {streams}")
+    ens.steps(ens.G)  # capture at the chosen stream count
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    dts = ens.steps(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    cells = sum(n * n for n in sizes)
+    st = [m.mesh.qg_helmholtz.read_stats() for m in members]
+    # wet cells (dry ones may hold the (1/0)*0 of verticestocenters)
+    finite = all(bool(torch.isfinite(m.integrator.dstate.h[m.mesh.d_msk == 1]).all())
+                 for m in members)
+    return {"workload": f"{count} qgrsw members, sizes from {ENSEMBLE_SIZES} squared "
+                        f"(seed {seed}), closed smoothed-noise basins, {precision} with "
+                        f"FP64 refinement, {steps} steps from one captured graph",
+            "cells": cells, "members": count,
+            "cell_steps_per_s": cells * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+            "setup_s": setup, "streams": ens.nstreams, "stream_ms_per_step": streams,
+            "tuned": {str(k): v for k, v in tuned.items()},
+            "failed_solves": int(sum(x["failed"] for x in st)), "finite": finite,
+            "dt_range": [float(dts.min()), float(dts.max())]}
+
+
 def traffic_of(kernel, nx, prec):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary (profiles/traffic_<workload>.txt, JSON: FETCH_SIZE +
@@ -223,6 +358,9 @@ def main():
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint cost measurement")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="MEMBERS",
+                    help="also time the batched qgrsw ensemble with this many members")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB three times "
                          "(a kernel of known traffic, to scale rocprofv3 counters)")
@@ -319,6 +457,9 @@ def main():
                                       "checkpoints every 20, gradient of a final misfit "
                                       "w.r.t. the initial vorticity",
                           "forward_s": tfw, "forward_plus_adjoint_s": tfb}
+    if a.ensemble > 0:
+        out["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps,
+                                         log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out))
 
 

@@ -21,6 +21,7 @@ EXPORTS = [
     "f2d_compflux", "f2d_vortexforce", "f2d_innerproduct",
     "f2d_transport_stage", "f2d_rk_addto", "f2d_c2v", "f2d_perpgrad",
     "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar", "f2d_copy_f64",
+    "f2d_reserve_stream_scratch",
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
     "f2d_poisson_solve_c2v", "f2d_poisson_stats", "f2d_poisson_reset_stats",
     "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
@@ -38,7 +39,10 @@ class Grid(ctypes.Structure):
     _fields_ = [("ny", ctypes.c_int64), ("nx", ctypes.c_int64),
                 ("halowidth", ctypes.c_int32), ("xperiodic", ctypes.c_int32),
                 ("yperiodic", ctypes.c_int32), ("dx", ctypes.c_double),
-                ("dy", ctypes.c_double)]
+                ("dy", ctypes.c_double), ("tile", ctypes.c_int32)]
+
+
+TILES = {0: (16, 64), 1: (8, 64), 2: (16, 32), 3: (32, 32), 4: (8, 128)}
 
 
 class PoissonOpts(ctypes.Structure):
@@ -93,6 +97,7 @@ def lib():
         "f2d_cfl_dt": (i32, [i32, vp, vp, i64, d, d, vp, vp]),
         "f2d_set_scalar": (i32, [vp, d, vp]),
         "f2d_copy_f64": (i32, [vp, vp, i32, vp]),
+        "f2d_reserve_stream_scratch": (i32, [vp]),
         "f2d_poisson_create": (i32, [vp, gp, d, ctypes.POINTER(PoissonOpts),
                                      ctypes.POINTER(vp)]),
         "f2d_poisson_destroy": (i32, [vp]),
@@ -167,4 +172,5 @@ def make_grid(param):
     g.yperiodic = int(bool(getattr(param, "yperiodic", False)))
     g.dx = param.Lx / param.nx
     g.dy = param.Ly / param.ny
+    g.tile = int(getattr(param, "tile", 0) or 0)
     return g

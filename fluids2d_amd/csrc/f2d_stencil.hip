@@ -7,6 +7,9 @@
 //   c2v / perpgrad      operators.py:126-133, 144-149
 //   halo fill           meshes.py:135-143 (+ y-periodic extension)
 //   CFL time step       model.py:71-87
+#include <map>
+#include <mutex>
+
 #include "f2d_common.hpp"
 
 namespace f2d {
@@ -316,9 +319,8 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
   stage_update<T>(a, k, ds, a.q[k]);
 }
 
-template <class T, int M>
-void launch_transport(const TransportArgs<T>& a0, hipStream_t s) {
-  constexpr int TY = 16, TX = 64;
+template <class T, int M, int TY, int TX>
+void launch_transport_tile(const TransportArgs<T>& a0, hipStream_t s) {
   TransportArgs<T> a = a0;
   a.tiles_x = (a.nx + TX - 1) / TX;
   int tiles_y = (a.ny + TY - 1) / TY;
@@ -326,6 +328,19 @@ void launch_transport(const TransportArgs<T>& a0, hipStream_t s) {
   a.nhalo = 6 * (int64_t)(a.nx + 6) + 6 * (int64_t)a.ny;
   int nb = a.ntiles + (int)((a.nhalo + 255) / 256);
   k_transport<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
+}
+
+// tile shape by code (f2d_grid.tile): the arithmetic per cell is the same
+// for every shape, only the work split and the halo re-reads change
+template <class T, int M>
+void launch_transport(const TransportArgs<T>& a, int tile, hipStream_t s) {
+  switch (tile) {
+    case 1: launch_transport_tile<T, M, 8, 64>(a, s); break;
+    case 2: launch_transport_tile<T, M, 16, 32>(a, s); break;
+    case 3: launch_transport_tile<T, M, 32, 32>(a, s); break;
+    case 4: launch_transport_tile<T, M, 8, 128>(a, s); break;
+    default: launch_transport_tile<T, M, 16, 64>(a, s); break;
+  }
 }
 
 template <class T>
@@ -533,30 +548,39 @@ __global__ void k_copy_f64(const double* __restrict__ src, double* __restrict__ 
   if (i < n) dst[i] = src[i];
 }
 
-// Per-device scratch of the grid reductions of this file: partial maxima
-// written by one kernel and read by the one-block finaliser that follows it
-// on the same stream (calls on different streams of one device must not
-// overlap).
+// Scratch of the grid reductions of this file: partial maxima written by
+// one kernel and read by the one-block finaliser that follows it on the same
+// stream.  One buffer per device by default (calls on different streams of
+// one device must then not overlap); a stream registered with
+// f2d_reserve_stream_scratch gets its own, so that work on concurrent
+// streams (graph branches) does not share it.
 struct ReduceScratch {
   double* partials = nullptr;
   int cap = 0;
 };
 static ReduceScratch g_red[64];
+static std::map<std::pair<int, void*>, ReduceScratch> g_red_stream;
+static std::mutex g_red_mu;
 
-static double* red_scratch(int nblocks) {
+static void red_grow(ReduceScratch& r, int nblocks) {
+  if (r.cap >= nblocks) return;
+  if (r.partials) {
+    F2D_HIP(hipDeviceSynchronize());
+    F2D_HIP(hipFree(r.partials));
+  }
+  int cap = nblocks < 8192 ? 8192 : nblocks;
+  F2D_HIP(hipMalloc((void**)&r.partials, sizeof(double) * 2 * cap));
+  r.cap = cap;
+}
+
+static double* red_scratch(int nblocks, void* stream) {
   int dev = 0;
   F2D_HIP(hipGetDevice(&dev));
   F2D_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
-  ReduceScratch& r = g_red[dev];
-  if (r.cap < nblocks) {
-    if (r.partials) {
-      F2D_HIP(hipDeviceSynchronize());
-      F2D_HIP(hipFree(r.partials));
-    }
-    int cap = nblocks < 8192 ? 8192 : nblocks;
-    F2D_HIP(hipMalloc((void**)&r.partials, sizeof(double) * 2 * cap));
-    r.cap = cap;
-  }
+  std::lock_guard<std::mutex> lock(g_red_mu);
+  auto it = g_red_stream.find({dev, stream});
+  ReduceScratch& r = it != g_red_stream.end() ? it->second : g_red[dev];
+  red_grow(r, nblocks);
   return r.partials;
 }
 
@@ -672,10 +696,10 @@ int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
       a.xper = g->xperiodic;
       a.yper = g->yperiodic;
       switch (method) {
-        case F2D_WENO: launch_transport<T, F2D_WENO>(a, S(stream)); break;
-        case F2D_UPWIND: launch_transport<T, F2D_UPWIND>(a, S(stream)); break;
-        case F2D_CENTERED: launch_transport<T, F2D_CENTERED>(a, S(stream)); break;
-        default: launch_transport<T, F2D_CWENO>(a, S(stream)); break;
+        case F2D_WENO: launch_transport<T, F2D_WENO>(a, g->tile, S(stream)); break;
+        case F2D_UPWIND: launch_transport<T, F2D_UPWIND>(a, g->tile, S(stream)); break;
+        case F2D_CENTERED: launch_transport<T, F2D_CENTERED>(a, g->tile, S(stream)); break;
+        default: launch_transport<T, F2D_CWENO>(a, g->tile, S(stream)); break;
       }
     });
   });
@@ -712,7 +736,7 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
     check_grid(g);
     int64_t n = (g->ny + 6) * (g->nx + 6);
     int nb = grid1d(n, 256 * PG_PTS);
-    double* part = (maxabs || dt) ? red_scratch(nb) : nullptr;
+    double* part = (maxabs || dt) ? red_scratch(nb, stream) : nullptr;
     F2D_DTYPE_DISPATCH(dtype, {
       PerpArgs<T> a{};
       a.psi = (const T*)psi;
@@ -766,7 +790,7 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
     F2D_REQUIRE(dt, "dt is NULL");
     int nb = grid1d(n, 256);
     if (nb > 1024) nb = 1024;
-    double* part = red_scratch(nb);
+    double* part = red_scratch(nb, stream);
     F2D_DTYPE_DISPATCH(dtype, k_absmax<T><<<nb, 256, 0, S(stream)>>>(
                                   (const T*)Ux, (const T*)Uy, n, part));
     k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, nullptr, dt, cfl, dtmax);
@@ -775,6 +799,16 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
 
 int f2d_set_scalar(double* dst, double value, void* stream) {
   return guarded([&] { k_set_scalar<<<1, 1, 0, S(stream)>>>(dst, value); });
+}
+
+int f2d_reserve_stream_scratch(void* stream) {
+  return guarded([&] {
+    int dev = 0;
+    F2D_HIP(hipGetDevice(&dev));
+    F2D_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_red_mu);
+    red_grow(g_red_stream[{dev, stream}], 8192);
+  });
 }
 
 int f2d_copy_f64(double* dst, const double* src, int n, void* stream) {

@@ -23,8 +23,14 @@ class PoissonNotConverged(RuntimeError):
     pass
 
 
+TUNABLE = ("nu", "nu_coarse", "nu_tail", "tail_points", "tail_threads", "refine_inner",
+           "coarse_sweeps")
+
+
 class Poisson2D:
-    def __init__(self, mesh, location, maindiag=0):
+    def __init__(self, mesh, location, maindiag=0, tuning=None):
+        """tuning: solver schedule fields of f2d_poisson_opts (TUNABLE), as
+        picked by fluids2d_amd.autotune; default ``param.poisson_opts``."""
         assert location in ["c", "v"]
         if location == "c":
             raise NotImplementedError("the Neumann (cell) operator is not on the GPU path")
@@ -42,6 +48,12 @@ class Poisson2D:
         # direct FFT solve in FP32: refine from the previous solution held in
         # x (the state's streamfunction) instead of a cold FP32 solve
         opts.fft_warm = int(bool(getattr(p, "poisson_warm", True)))
+        tuning = getattr(p, "poisson_opts", None) if tuning is None else tuning
+        self.tuning = dict(tuning or {})
+        for k, v in self.tuning.items():
+            if k not in TUNABLE:
+                raise ValueError(f"poisson_opts: {k} is not a tunable solver option")
+            setattr(opts, k, int(v))
         rt = getattr(p, "poisson_rtol", None)
         mi = getattr(p, "poisson_maxit", None)
         self.rtol = DEFAULT_RTOL[self.precision] if rt is None else float(rt)

@@ -18,6 +18,12 @@ Fields added for the MI355X path (declared in the constructor, so that
 * ``poisson_warm``  FP32 direct (FFT) solves: the FP64-residual refinement
                     starts from the solution held in x (the previous stage's
                     streamfunction) instead of a cold FP32 solve
+* ``poisson_opts``  dict of multigrid schedule options (sweeps per level,
+                    tail size and workgroup, inner FP32 iterations), as
+                    picked by ``fluids2d_amd.autotune``
+* ``tile``          LDS tile code of the fused transport stage (0 16x64,
+                    1 8x64, 2 16x32, 3 32x32, 4 8x128), as picked by
+                    ``fluids2d_amd.autotune``; results do not depend on it
 * ``hipgraph``      replay runs of device steps from captured HIP graphs
 * ``device``        torch device index of the GPU
 """
@@ -93,6 +99,8 @@ class Param:
         self.poisson_nu = 2
         self.poisson_solver = "auto"
         self.poisson_warm = True
+        self.poisson_opts = None
+        self.tile = 0
         self.hipgraph = True
         self.device = 0
 

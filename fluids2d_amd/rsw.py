@@ -140,6 +140,38 @@ class QGRSW(_SW):
         op.fill(self.mesh, s.omega)
 
 
+def qg_projection(model):
+    """Geostrophic balance of a model's initial state on the device, as
+    geos_adj.py:43-44 sets it up: qg_projection (operators.py:176-183) with
+    anomaly=False -- pv = vorticity(u) + c2v((h - h0) (-f0/H)) with
+    h0 = H area - hb (add_stretching, operators.py:194-201), psi = Helmholtz
+    solve of pv, h = v2c(psi f0 area/g) + h0, times msk
+    (thickness_from_psi, operators.py:204-211), u = perpgrad(psi)
+    (covariant) -- then the model's diag (geos_adj.py:55).  Works on the
+    host state: it is copied in and back."""
+    m, p = model.mesh, model.param
+    if p.model not in ("qgrsw", "rsw"):
+        raise NotImplementedError("qg_projection: rsw / qgrsw states (u, h)")
+    integ = model.integrator
+    integ.begin(model.state)
+    s = integ.dstate
+    dt, g = _lib.dtype_code(s.h.dtype), m.grid_ref()
+    _ops("f2d_vorticity", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y), _lib.ptr(m.d_slipcoef),
+         _lib.ptr(s.pv))
+    h0 = p.H * m.area - m.hb
+    dh = s.h - h0
+    _ops("f2d_c2v_scaled", dt, g, _lib.ptr(dh), float(-p.f0 / p.H), _lib.ptr(s.pv),
+         _lib.ptr(m.d_mskv), 1)
+    m.qg_helmholtz.solve(s.pv, s.psi)
+    _ops("f2d_v2c_scaled", dt, g, _lib.ptr(s.psi), float(p.f0 * m.area / p.g),
+         _lib.ptr(m.d_mskv), _lib.ptr(m.d_msk), _lib.ptr(s.h))
+    s.h.add_(h0)
+    s.h.mul_(m.d_msk)
+    op.perpgrad(m, s.psi, s.u, contravariant=False)
+    integ.end(model.state)
+    integ.diag(model.state)
+
+
 MODELS = {"rsw": RSW, "qgrsw": QGRSW}
 
 

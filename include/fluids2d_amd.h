@@ -63,6 +63,10 @@ typedef struct f2d_grid {
   int32_t halowidth;/* must be 3 (param.py:23)                          */
   int32_t xperiodic, yperiodic;
   double dx, dy;    /* Lx/nx, Ly/ny (meshes.py:24-25)                    */
+  int32_t tile;     /* LDS tile of the fused transport stage (no
+                       counterpart in the reference): 0 16x64 (default),
+                       1 8x64, 2 16x32, 3 32x32, 4 8x128 rows x columns;
+                       results do not depend on it                      */
 } f2d_grid;
 
 /* ------------------------------------------------------------------------
@@ -155,6 +159,13 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
 /* dt = value (device scalar): the fixed-dt and rsw branches of Model.set_dt,
  * model.py:71-82 */
 int f2d_set_scalar(double* dst, double value, void* stream);
+/* Give `stream` (on the current device) its own scratch for the grid
+ * reductions of f2d_cfl_dt and f2d_perpgrad_ex, allocated here.  It has
+ * no counterpart in the reference: it is for callers that run models on
+ * concurrent streams or graph branches.  Without it all streams of a
+ * device share one buffer, and reductions on different streams must not
+ * overlap. */
+int f2d_reserve_stream_scratch(void* stream);
 /* dst[0:n] = src[0:n] on the device, as a kernel (a graph-capturable copy
  * of device scalars such as the step's dt into a log): the device form of
  * the per-step dt the host reads in Model.set_dt, model.py:71-87 */
