@@ -358,8 +358,9 @@ def main():
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint cost measurement")
-    ap.add_argument("--ensemble", type=int, default=0, metavar="MEMBERS",
-                    help="also time the batched qgrsw ensemble with this many members")
+    ap.add_argument("--ensemble", type=int, default=64, metavar="MEMBERS",
+                    help="also time the batched qgrsw ensemble with this many members "
+                         "(0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB three times "

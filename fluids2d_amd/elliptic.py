@@ -16,7 +16,7 @@ import torch
 from . import _lib
 
 DEFAULT_RTOL = {"fp64": 1e-13, "fp32": 1e-9}
-DEFAULT_MAXIT = {"fp64": 100, "fp32": 4}
+DEFAULT_MAXIT = {"fp64": 100, "fp32": 8}
 
 
 class PoissonNotConverged(RuntimeError):

@@ -10,7 +10,7 @@ Fields added for the MI355X path (declared in the constructor, so that
 * ``poisson_rtol``  relative residual of the Poisson solves (None: 1e-13 in
                     FP64, 1e-9 for the refined FP32 mode)
 * ``poisson_maxit`` iteration cap of the Poisson solves (None: 100 in FP64,
-                    4 refinements in FP32)
+                    8 refinements in FP32)
 * ``poisson_nu``    red-black Gauss-Seidel sweeps before/after each level
 * ``poisson_solver`` "auto" (direct FFT solve for the fully wet doubly
                     periodic operator with power-of-two sizes, multigrid CG
