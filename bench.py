@@ -126,7 +126,27 @@ def accuracy(nx, seed, nsteps):
         m.integrator.device_steps(nsteps - 2)
         m.integrator.end(m.state)
     a, b = m32.state.omega[3:-3, 3:-3], m64.state.omega[3:-3, 3:-3]
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b)), dt
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b)), dt, diag_errors(m32, m64)
+
+
+def diag_errors(m32, m64):
+    """FP64 bulk diagnostics (diagnostics.py:45-66) of the FP32 run against
+    the FP64 run's: relative error of ke, enstrophy and angular momentum;
+    the domain-mean vorticity (zero up to rounding on this periodic field)
+    by its difference scaled by max|omega|/area."""
+    import torch
+    from fluids2d_amd.diagnostics import Bulk
+    out = []
+    for m in (m32, m64):
+        bk = Bulk(m, ncfile=None, ndiags=1)
+        r = torch.zeros(4, dtype=torch.float64, device=m.mesh.device)
+        bk.measure(r)
+        out.append(r.cpu().numpy())
+    a, b = out
+    w = float(np.abs(m64.state.omega).max()) / m64.mesh.area
+    return {"ke": float(abs(a[0] - b[0]) / abs(b[0])), "ens": float(abs(a[1] - b[1]) / abs(b[1])),
+            "angular": float(abs(a[3] - b[3]) / max(abs(b[3]), 1e-300)),
+            "vort_over_max": float(abs(a[2] - b[2]) / w)}
 
 
 def adjoint_cost(n=512, steps=200, checkpoint=20, seed=2024, reps=2):
@@ -398,9 +418,9 @@ def main():
         del m
         torch.cuda.empty_cache()
 
-    acc_err = None
+    acc_err = acc_diag = None
     if a.only == "both" and a.accuracy_steps > 0:
-        acc_err, acc_dt = accuracy(nx, a.seed, a.accuracy_steps)
+        acc_err, acc_dt, acc_diag = accuracy(nx, a.seed, a.accuracy_steps)
 
     head = res.get("fp32") or res.get("fp64")
     head_prec = "fp32" if "fp32" in res else "fp64"
@@ -448,6 +468,7 @@ def main():
         out["fp32_refine_over_fp64"] = res["fp32"]["csps"] / res["fp64"]["csps"]
     if acc_err is not None:
         out["fp32_vs_fp64_rel_l2_omega"] = acc_err
+        out["fp32_vs_fp64_diagnostics"] = acc_diag
         out["accuracy_steps"] = a.accuracy_steps
     out["poisson"] = {k: v["stats"] for k, v in res.items()}
     out["adjoint_cost_ratio"] = None

@@ -27,7 +27,7 @@ EXPORTS = [
     "f2d_poisson_bench_kernel", "f2d_perpgrad_ex", "f2d_poisson_read_stats",
     "f2d_poisson_method", "f2d_coriolis", "f2d_addgrad", "f2d_sharp", "f2d_vorticity",
     "f2d_ke_finish", "f2d_pressure", "f2d_div", "f2d_c2v_scaled", "f2d_v2c_scaled",
-    "f2d_adj_stage", "f2d_adj_combine",
+    "f2d_adj_stage", "f2d_adj_combine", "f2d_bulk_diagnostics",
 ]
 
 
@@ -114,6 +114,7 @@ def lib():
         "f2d_adj_stage": (i32, [i32, gp, vp, vp, vp, vp, vp, vp, d, d, d, vp, vp, vp, vp,
                                 vp, vp]),
         "f2d_adj_combine": (i32, [i32, gp, vp, vp, vp, vp, vp]),
+        "f2d_bulk_diagnostics": (i32, [i32, gp, vp, vp, vp, vp, vp, vp, i32, vp, vp]),
         "f2d_poisson_solve": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
                                     ctypes.POINTER(d), vp]),
         "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),

@@ -150,6 +150,88 @@ __global__ void k_v2c_scaled(const T* __restrict__ vh, double scale,
   h[k] = val * T(msk[k]);
 }
 
+
+// Bulk diagnostics (diagnostics.py:45-66): FP64 sums over every point of
+// the padded arrays, as numpy's domsum does.  Per block the partial sums of
+// ke, omega^2, omega, Uy xv, Ux yu and msk (6 doubles), then one block
+// reduces them in a fixed order (deterministic) and forms
+//   ke = S_ke / S_m, ens = 0.5 (S_w2 / S_m) / area^2, vort = (S_w / S_m) / area,
+//   angular = (S_uyx / S_m) dx - (S_uxy / S_m) dy,
+// with xv = (i - 3 + 1/2) dx, yu = (j - 3 + 1/2) dy (mesh.xy, meshes.py:57-68).
+// ke NULL: the cell-centred 1/2 u.U of the contravariant U (covariant
+// u = U dx^2 in x, U dy^2 in y), a quarter of the four face products.
+struct BulkArgs {
+  int NY, NX;
+  double dx, dy;
+};
+
+__device__ __forceinline__ double bsum_block(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bulk_part(BulkArgs a, const T* __restrict__ ke,
+                                                   const T* __restrict__ w,
+                                                   const T* __restrict__ Ux,
+                                                   const T* __restrict__ Uy,
+                                                   const int8_t* __restrict__ msk,
+                                                   double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int64_t n = (int64_t)a.NY * a.NX;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(k / a.NX), i = (int)(k - (int64_t)j * a.NX);
+    const double ux = (double)Ux[k], uy = (double)Uy[k], wv = (double)w[k];
+    double kv;
+    if (ke) {
+      kv = (double)ke[k];
+    } else {
+      const double uxe = i + 1 < a.NX ? (double)Ux[k + 1] : 0.0;
+      const double uyn = j + 1 < a.NY ? (double)Uy[k + a.NX] : 0.0;
+      kv = 0.25 * (a.dx * a.dx * (ux * ux + uxe * uxe) + a.dy * a.dy * (uy * uy + uyn * uyn)) *
+           (double)msk[k];
+    }
+    s[0] += kv;
+    s[1] += wv * wv;
+    s[2] += wv;
+    s[3] += uy * ((i - 3 + 0.5) * a.dx);
+    s[4] += ux * ((j - 3 + 0.5) * a.dy);
+    s[5] += (double)msk[k];
+  }
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const double v = bsum_block(s[q], sh);
+    if (threadIdx.x == 0) part[(int64_t)q * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bulk_fin(BulkArgs a, const double* __restrict__ part,
+                                                  int nb, double area, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double S[6];
+  for (int q = 0; q < 6; q++) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[(int64_t)q * nb + b];
+    S[q] = bsum_block(v, sh);
+  }
+  if (threadIdx.x == 0) {
+    const double m = S[5];
+    out[0] = S[0] / m;
+    out[1] = 0.5 * (S[1] / m) / (area * area);
+    out[2] = (S[2] / m) / area;
+    out[3] = (S[3] / m) * a.dx - (S[4] / m) * a.dy;
+  }
+}
+
 }  // namespace f2d
 
 using namespace f2d;
@@ -267,6 +349,24 @@ int f2d_v2c_scaled(int dtype, const f2d_grid* g, const void* vh, double scale,
     F2D_OPS_DISPATCH(dtype, k_v2c_scaled<T><<<grid1d((int64_t)NY * NX, 256), 256, 0,
                                               S(stream)>>>(
                                 (const T*)vh, scale, mskv, msk, (T*)h, NY, NX));
+  });
+}
+
+int f2d_bulk_diagnostics(int dtype, const f2d_grid* g, const void* ke, const void* omega,
+                         const void* Ux, const void* Uy, const int8_t* msk, double* partials,
+                         int nparts, double* out, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(g && omega && Ux && Uy && msk && partials && out, "NULL argument");
+    F2D_REQUIRE(nparts >= 1, "nparts must be >= 1");
+    BulkArgs a{(int)g->ny + 6, (int)g->nx + 6, g->dx, g->dy};
+    const int64_t n = (int64_t)a.NY * a.NX;
+    int nb = grid1d(n, 256);
+    if (nb > nparts) nb = nparts;
+    hipStream_t s = S(stream);
+    F2D_OPS_DISPATCH(dtype, k_bulk_part<T><<<nb, 256, 0, s>>>(a, (const T*)ke, (const T*)omega,
+                                                               (const T*)Ux, (const T*)Uy, msk,
+                                                               partials));
+    k_bulk_fin<<<1, 256, 0, s>>>(a, partials, nb, g->dx * g->dy, out);
   });
 }
 

@@ -61,12 +61,18 @@ class Model:
             signal.signal(signal.SIGINT, old)
 
     def _host_hooks(self):
-        """Points of the loop where host code reads (or edits) the state."""
-        need = bool(self.diags) or self.time.save_to_file
-        if not need:
+        """Points of the loop where host code reads (or edits) the state.
+        Device diagnostics (``diag.device``) read the device state: they
+        need no host copy."""
+        host = [d for d in self.diags if not getattr(d, "device", False)]
+        for d in self.diags:
+            if getattr(d, "device", False):
+                d()
+        if not (host or self.time.save_to_file):
             return
         self.integrator.end(self.state)
-        self.compute_diags()
+        for d in host:
+            d()
         self.save_to_file()
 
     def finalize(self):

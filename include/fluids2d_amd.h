@@ -212,6 +212,15 @@ int f2d_c2v_scaled(int dtype, const f2d_grid* g, const void* c, double scale,
 int f2d_v2c_scaled(int dtype, const f2d_grid* g, const void* vh, double scale,
                    const int8_t* mskv, const int8_t* msk, void* h, void* stream);
 
+/* Bulk diagnostics, diagnostics.py:45-66 (Bulk.__call__): out[0..3] =
+ * ke, ens, vort, angular from FP64 sums over every point of the padded
+ * arrays (numpy's domsum), on the device; ke NULL (eulerpsi, whose state
+ * has no ke) takes the cell-centred 1/2 u.U of the contravariant U.
+ * partials: scratch of at least 6 * nparts doubles (nparts blocks at most). */
+int f2d_bulk_diagnostics(int dtype, const f2d_grid* g, const void* ke, const void* omega,
+                         const void* Ux, const void* Uy, const int8_t* msk, double* partials,
+                         int nparts, double* out, void* stream);
+
 /* ------------------------------------------------------------------------
  * Adjoint of the eulerpsi RK stage on a fully wet, doubly periodic grid (no
  * counterpart in the reference, which has no derivatives): the stage
