@@ -323,6 +323,11 @@ def ensemble_bench(count=64, steps=200, seed=99, precision="fp32", tune=True, lo
             "dt_range": [float(dts.min()), float(dts.max())]}
 
 
+def _lib_tiles():
+    from fluids2d_amd import _lib
+    return _lib.TILES
+
+
 def traffic_of(kernel, nx, prec):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary (profiles/traffic_<workload>.txt, JSON: FETCH_SIZE +
@@ -402,11 +407,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    from fluids2d_amd import autotune
     nx = a.nx
     res = {}
+    tiles = {}
     for prec in (["fp64", "fp32"] if a.only == "both" else [a.only]):
         m = build(nx, prec, a.seed)
         calibrate(m, 3)
+        # the transport tile by measurement (results do not depend on it)
+        tiles[prec] = autotune.tune_tile(m, cache=False)
         msps, wall, st, dts = timed(m, a.steps, a.warmup)
         if st["failed"] > 0:
             raise SystemExit(f"{prec}: {st['failed']} Poisson solves did not converge "
@@ -471,6 +480,7 @@ def main():
         out["fp32_vs_fp64_diagnostics"] = acc_diag
         out["accuracy_steps"] = a.accuracy_steps
     out["poisson"] = {k: v["stats"] for k, v in res.items()}
+    out["transport_tile"] = {k: "x".join(map(str, _lib_tiles()[v])) for k, v in tiles.items()}
     out["adjoint_cost_ratio"] = None
     if not a.no_adjoint:
         ratio, tfw, tfb = adjoint_cost()

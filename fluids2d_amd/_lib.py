@@ -19,7 +19,7 @@ EXPORTS = [
     "f2d_abi_version", "f2d_last_error", "f2d_device_synchronize",
     "f2d_set_masks", "f2d_set_order", "f2d_slipcoef", "f2d_gindex",
     "f2d_compflux", "f2d_vortexforce", "f2d_innerproduct",
-    "f2d_transport_stage", "f2d_rk_addto", "f2d_c2v", "f2d_perpgrad",
+    "f2d_transport_stage", "f2d_transport_stage_psi", "f2d_rk_addto", "f2d_c2v", "f2d_perpgrad",
     "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar", "f2d_copy_f64",
     "f2d_reserve_stream_scratch",
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
@@ -88,6 +88,8 @@ def lib():
         "f2d_innerproduct": (i32, [i32, vp, vp, vp, vp, i64, i64, i32, vp]),
         "f2d_transport_stage": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
                                       vp, vp, vp, i32, i32, vp, vp]),
+        "f2d_transport_stage_psi": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, d, d, i32, vp,
+                                          vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "f2d_rk_addto": (i32, [i32, vp, vp, vp, vp, i64, i32, i32, vp, vp]),
         "f2d_c2v": (i32, [i32, gp, vp, vp, vp, i32, vp]),
         "f2d_perpgrad": (i32, [i32, gp, vp, vp, vp, vp, vp, i32, vp, vp]),
@@ -154,6 +156,40 @@ def stream_ptr(stream=None):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+class capture:
+    """``with capture(graph):`` -- torch.cuda.graph with Python's cyclic
+    garbage collector held off: a collection during the capture could run a
+    finaliser that frees device memory (a solver handle), which a capture in
+    global mode does not allow."""
+
+    def __init__(self, graph, **kw):
+        import torch
+        self._ctx = torch.cuda.graph(graph, **kw)
+
+    def __enter__(self):
+        import gc
+        self._gc = gc.isenabled()
+        gc.disable()
+        return self._ctx.__enter__()
+
+    def __exit__(self, *exc):
+        import gc
+        try:
+            return self._ctx.__exit__(*exc)
+        finally:
+            if self._gc:
+                gc.enable()
+
+
+def capturing():
+    """True while the current stream is being captured into a graph."""
+    import torch
+    try:
+        return torch.cuda.is_current_stream_capturing()
+    except Exception:
+        return False
 
 
 def dtype_code(dt):

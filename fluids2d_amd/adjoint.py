@@ -167,7 +167,7 @@ class EulerAdjoint:
         if g is None:
             torch.cuda.synchronize()
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
+            with _lib.capture(g):
                 self._segment_ops(lane, seg, L)
             lane.graphs[key] = g
         g.replay()
@@ -243,7 +243,7 @@ class EulerAdjoint:
         if g is None:
             torch.cuda.synchronize()
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
+            with _lib.capture(g):
                 self._reverse_ops(seg, L)
             self.rgraphs[key] = g
         g.replay()

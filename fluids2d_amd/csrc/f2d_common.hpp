@@ -140,6 +140,71 @@ __device__ __forceinline__ T weno5z(T qmm, T qm, T q0, T qp, T qpp) {
   return ((w1 * qi1 + w2 * qi2) + w3 * qi3) / ((w1 + w2) + w3);
 }
 
+// FP32 forms of the reconstructions above.  The FP32 path has no bit-exact
+// target (its bar is the FP64 result within 1e-5 after 1000 steps), so
+// these use fused multiply-adds and the hardware reciprocal (v_rcp_f32,
+// 1 ulp) instead of IEEE divisions: about half the VALU instructions of a
+// face.  The divisors are normal numbers: beta + eps >= eps and the weight
+// sum >= 1.  The FP64 templates above are untouched.
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <>
+__device__ __forceinline__ float weno5z<float>(float qmm, float qm, float q0, float qp,
+                                               float qpp) {
+  const float eps = 1e-16f;
+  const float qi1 = fmaf(11.f / 6.f, q0, fmaf(1.f / 3.f, qmm, (-7.f / 6.f) * qm));
+  const float qi2 = fmaf(1.f / 3.f, qp, fmaf(-1.f / 6.f, qm, (5.f / 6.f) * q0));
+  const float qi3 = fmaf(-1.f / 6.f, qpp, fmaf(1.f / 3.f, q0, (5.f / 6.f) * qp));
+  const float k1 = 13.f / 12.f, k2 = 0.25f;
+  const float a1 = fmaf(-2.f, qm, qmm) + q0, b1 = fmaf(3.f, q0, fmaf(-4.f, qm, qmm));
+  const float a2 = fmaf(-2.f, q0, qm) + qp, b2 = qm - qp;
+  const float a3 = fmaf(-2.f, qp, q0) + qpp, b3 = fmaf(3.f, q0, fmaf(-4.f, qp, qpp));
+  const float beta1 = fmaf(k1, a1 * a1, k2 * (b1 * b1));
+  const float beta2 = fmaf(k1, a2 * a2, k2 * (b2 * b2));
+  const float beta3 = fmaf(k1, a3 * a3, k2 * (b3 * b3));
+  const float tau5 = fabsf(beta1 - beta3);
+  const float w1 = fmaf(0.1f * tau5, rcp32(beta1 + eps), 0.1f);
+  const float w2 = fmaf(0.6f * tau5, rcp32(beta2 + eps), 0.6f);
+  const float w3 = fmaf(0.3f * tau5, rcp32(beta3 + eps), 0.3f);
+  const float num = fmaf(w3, qi3, fmaf(w2, qi2, w1 * qi1));
+  return num * rcp32((w1 + w2) + w3);
+}
+
+template <>
+__device__ __forceinline__ float weno3z<float>(float qm, float q0, float qp) {
+  const float eps = 1e-14f;
+  const float qi1 = fmaf(-0.5f, qm, 1.5f * q0);
+  const float qi2 = 0.5f * (q0 + qp);
+  const float d1 = q0 - qm, d2 = qp - q0;
+  const float beta1 = d1 * d1, beta2 = d2 * d2;
+  const float tau = fabsf(beta2 - beta1);
+  const float w1 = fmaf((1.f / 3.f) * tau, rcp32(beta1 + eps), 1.f / 3.f);
+  const float w2 = fmaf((2.f / 3.f) * tau, rcp32(beta2 + eps), 2.f / 3.f);
+  return fmaf(w2, qi2, w1 * qi1) * rcp32(w1 + w2);
+}
+
+template <>
+__device__ __forceinline__ float up3<float>(float qm, float q0, float qp) {
+  return fmaf(5.f / 6.f, q0, fmaf(1.f / 3.f, qp, (-1.f / 6.f) * qm));
+}
+template <>
+__device__ __forceinline__ float up5<float>(float qmmm, float qmm, float qm, float qp,
+                                            float qpp) {
+  return fmaf(-3.f / 60.f, qpp,
+              fmaf(27.f / 60.f, qp,
+                   fmaf(47.f / 60.f, qm, fmaf(-13.f / 60.f, qmm, (2.f / 60.f) * qmmm))));
+}
+template <>
+__device__ __forceinline__ float ce4<float>(float qmm, float qm, float qp, float qpp) {
+  return fmaf(7.f / 12.f, qm + qp, (-1.f / 12.f) * (qmm + qpp));
+}
+template <>
+__device__ __forceinline__ float ce6<float>(float qmmm, float qmm, float qm, float qp,
+                                            float qpp, float qppp) {
+  return fmaf(37.f / 60.f, qm + qp,
+              fmaf(-8.f / 60.f, qmm + qpp, (1.f / 60.f) * (qmmm + qppp)));
+}
+
 // weno.py:255-300 (cweno5z_v0)
 template <class T>
 __device__ __forceinline__ T cweno5z(T qmmm, T qmm, T qm, T qp, T qpp,

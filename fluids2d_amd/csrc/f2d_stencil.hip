@@ -124,7 +124,45 @@ struct TransportArgs {
   int xper, yper;
   int tiles_x, ntiles;
   int64_t nhalo;
+  // U = perpgrad(psi) formed on the fly instead of read (psi non-NULL):
+  // masks, contravariant factors and the periodic halo fill of k_perpgrad
+  const T* psi;
+  const int8_t* pmx;
+  const int8_t* pmy;
+  double psx, psy;
+  int pfill;
 };
+
+// U at x face (j, i): read, or perpgrad(psi) with k_perpgrad's arithmetic
+// (wrapped source when filling, (-(psi[j+1] - psi[j])) mask, then the
+// factor).  Faces read by a stage lie on rows <= ny + 3 and columns
+// <= nx + 3, never on the last padded row / column that k_perpgrad leaves
+// unchanged.
+template <class T>
+__device__ __forceinline__ T ux_at(const TransportArgs<T>& a, int j, int i) {
+  if (!a.psi) return a.Ux[(int64_t)j * a.P + i];
+  int js = j, is = i;
+  if (a.pfill) {
+    if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
+    if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
+  }
+  const int64_t ks = (int64_t)js * a.P + is;
+  const T u = (-(a.psi[ks + a.P] - a.psi[ks])) * T(a.pmx[ks]);
+  return u * T(a.psx);
+}
+
+template <class T>
+__device__ __forceinline__ T uy_at(const TransportArgs<T>& a, int j, int i) {
+  if (!a.psi) return a.Uy[(int64_t)j * a.P + i];
+  int js = j, is = i;
+  if (a.pfill) {
+    if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
+    if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
+  }
+  const int64_t ks = (int64_t)js * a.P + is;
+  const T u = (a.psi[ks + 1] - a.psi[ks]) * T(a.pmy[ks]);
+  return u * T(a.psy);
+}
 
 template <class T, int M>
 __device__ __forceinline__ T xface_global(const TransportArgs<T>& a, int j,
@@ -132,7 +170,7 @@ __device__ __forceinline__ T xface_global(const TransportArgs<T>& a, int j,
   const int64_t k = (int64_t)j * a.P + i;
   const int o = a.ocx[k];
   if (o <= 0) return T(0);
-  const T u = a.Ux[k];
+  const T u = ux_at(a, j, i);
   const T* q = a.q + k;
   return flux_by_order<T, M>(o, u, q[-3], q[-2], q[-1], q[0], q[1], q[2]) * u;
 }
@@ -143,7 +181,7 @@ __device__ __forceinline__ T yface_global(const TransportArgs<T>& a, int j,
   const int64_t k = (int64_t)j * a.P + i;
   const int o = a.ocy[k];
   if (o <= 0) return T(0);
-  const T u = a.Uy[k];
+  const T u = uy_at(a, j, i);
   const T* q = a.q + k;
   const int64_t P = a.P;
   return flux_by_order<T, M>(o, u, q[-3 * P], q[-2 * P], q[-P], q[0], q[P],
@@ -202,7 +240,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
       const bool ok = f < TY * (TX + 1) && gj < 3 + a.ny && gi <= 3 + a.nx;
       const int64_t k = ok ? (int64_t)gj * P + gi : 0;
       ox[i] = ok ? a.ocx[k] : int8_t(0);
-      ux[i] = ok ? a.Ux[k] : T(0);
+      ux[i] = ok ? ux_at(a, gj, gi) : T(0);
     }
 #pragma unroll
     for (int i = 0; i < NFY; i++) {
@@ -212,7 +250,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
       const bool ok = f < (TY + 1) * TX && gj <= 3 + a.ny && gi < 3 + a.nx;
       const int64_t k = ok ? (int64_t)gj * P + gi : 0;
       oy[i] = ok ? a.ocy[k] : int8_t(0);
-      uy[i] = ok ? a.Uy[k] : T(0);
+      uy[i] = ok ? uy_at(a, gj, gi) : T(0);
     }
     // the stage's earlier tendencies, as many as its coefficients need
     double cf[3] = {0.0, 0.0, 0.0};
@@ -653,15 +691,21 @@ int f2d_innerproduct(int dtype, void* ke, const void* U, const void* q,
   });
 }
 
-int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
-                        const void* q_in, void* q_out, const void* Ux,
-                        const void* Uy, const int8_t* ocx, const int8_t* ocy,
-                        const int8_t* msk, const void* ds_prev0,
-                        const void* ds_prev1, void* ds_out, int integrator,
-                        int stage, const double* dt, void* stream) {
+static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
+                                const void* q_in, void* q_out, const void* Ux,
+                                const void* Uy, const void* psi, const int8_t* mskx,
+                                const int8_t* msky, double sx, double sy, int fill,
+                                const int8_t* ocx, const int8_t* ocy, const int8_t* msk,
+                                const void* ds_prev0, const void* ds_prev1, void* ds_out,
+                                int integrator, int stage, const double* dt, void* stream) {
   return guarded([&] {
     check_grid(g);
     check_method(method);
+    if (psi) {
+      F2D_REQUIRE(mskx && msky, "mskx and msky are required with psi");
+    } else {
+      F2D_REQUIRE(Ux && Uy, "Ux and Uy are required");
+    }
     if (integrator >= 0) {
       F2D_REQUIRE(integrator <= F2D_ENRK3, "unknown integrator");
       int nst = integrator == F2D_EF ? 1 : 3;
@@ -681,6 +725,12 @@ int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
       a.qo = (T*)q_out;
       a.Ux = (const T*)Ux;
       a.Uy = (const T*)Uy;
+      a.psi = (const T*)psi;
+      a.pmx = mskx;
+      a.pmy = msky;
+      a.psx = sx;
+      a.psy = sy;
+      a.pfill = fill;
       a.ocx = ocx;
       a.ocy = ocy;
       a.msk = msk;
@@ -703,6 +753,29 @@ int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
       }
     });
   });
+}
+
+int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
+                        const void* q_in, void* q_out, const void* Ux,
+                        const void* Uy, const int8_t* ocx, const int8_t* ocy,
+                        const int8_t* msk, const void* ds_prev0,
+                        const void* ds_prev1, void* ds_out, int integrator,
+                        int stage, const double* dt, void* stream) {
+  return transport_stage_impl(dtype, g, method, q_in, q_out, Ux, Uy, nullptr, nullptr,
+                              nullptr, 0.0, 0.0, 0, ocx, ocy, msk, ds_prev0, ds_prev1,
+                              ds_out, integrator, stage, dt, stream);
+}
+
+int f2d_transport_stage_psi(int dtype, const f2d_grid* g, int method, const void* q_in,
+                            void* q_out, const void* psi, const int8_t* mskx,
+                            const int8_t* msky, double sx, double sy, int fill,
+                            const int8_t* ocx, const int8_t* ocy, const int8_t* msk,
+                            const void* ds_prev0, const void* ds_prev1, void* ds_out,
+                            int integrator, int stage, const double* dt, void* stream) {
+  // a NULL psi reaches the Ux/Uy check of the common path and is refused
+  return transport_stage_impl(dtype, g, method, q_in, q_out, nullptr, nullptr, psi, mskx,
+                              msky, sx, sy, fill, ocx, ocy, msk, ds_prev0, ds_prev1, ds_out,
+                              integrator, stage, dt, stream);
 }
 
 int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,

@@ -19,6 +19,23 @@ DEFAULT_RTOL = {"fp64": 1e-13, "fp32": 1e-9}
 DEFAULT_MAXIT = {"fp64": 100, "fp32": 8}
 
 
+def _destroy(h):
+    try:
+        _lib.lib().f2d_poisson_destroy(h)
+    except Exception:
+        pass
+
+
+# handles released during a graph capture, destroyed at the next safe point
+_PENDING = []
+
+
+def _flush_pending():
+    """Destroy handles whose owners went away during a graph capture."""
+    while _PENDING and not _lib.capturing():
+        _destroy(_PENDING.pop())
+
+
 class PoissonNotConverged(RuntimeError):
     pass
 
@@ -59,6 +76,7 @@ class Poisson2D:
         self.rtol = DEFAULT_RTOL[self.precision] if rt is None else float(rt)
         self.maxit = DEFAULT_MAXIT[self.precision] if mi is None else int(mi)
         self.dtype = torch.float64 if self.precision == "fp64" else torch.float32
+        _flush_pending()
         unk = np.ascontiguousarray(self.msk, dtype=np.int8)
         h = ctypes.c_void_p()
         _lib.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), mesh.grid_ref(),
@@ -91,11 +109,13 @@ class Poisson2D:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            try:
-                _lib.lib().f2d_poisson_destroy(h)
-            except Exception:
-                pass
             self._h = None
+            if _lib.capturing():
+                # freeing device memory would invalidate the capture: later
+                _PENDING.append(h)
+                return
+            _destroy(h)
+
 
     def handle(self):
         return self._h

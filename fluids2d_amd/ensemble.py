@@ -83,7 +83,7 @@ class Ensemble:
         key = self._graph_key()
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with _lib.capture(g):
             def body(i):
                 for n in range(self.G):
                     self._member_step(i, self.dtlog[i, n:n + 1])

@@ -8,6 +8,7 @@ Each model class offers
     diagnostic fields (Poisson solve with the right-hand side formed inside
     the solver, perp-gradient + fill + next CFL step in one kernel).
 """
+from . import _lib
 from . import operators as op
 from .elliptic import PoissonNotConverged  # noqa: F401  (re-export)
 
@@ -51,11 +52,23 @@ class EulerPsi(_Base):
                     cfl=p.cfl, dtmax=p.dtmax)
 
     def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
-        p = self.param
-        op.transport_stage(self.mesh, p.compflux, s.omega, alt.omega, s.U, ds_prev,
-                           ds_out, p.integrator, k, dt)
+        """One RK stage: transport + update, then the diag.  U is stored
+        only by the step's last stage (it feeds the CFL step and the state):
+        a later stage forms U = perpgrad(psi) inside its transport kernel,
+        bit for bit the same velocity, and the earlier stages skip the
+        perpgrad pass."""
+        m, p = self.mesh, self.param
+        if k > 0:
+            op.transport_stage_psi(m, p.compflux, s.omega, alt.omega, s.psi, True, True,
+                                   ds_prev, ds_out, p.integrator, k, dt)
+        else:
+            op.transport_stage(m, p.compflux, s.omega, alt.omega, s.U, ds_prev, ds_out,
+                               p.integrator, k, dt)
         s, alt = s._replace(omega=alt.omega), alt._replace(omega=s.omega)
-        self.diag_fused(s, dt_next, stats)
+        m.poisson_vertices.solve_c2v(s.omega, s.psi, stats=stats)
+        if k == _lib.NSTAGES[p.integrator] - 1:
+            op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
+                        cfl=p.cfl, dtmax=p.dtmax)
         return s, alt
 
     def materialize(self, s):
@@ -108,13 +121,20 @@ class QG(_Base):
         op.perpgrad(m, s.psi, s.U, contravariant=True)
 
     def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+        """As EulerPsi.stage: U stored by the last stage only, formed from psi
+        inside the later stages' transport."""
         m, p = self.mesh, self.param
-        op.transport_stage(m, p.compflux, s.pv, alt.pv, s.U, ds_prev, ds_out,
-                           p.integrator, k, dt)
+        if k > 0:
+            op.transport_stage_psi(m, p.compflux, s.pv, alt.pv, s.psi, True, False,
+                                   ds_prev, ds_out, p.integrator, k, dt)
+        else:
+            op.transport_stage(m, p.compflux, s.pv, alt.pv, s.U, ds_prev, ds_out,
+                               p.integrator, k, dt)
         s, alt = s._replace(pv=alt.pv), alt._replace(pv=s.pv)
         m.qg_helmholtz.solve_c2v(s.pv, s.psi, stats=stats)
-        op.perpgrad(m, s.psi, s.U, contravariant=True, fill=False, dt_dev=dt_next,
-                    cfl=p.cfl, dtmax=p.dtmax)
+        if k == _lib.NSTAGES[p.integrator] - 1:
+            op.perpgrad(m, s.psi, s.U, contravariant=True, fill=False, dt_dev=dt_next,
+                        cfl=p.cfl, dtmax=p.dtmax)
         return s, alt
 
     def materialize(self, s):

@@ -230,7 +230,7 @@ class RKIntegrator:
         before = [e.calls for e in sols]
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with _lib.capture(g):
             for i in range(G):
                 self._select_dt()
                 _lib.call("f2d_copy_f64", _lib.ptr(self._glog[i:i + 1]), _lib.ptr(self.d_dt), 1,

@@ -51,6 +51,19 @@ def transport_stage(mesh, method, q_in, q_out, U, ds_prev, ds_out, integrator, s
          stage, ptr(dt_dev), stream_ptr())
 
 
+def transport_stage_psi(mesh, method, q_in, q_out, psi, contravariant, fill, ds_prev, ds_out,
+                        integrator, stage, dt_dev):
+    """transport_stage with U = perpgrad(psi) formed in the kernel
+    (f2d_transport_stage_psi): bit for bit the U perpgrad would store."""
+    d0 = ds_prev[0] if len(ds_prev) > 0 else None
+    d1 = ds_prev[1] if len(ds_prev) > 1 else None
+    sx, sy = (1 / mesh.dy ** 2, 1 / mesh.dx ** 2) if contravariant else (1.0, 1.0)
+    call("f2d_transport_stage_psi", _dt(q_in), mesh.grid_ref(), _lib.METHODS[method],
+         ptr(q_in), ptr(q_out), ptr(psi), ptr(mesh.d_mskx), ptr(mesh.d_msky), sx, sy,
+         int(fill), ptr(mesh.d_oc.x), ptr(mesh.d_oc.y), ptr(mesh.d_msk), ptr(d0), ptr(d1),
+         ptr(ds_out), _lib.INTEGRATORS[integrator], stage, ptr(dt_dev), stream_ptr())
+
+
 def rk_addto(y, xs, integrator, stage, dt_dev):
     """addto (integrators.py:154-196) with the stage coefficients from dt."""
     x0 = xs[0]

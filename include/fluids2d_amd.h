@@ -124,6 +124,18 @@ int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
                         const void* ds_prev1, void* ds_out, int integrator,
                         int stage, const double* dt, void* stream);
 
+/* The same stage with U = perpgrad(psi) formed on the fly rather than read:
+ * perpgrad, operators.py:144-149 (Ux = -(psi[j+1] - psi[j]) mskx sx,
+ * Uy = (psi[i+1] - psi[i]) msky sy, with fill != 0 the periodic halo fill
+ * of meshes.py:135-143), bit for bit what f2d_perpgrad_ex would store.
+ * A stage whose velocity is not needed elsewhere then skips writing U. */
+int f2d_transport_stage_psi(int dtype, const f2d_grid* g, int method, const void* q_in,
+                            void* q_out, const void* psi, const int8_t* mskx,
+                            const int8_t* msky, double sx, double sy, int fill,
+                            const int8_t* ocx, const int8_t* ocy, const int8_t* msk,
+                            const void* ds_prev0, const void* ds_prev1, void* ds_out,
+                            int integrator, int stage, const double* dt, void* stream);
+
 /* addto, integrators.py:154-196 (the sum of :174): over n entries
  * y += ((c0*x0 + c1*x1) + c2*x2), with the coefficients of `integrator`
  * stage `stage` evaluated from *dt as above. */

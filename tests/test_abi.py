@@ -85,3 +85,30 @@ def test_product_package_does_not_import_the_oracle():
                 src = open(os.path.join(dirpath, f), errors="replace").read()
                 assert not re.search(r"^\s*(import|from)\s+oracle\b", src, re.M), f
                 assert "liboracle" not in src, f
+
+
+def test_package_api_surface():
+    """The drop-in classes carry the reference's methods (checked without a
+    GPU: import and attribute lookup only)."""
+    import importlib
+    for mod in ("fluids2d_amd", "fluids2d_amd.model", "fluids2d_amd.integrators",
+                "fluids2d_amd.elliptic", "fluids2d_amd.meshes", "fluids2d_amd.equations",
+                "fluids2d_amd.rsw", "fluids2d_amd.adjoint", "fluids2d_amd.ensemble",
+                "fluids2d_amd.autotune", "fluids2d_amd.diagnostics", "fluids2d_amd.operators",
+                "fluids2d_amd.roofline"):
+        importlib.import_module(mod)
+    from fluids2d_amd.elliptic import Poisson2D
+    from fluids2d_amd.model import Model
+    from fluids2d_amd.integrators import RKIntegrator
+    from fluids2d_amd.diagnostics import Bulk
+    from fluids2d_amd.ensemble import Ensemble
+    expect = {
+        Poisson2D: ("solve", "solve_c2v", "handle", "read_stats", "reset_stats"),  # elliptic.py:71-87
+        Model: ("run", "step", "set_dt", "add_forcing", "compute_diags", "finalize"),  # model.py
+        RKIntegrator: ("step", "advance", "device_steps", "begin", "end"),  # integrators.py:63-79
+        Bulk: ("__call__", "finalize", "write", "read"),  # diagnostics.py:8-76
+        Ensemble: ("steps", "run", "end", "tune_streams"),
+    }
+    for cls, names in expect.items():
+        for n in names:
+            assert callable(getattr(cls, n, None)), (cls.__name__, n)

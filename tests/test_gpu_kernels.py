@@ -131,6 +131,34 @@ def test_flat_loops_bitexact(dev, L, method):
 # ---------------------------------------------------------------------------
 # fused transport stage == divflux + fill + addto
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("method", ["weno", "upwind", "centered", "cweno"])
+def test_flat_loops_fp32(dev, L, method):
+    """FP32 loops (FMA and hardware-reciprocal forms of the reconstructions)
+    against the oracle in FP64 on the same FP32-rounded inputs: within
+    3e-6 of the largest value (a few FP32 ulps)."""
+    import torch
+    ny, nx = 20, 27
+    m = random_mask(ny, nx, 5, True, False)
+    mesh = O.Mesh(P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True), msk=m)
+    q, U, V = (a.astype(np.float32).astype(np.float64) for a in _fields(m.shape, 8))
+    n = q.size
+    dq, dU, dV = (T(a, dev).float() for a in (q, U, V))
+    for o, s in ((mesh.oc.x, 1), (mesh.oc.y, nx + 6)):
+        flx = np.zeros_like(q)
+        O.compflux(flx, U, q, o, s, method)
+        d = torch.zeros(m.shape, dtype=torch.float32, device=dev)
+        L.call("f2d_compflux", L.F32, L.ptr(d), L.ptr(dU), L.ptr(dq), L.ptr(T(o, dev)), n, s,
+               L.METHODS[method], L.stream_ptr())
+        assert_same(N(d), flx, f"compflux fp32 s={s}", rtol=3e-6)
+    for o, s, s2, sg in ((mesh.ov.y, nx + 6, 1, 1), (mesh.ov.x, 1, nx + 6, -1)):
+        du = np.zeros_like(q)
+        O.vortexforce(du, V, q, o, s, s2, sg, method)
+        d = torch.zeros(m.shape, dtype=torch.float32, device=dev)
+        L.call("f2d_vortexforce", L.F32, L.ptr(d), L.ptr(dV), L.ptr(dq), L.ptr(T(o, dev)), n,
+               s, s2, sg, L.METHODS[method], L.stream_ptr())
+        assert_same(N(d), du, f"vortexforce fp32 s={s}", rtol=3e-6)
+
+
 @pytest.mark.parametrize("xper,yper,method,integ", [
     (False, False, "weno", "rk3"), (True, False, "weno", "rk3"),
     (True, True, "weno", "enrk3"), (False, True, "upwind", "rk3"),
