@@ -55,7 +55,8 @@ class PoissonOpts(ctypes.Structure):
                 ("nu_tail", ctypes.c_int32),
                 ("solver", ctypes.c_int32),
                 ("fft_refine", ctypes.c_int32),
-                ("fft_warm", ctypes.c_int32)]
+                ("fft_warm", ctypes.c_int32),
+                ("neumann", ctypes.c_int32)]
 
 
 SOLVERS = {"auto": 0, "mg": 1, "fft": 2}
@@ -185,10 +186,10 @@ class capture:
 
 def capturing():
     """True while the current stream is being captured into a graph."""
-    import torch
     try:
+        import torch
         return torch.cuda.is_current_stream_capturing()
-    except Exception:
+    except Exception:  # also at interpreter shutdown
         return False
 
 

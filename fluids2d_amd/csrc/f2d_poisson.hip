@@ -38,6 +38,7 @@
 
 #include "f2d_common.hpp"
 #include "f2d_fft.hpp"
+#include "f2d_neumann.hpp"
 
 namespace f2d {
 
@@ -1508,6 +1509,7 @@ struct f2d_poisson {
                                    // levels, single-workgroup tail
   double inner_rtol = 1e-4;
   f2d::FftSolver* fft = nullptr;  // direct spectral solve, when it applies
+  f2d::NeumannSolver* neu = nullptr;  // Neumann (cell) operator: Jacobi-PCG
   int64_t fft_unmeasured = 0;     // FFT solves made without statistics
   std::vector<void*> allocs;
   double* reg(int r) { return part + (int64_t)r * region; }
@@ -2197,6 +2199,7 @@ int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,
                              double* bytes, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(h && us && bytes, "NULL argument");
+    F2D_REQUIRE(!h->neu, "no kernel timing on a Neumann (Jacobi-PCG) handle");
     F2D_REQUIRE(kind >= 0 && kind <= 9, "kind must be in 0 .. 9");
     F2D_REQUIRE(reps > 0, "reps must be > 0");
     hipStream_t s = S(stream);
@@ -2262,6 +2265,13 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
         nwet += v;
       }
     h->nwet = (double)nwet;
+    if (h->opts.neumann) {
+      F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT,
+                  "the Neumann operator is solved by Jacobi-PCG, not the FFT");
+      h->neu = neu_create(m.data(), h->ny, h->nx, h->xper, h->yper, g->dx, g->dy, maindiag);
+      *out = h;
+      return;
+    }
     h->singular = (maindiag == 0.0 && h->xper && h->yper &&
                    nwet == h->ny * h->nx)
                       ? 1
@@ -2313,6 +2323,7 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
   });
   if (rc != 0 && h) {
     if (h->fft) fft_destroy(h->fft);
+    if (h->neu) neu_destroy(h->neu);
     for (void* p : h->allocs) (void)hipFree(p);
     delete h;
   }
@@ -2324,6 +2335,7 @@ int f2d_poisson_destroy(f2d_poisson* h) {
     if (!h) return;
     F2D_HIP(hipDeviceSynchronize());
     if (h->fft) fft_destroy(h->fft);
+    if (h->neu) neu_destroy(h->neu);
     for (void* p : h->allocs) F2D_HIP(hipFree(p));
     delete h;
   });
@@ -2340,6 +2352,19 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
     hipStream_t s = S(stream);
     bool sync = iters != nullptr || relres != nullptr;
     const int mode = h->opts.mode;
+    if (h->neu) {
+      F2D_REQUIRE(gmode == G_FROM_B, "the Neumann operator takes b on the cells");
+      int it = 0;
+      double rr = 0.0;
+      neu_solve(h->neu, mode == F2D_POISSON_F64 ? F2D_F64 : F2D_F32, b, x, rtol, maxit, sync,
+                &it, &rr, s);
+      if (sync) {
+        if (iters) *iters = it;
+        if (relres) *relres = rr;
+        h->last_iters = it;
+      }
+      return;
+    }
     if (h->fft) {
       Ctl* c = mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
       if (mode == F2D_POISSON_F64) {
@@ -2385,23 +2410,30 @@ int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
 
 int f2d_poisson_method(f2d_poisson* h) {
   if (!h) return -1;
+  if (h->neu) return F2D_SOLVER_NEUMANN_CG;
   return h->fft ? F2D_SOLVER_FFT : F2D_SOLVER_MG;
 }
 
 int f2d_poisson_reset_stats(f2d_poisson* h, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(h, "handle is NULL");
+    h->fft_unmeasured = 0;
+    if (h->neu) {
+      neu_reset_stats(h->neu, S(stream));
+      return;
+    }
     Ctl* c = h->opts.mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
     k_reset_stats<<<1, 64, 0, S(stream)>>>(c);
-    h->fft_unmeasured = 0;
   });
 }
 
 int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(h && stats_host, "NULL argument");
-    Ctl* c = h->opts.mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
-    F2D_HIP(hipMemcpyAsync(stats_host, c->stats, 8 * sizeof(double),
+    const double* src = h->neu ? neu_stats(h->neu)
+                               : (h->opts.mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl)
+                                     ->stats;
+    F2D_HIP(hipMemcpyAsync(stats_host, src, 8 * sizeof(double),
                            hipMemcpyDeviceToHost, S(stream)));
     F2D_HIP(hipStreamSynchronize(S(stream)));
     // direct solves made without statistics count as converged solves
@@ -2411,6 +2443,7 @@ int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream) {
 
 const double* f2d_poisson_stats(f2d_poisson* h) {
   if (!h) return nullptr;
+  if (h->neu) return neu_stats(h->neu);
   Ctl* c = h->opts.mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
   return c->stats;
 }

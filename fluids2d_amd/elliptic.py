@@ -49,8 +49,6 @@ class Poisson2D:
         """tuning: solver schedule fields of f2d_poisson_opts (TUNABLE), as
         picked by fluids2d_amd.autotune; default ``param.poisson_opts``."""
         assert location in ["c", "v"]
-        if location == "c":
-            raise NotImplementedError("the Neumann (cell) operator is not on the GPU path")
         self.mesh = mesh
         self.location = location
         self.maindiag = float(maindiag)
@@ -65,6 +63,10 @@ class Poisson2D:
         # direct FFT solve in FP32: refine from the previous solution held in
         # x (the state's streamfunction) instead of a cold FP32 solve
         opts.fft_warm = int(bool(getattr(p, "poisson_warm", True)))
+        # "c": the Neumann cell operator (elliptic.py:186-190), Jacobi-PCG in FP64
+        opts.neumann = int(location == "c")
+        if location == "c":
+            opts.solver = _lib.SOLVERS["mg"] if opts.solver == _lib.SOLVERS["fft"] else opts.solver
         tuning = getattr(p, "poisson_opts", None) if tuning is None else tuning
         self.tuning = dict(tuning or {})
         for k, v in self.tuning.items():
@@ -75,6 +77,10 @@ class Poisson2D:
         mi = getattr(p, "poisson_maxit", None)
         self.rtol = DEFAULT_RTOL[self.precision] if rt is None else float(rt)
         self.maxit = DEFAULT_MAXIT[self.precision] if mi is None else int(mi)
+        if location == "c":
+            # CG iterations (FP64 inside): grows with the grid
+            self.rtol = 1e-12 if rt is None else float(rt)
+            self.maxit = 8 * max(mesh.nx, mesh.ny) + 100 if mi is None else int(mi)
         self.dtype = torch.float64 if self.precision == "fp64" else torch.float32
         _flush_pending()
         unk = np.ascontiguousarray(self.msk, dtype=np.int8)
@@ -84,7 +90,7 @@ class Poisson2D:
         self._h = h
         # "fft": direct spectral solve (exact to rounding, no rtol test);
         # "mg": multigrid-preconditioned CG to rtol
-        self.method = "fft" if _lib.lib().f2d_poisson_method(h) == 2 else "mg"
+        self.method = {2: "fft", 3: "neumann"}.get(_lib.lib().f2d_poisson_method(h), "mg")
         self.last_iters = 0
         self.last_relres = 0.0
         # iterations (refinements in FP32) the no-sync solves enqueue:

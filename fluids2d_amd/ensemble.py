@@ -93,6 +93,8 @@ class Ensemble:
         self.graph, self._key = g, key
 
     def _ready(self):
+        if not all(getattr(m.integrator.model, "capturable", True) for m in self.models):
+            return False  # a member's steps check on the host: eager only
         return self._eager >= 2 and all(
             m.integrator._dtn_valid or not m.integrator._wants_dtn() for m in self.models)
 

@@ -173,7 +173,7 @@ class RKIntegrator:
         identical."""
         dts = torch.zeros(nsteps, dtype=torch.float64, device=self.d_dt.device)
         use = getattr(self.param, "hipgraph", True) if graph is None else graph
-        use = use and not stats
+        use = use and not stats and getattr(self.model, "capturable", True)
         G = self.graph_steps
         n = 0
         while n < nsteps:

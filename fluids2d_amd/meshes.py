@@ -55,7 +55,8 @@ class Mesh:
         self.set_masks()
         self.slipcoef = self._slipcoef()
         self.set_stencils(self.param.maxorder)
-        self.poisson_centers = None  # Neumann "c" operator: euler model only
+        # the Neumann "c" operator of the euler pressure projection
+        self.poisson_centers = Poisson2D(self, "c") if self.param.model == "euler" else None
         self.poisson_vertices = Poisson2D(self, "v")
         if self.param.model in ["qg", "qgrsw", "rsw"]:
             f0, H, g = self.param.f0, self.param.H, self.param.g

@@ -36,7 +36,7 @@ _methods_extended = _methods + ["classic"]
 _integrators = ["rk3", "ef", "enrk3", "LFRA"]
 
 # models whose rhs/diag run on the GPU path
-_gpu_models = ["eulerpsi", "advection", "qg", "rsw", "qgrsw"]
+_gpu_models = ["euler", "eulerpsi", "advection", "qg", "rsw", "qgrsw"]
 _precisions = ["fp64", "fp32"]
 
 QUIET = False  # suppress the help text printed by the constructor

@@ -5,6 +5,8 @@ by operator on the device in the reference's order.
 The RK stage is the reference's: rhs into the stage's tendency, `addto` on
 every prognostic array (u.x, u.y, h), then diag.
 """
+from collections import namedtuple
+
 from . import _lib
 from . import operators as op
 from .equations import _Base
@@ -50,7 +52,7 @@ class _SW(_Base):
     def _sharp_vorticity(self, s):
         m = self.mesh
         g = self._grid()
-        dt = self._dt(s.h)
+        dt = self._dt(s.u.x)
         _ops("f2d_sharp", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y), _lib.ptr(s.U.x),
              _lib.ptr(s.U.y))
         _ops("f2d_vorticity", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y),
@@ -140,6 +142,73 @@ class QGRSW(_SW):
         op.fill(self.mesh, s.omega)
 
 
+_Du = namedtuple("prognostic", ("u",))
+
+
+class Euler(_SW):
+    """Euler in momentum-pressure form (equations.py:9-24): vortex force and
+    kinetic-energy gradient, then the pressure projection
+    (operators.py:114-119) with the Neumann cell Poisson solve.  The solve
+    checks convergence on the host, so steps are not graph-captured."""
+    prognostic = ("u",)
+    capturable = False
+
+    def rhs(self, s, ds):
+        m, p = self.mesh, self.param
+        op.vortexforce(ds.u.x, s.U.y, s.omega, m.d_ov.y, m.yshift, m.xshift, +1,
+                       p.vortexforce)
+        op.vortexforce(ds.u.y, s.U.x, s.omega, m.d_ov.x, m.xshift, m.yshift, -1,
+                       p.vortexforce)
+        _ops("f2d_addgrad", self._dt(s.u.x), self._grid(), _lib.ptr(s.ke), _lib.ptr(m.d_mskx),
+             _lib.ptr(m.d_msky), _lib.ptr(ds.u.x), _lib.ptr(ds.u.y))
+        op.fill(m, ds.u.x)
+        op.fill(m, ds.u.y)
+
+    def diag(self, s):
+        m, p = self.mesh, self.param
+        dt, g = self._dt(s.u.x), self._grid()
+        # pressure_projection (operators.py:114-119)
+        _ops("f2d_sharp", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y), _lib.ptr(s.U.x),
+             _lib.ptr(s.U.y))
+        _ops("f2d_div", dt, g, _lib.ptr(s.U.x), _lib.ptr(s.U.y), _lib.ptr(m.d_msk),
+             _lib.ptr(s.div))
+        rhs = s.div * (-m.area)
+        m.poisson_centers.solve(rhs, s.p)
+        _ops("f2d_addgrad", dt, g, _lib.ptr(s.p), _lib.ptr(m.d_mskx), _lib.ptr(m.d_msky),
+             _lib.ptr(s.u.x), _lib.ptr(s.u.y))
+        op.fill(m, s.u.x)
+        op.fill(m, s.u.y)
+        # sharp, vorticity, kinetic energy, fill (equations.py:17-22)
+        self._sharp_vorticity(s)
+        s.ke.zero_()
+        op.innerproduct(s.ke, s.U.x, s.u.x, m.d_ok.x, m.xshift, p.innerproduct)
+        op.innerproduct(s.ke, s.U.y, s.u.y, m.d_ok.y, m.yshift, p.innerproduct)
+        _ops("f2d_ke_finish", dt, g, _lib.ptr(s.ke), _lib.ptr(m.d_msk))
+        op.fill(m, s.omega)
+        op.fill(m, s.ke)
+
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+        p = self.param
+        if ds_out is None:
+            if self._ds_last is None:
+                self._ds_last = allocate_device(self.prognostic, self.mesh.shape,
+                                                s.u.x.dtype, s.u.x.device)
+            ds = self._ds_last
+        else:
+            # the integrator hands a one-field prognostic as the field itself
+            ds = ds_out if hasattr(ds_out, "u") else _Du(u=ds_out)
+        self.rhs(s, ds)
+        ys = leaves(s.u)
+        dsl = leaves(ds.u)
+        prev = [leaves(x.u if hasattr(x, "u") else x) for x in ds_prev]
+        for n, y in enumerate(ys):
+            op.rk_addto(y, [pv[n] for pv in prev] + [dsl[n]], p.integrator, k, dt)
+        self.diag(s)
+        if dt_next is not None:
+            op.cfl_dt(s.U, p.cfl, p.dtmax, dt_next)
+        return s, alt
+
+
 def qg_projection(model):
     """Geostrophic balance of a model's initial state on the device, as
     geos_adj.py:43-44 sets it up: qg_projection (operators.py:176-183) with
@@ -172,7 +241,7 @@ def qg_projection(model):
     integ.diag(model.state)
 
 
-MODELS = {"rsw": RSW, "qgrsw": QGRSW}
+MODELS = {"rsw": RSW, "qgrsw": QGRSW, "euler": Euler}
 
 
 def get_device_model(param, mesh):

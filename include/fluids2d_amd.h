@@ -287,6 +287,13 @@ typedef struct f2d_poisson_opts {
                             solve on, the first refinement starts from x
                             (its interior) instead of a cold FP32 solve;
                             0 = cold (default)                             */
+  int32_t neumann;       /* 1: the Neumann ("c") operator of
+                            get_Laplacian_sparse (elliptic.py:186-190,
+                            diagonal minus the sum of the unknown
+                            neighbours' coefficients minus maindiag), solved
+                            by Jacobi-preconditioned CG in FP64; b must be
+                            consistent when singular, x keeps the constant
+                            of its initial guess                           */
 } f2d_poisson_opts;
 
 /* Solver selection.  AUTO takes the direct spectral (FFT) solve when every
@@ -296,6 +303,7 @@ typedef struct f2d_poisson_opts {
 #define F2D_SOLVER_AUTO 0
 #define F2D_SOLVER_MG 1
 #define F2D_SOLVER_FFT 2
+#define F2D_SOLVER_NEUMANN_CG 3  /* reported by f2d_poisson_method only */
 
 /* unknowns_host: int8, padded shape, 1 at unknowns (elliptic.py:102-111:
  * mskv with the periodic halos cleared).  opts may be NULL (defaults). */

@@ -6,7 +6,8 @@ compared against.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 (``fluids2d_amd``) never does.
 
 It restates, in numpy plus the C loops of ``oracle.c``, the reference's
-algorithm for the in-scope models (eulerpsi, advection, rsw, qgrsw, qg):
+algorithm for the in-scope models (euler, eulerpsi, advection, rsw, qgrsw,
+qg):
 
 * masks / stencil orders / slip coefficient      meshes.py:70-186, noslip.py:4-36
 * WENO flux loops (C)                            weno.py:346-405
@@ -356,6 +357,8 @@ class Mesh:
         self.ok = S(set_order(self.mskx, -self.xshift, mo),
                     set_order(self.msky, -self.yshift, mo))
         self.poisson_vertices = Poisson2D(p, self, "v")
+        # meshes.py:45-47: the Neumann cell operator of the euler projection
+        self.poisson_centers = Poisson2D(p, self, "c") if p.model == "euler" else None
         if p.model in ("qg", "qgrsw", "rsw"):
             maindiag = self.area * p.f0 ** 2 / (p.g * p.H)
             self.qg_helmholtz = Poisson2D(p, self, "v", maindiag=maindiag)
@@ -512,6 +515,7 @@ def qg_inversion(mesh, pv, work, psi):
 # states, integrators, models
 # --------------------------------------------------------------------------
 SPECS = {
+    "euler": (("u", "U", "omega", "ke", "p", "div", "flx"), ("u",)),
     "eulerpsi": (("omega", "U", "psi", "vomega", "flx"), ("omega",)),
     "advection": (("q", "U", "flx"), ("q",)),
     "rsw": (("u", "h", "U", "omega", "ke", "p", "flx", "pv"), ("u", "h")),
@@ -589,7 +593,25 @@ def get_rhs_and_diag(param, mesh):
     m = param.model
     F = mesh.fill
 
-    if m == "eulerpsi":
+    if m == "euler":
+        # equations.py:9-24 with pressure_projection, operators.py:114-119
+        def rhs(s, ds):
+            addvortexforce(param, mesh, s.U, s.omega, ds.u)
+            addgrad(mesh, s.ke, ds.u)
+            F(ds.u)
+
+        def diag(s):
+            sharp(mesh, s.u, s.U)
+            div(mesh, s.U, s.div)
+            mesh.poisson_centers.solve(-s.div * mesh.area, s.p)
+            addgrad(mesh, s.p, s.u)
+            F(s.u)
+            sharp(mesh, s.u, s.U)
+            compute_vorticity(mesh, s.u, s.omega)
+            compute_kinetic_energy(param, mesh, s.u, s.U, s.ke)
+            F(s.omega)
+            F(s.ke)
+    elif m == "eulerpsi":
         def rhs(s, ds):
             divflux(param, mesh, s.flx, s.omega, s.U, ds.omega)
             F(ds.omega)
