@@ -57,6 +57,9 @@ def initial_omega(mesh, seed):
     return w * mesh.msk * mesh.area
 
 
+HIPGRAPH = True  # --no-graph: eager launches (for per-dispatch counter passes)
+
+
 def make_param(nx, precision, dt=0.0):
     import fluids2d_amd.param as fp
     fp.QUIET = True
@@ -72,6 +75,7 @@ def make_param(nx, precision, dt=0.0):
     p.maxite = 10 ** 9
     p.precision = precision
     p.dt = dt
+    p.hipgraph = HIPGRAPH
     return p
 
 
@@ -383,6 +387,8 @@ def main():
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint cost measurement")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel eagerly (no captured step graphs)")
     ap.add_argument("--ensemble", type=int, default=64, metavar="MEMBERS",
                     help="also time the batched qgrsw ensemble with this many members "
                          "(0: skip)")
@@ -391,6 +397,8 @@ def main():
                     help="first run a plain elementwise copy of MIB MiB three times "
                          "(a kernel of known traffic, to scale rocprofv3 counters)")
     a = ap.parse_args()
+    global HIPGRAPH
+    HIPGRAPH = not a.no_graph
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the multi-GPU (domain-decomposed) "
               "path is not part of this build", file=sys.stderr)

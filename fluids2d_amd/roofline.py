@@ -7,6 +7,7 @@ covers; achieved bandwidth = those bytes / mean launch time.
 """
 import torch
 
+from . import _lib
 from . import operators as op
 
 KINDS = {0: "k_down<CG> (CG update + pre-smooth + residual + restrict, fine level)",
@@ -28,9 +29,16 @@ def transport_timing(model, reps=20):
             integ.d_dt)
     op.transport_stage(*args)
     torch.cuda.synchronize()
+    # back-to-back launches replayed from a graph, as the step runs them:
+    # no host launch gap between dispatches
+    g = torch.cuda.CUDAGraph()
+    with _lib.capture(g):
+        for _ in range(reps):
+            op.transport_stage(*args)
+    g.replay()
+    torch.cuda.synchronize()
     e0.record()
-    for _ in range(reps):
-        op.transport_stage(*args)
+    g.replay()
     e1.record()
     torch.cuda.synchronize()
     us = 1e3 * e0.elapsed_time(e1) / reps
