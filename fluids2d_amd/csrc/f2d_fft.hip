@@ -11,7 +11,7 @@
 //            row, inverse FFT, real/imaginary parts stored as the two rows
 //            together with their periodic halo images
 // The FFTs are mixed radix-4/2 Stockham passes (self-sorting, ping-pong
-// between two LDS buffers) with twiddles exp(-2 pi i t / N) from a table;
+// between two LDS buffers) with twiddles from a table in pass order;
 // inverse transforms conjugate on the way in and out.
 #include <cmath>
 #include <vector>
@@ -87,19 +87,20 @@ __device__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int N, int lgN, int nf,
     b = t;
     Ns = 2;
   }
+  const int Ns0 = Ns;
   while (Ns < N) {  // radix-4 passes
     const int nb = N >> 2;
-    const int tstep = N / (4 * Ns);
     for (int w = tid; w < nf * nb; w += nth) {
       const int f = w >> (lgN - 2), j = w & (nb - 1);
       const cplx<T>* s = a + f * N;
       cplx<T>* o = b + f * N;
       const int jm = j & (Ns - 1);
-      const int st = jm * tstep;
+      // this pass's twiddles: three contiguous runs of Ns (conflict-free)
+      const cplx<T>* tp = tw + (Ns - Ns0);
       const cplx<T> a0 = s[j];
-      const cplx<T> a1 = cmul<T>(s[j + nb], tw[st]);
-      const cplx<T> a2 = cmul<T>(s[j + 2 * nb], tw[2 * st]);
-      const cplx<T> a3 = cmul<T>(s[j + 3 * nb], tw[3 * st]);
+      const cplx<T> a1 = cmul<T>(s[j + nb], tp[jm]);
+      const cplx<T> a2 = cmul<T>(s[j + 2 * nb], tp[Ns + jm]);
+      const cplx<T> a3 = cmul<T>(s[j + 3 * nb], tp[2 * Ns + jm]);
       const cplx<T> b0 = cadd<T>(a0, a2), b1 = csub<T>(a0, a2);
       const cplx<T> b2 = cadd<T>(a1, a3), b3 = csub<T>(a1, a3);
       const int base = ((j - jm) << 2) + jm;
@@ -526,15 +527,27 @@ static P* falloc(FftSolver* f, size_t bytes) {
   return reinterpret_cast<P*>(p);
 }
 
+// Twiddles in the order the radix-4 passes read them: for the pass of
+// sub-length Ns (Ns0 = 1 for even log2 N, 2 after the odd radix-2 pass),
+// three runs of Ns entries at offset Ns - Ns0, run q holding
+// exp(-2 pi i q jm / (4 Ns)), jm = 0 .. Ns-1.  A wavefront then reads
+// consecutive (or identical) entries: no LDS bank conflicts.  N entries in all.
 template <class C>
 static C* twiddles(FftSolver* f, int N) {
   std::vector<C> t(N);
-  for (int i = 0; i < N; i++) {
-    // exp(-2 pi i t / N), angle reduced to the first octant by symmetry
-    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * i / N;
-    t[i].x = (decltype(t[i].x))std::cos(ang);
-    t[i].y = (decltype(t[i].y))std::sin(ang);
-  }
+  int lg = 0;
+  while ((1 << lg) < N) lg++;
+  const int Ns0 = (lg & 1) ? 2 : 1;
+  for (int Ns = Ns0; Ns < N; Ns <<= 2)
+    for (int q = 1; q <= 3; q++)
+      for (int jm = 0; jm < Ns; jm++) {
+        const long double ang = -2.0L * 3.14159265358979323846264338327950288L *
+                                (long double)(q * jm) / (long double)(4 * Ns);
+        C c;
+        c.x = (decltype(c.x))cosl(ang);
+        c.y = (decltype(c.y))sinl(ang);
+        t[(Ns - Ns0) + (q - 1) * Ns + jm] = c;
+      }
   C* d = falloc<C>(f, sizeof(C) * N);
   F2D_HIP(hipMemcpy(d, t.data(), sizeof(C) * N, hipMemcpyHostToDevice));
   return d;
