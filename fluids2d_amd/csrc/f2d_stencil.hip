@@ -396,6 +396,27 @@ __global__ void k_rk_addto(T* __restrict__ y, const T* __restrict__ x0,
   y[k] = y[k] + rk_combine<T>(nc, c, a0, a1, a2);
 }
 
+// y += ((c0 x0 + c1 x1) + c2 x2) with given coefficients, c_i times *dt when
+// bit i of dtmask is set (the nc first terms)
+// (y may be one of the x: no restrict; each point is read before written)
+template <class T>
+__global__ void k_addto(T* y, const T* x0, const T* x1, const T* x2, int64_t n, int nc,
+                        double c0, double c1, double c2, const double* __restrict__ dt,
+                        int dtmask) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double c[3] = {c0, c1, c2};
+  if (dt) {
+    const double d = *dt;
+    for (int i = 0; i < 3; i++)
+      if ((dtmask >> i) & 1) c[i] = c[i] * d;
+  }
+  T a0 = x0[k];
+  T a1 = nc > 1 ? x1[k] : T(0);
+  T a2 = nc > 2 ? x2[k] : T(0);
+  y[k] = y[k] + rk_combine<T>(nc, c, a0, a1, a2);
+}
+
 // ---------------------------------------------------------------------------
 // centres -> vertices, operators.py:126-133
 // ---------------------------------------------------------------------------
@@ -787,6 +808,19 @@ int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,
                                               S(stream)>>>(
                                   (T*)y, (const T*)x0, (const T*)x1,
                                   (const T*)x2, n, integrator, stage, dt));
+  });
+}
+
+int f2d_addto(int dtype, void* y, const void* x0, const void* x1, const void* x2,
+              int64_t n, int nterms, double c0, double c1, double c2, const double* dt,
+              int dtmask, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(nterms >= 1 && nterms <= 3, "nterms must be 1, 2 or 3");
+    F2D_REQUIRE(y && x0 && (nterms < 2 || x1) && (nterms < 3 || x2), "NULL argument");
+    F2D_REQUIRE(!dtmask || dt, "dtmask needs dt");
+    F2D_DTYPE_DISPATCH(dtype, k_addto<T><<<grid1d(n, 256), 256, 0, S(stream)>>>(
+                                  (T*)y, (const T*)x0, (const T*)x1, (const T*)x2, n, nterms,
+                                  c0, c1, c2, dtmask ? dt : nullptr, dtmask));
   });
 }
 

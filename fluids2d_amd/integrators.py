@@ -38,6 +38,8 @@ def get_integrator(param, mesh, state):
     """integrators.py:9-17"""
     if param.integrator in _lib.NSTAGES:
         return RKIntegrator(param, mesh, state)
+    if param.integrator == "LFRA":
+        return LFRAIntegrator(param, mesh, state)
     raise NotImplementedError(f"{param.integrator} is not implemented on the GPU path")
 
 
@@ -70,8 +72,51 @@ class RKIntegrator:
         self._glog = None
         self._eager_steps = 0
         self._graph_mesh = None
+        self.forcings = []
+
+    # -- forcing (model.py:121-123, equations.py:229-238) -------------------------
+    def add_forcing(self, forcing):
+        """forcing(param, mesh, s, ds) is called after every rhs, with the
+        device state and tendency: torch tensors of the padded layout, on
+        which slicing, scalar arithmetic and numpy-array operands work as on
+        the reference's numpy arrays (a numpy operand is copied to the
+        device at each call; a forcing that keeps its arrays as device
+        tensors avoids the copy).  Stages then run unfused (rhs, forcing,
+        addto, diag) and steps are not graph-captured (the forcing is host
+        code).  A forcing that reads ``mesh.time`` sees the time at the
+        start of the step, as in the reference, when the steps are taken
+        by ``Model.run`` / ``device_step``."""
+        self.forcings.append(forcing)
+
+    def _force(self, s, ds):
+        """The forcing functions, on views of the device state and tendency
+        that accept numpy operands (see _HostOperands)."""
+        if not self.forcings:
+            return
+        sv, dsv = _host_operands(s), _host_operands(ds)
+        for f in self.forcings:
+            f(self.param, self.mesh, sv, dsv)
+
+    def _forced_stage(self, s, k, dtn):
+        """One RK stage in the reference's order (integrators.py:92-107):
+        rhs, forcing terms, addto over the prognostic arrays, diag."""
+        p = self.param
+        ds = self.dscratch[k]
+        self.model.rhs(s, ds)
+        self._force(s, ds)
+        ys = self._prog(s)
+        prev = [leaves(self.dscratch[j]) for j in range(k + 1)]
+        for n, y in enumerate(ys):
+            op.rk_addto(y, [pv[n] for pv in prev], p.integrator, k, self.d_dt)
+        self.model.diag(s)
+        if dtn is not None:
+            op.cfl_dt(self.model.cfl_velocity(s), p.cfl, p.dtmax, dtn)
 
     # -- host API (reference semantics) ----------------------------------------
+    def _prog(self, s):
+        """The prognostic arrays of state s, in field order."""
+        return [a for n in self.specs.prognostic for a in leaves(getattr(s, n))]
+
     def _host_rhs(self, s, ds):
         upload(s, self.dstate)
         dds = self.dscratch[0]
@@ -88,9 +133,11 @@ class RKIntegrator:
         self._dtn_valid = False
 
     def step(self, state, time):
-        """One step on the host state: copy in, advance, copy out."""
+        """RKIntegrator.step (integrators.py:76-79): one step of time.dt on
+        the host state (copy in, advance, copy out).  As in the reference,
+        dt is the time's current one; Model.set_dt recomputes it."""
         self.begin(state)
-        self.set_dt(time)
+        op.set_scalar(self.d_dt, time.dt)
         self.device_step(time)
         self.end(state)
 
@@ -134,6 +181,11 @@ class RKIntegrator:
         """One RK step on the device with the dt already in d_dt."""
         s, alt = self.dstate, self.dalt
         dtn = self.d_dtn if self._wants_dtn() else None
+        if self.forcings:
+            for k in range(self.nstages):
+                self._forced_stage(s, k, dtn if k == self.nstages - 1 else None)
+            self._dtn_valid = dtn is not None
+            return
         for k in range(self.nstages):
             ds_out = self.dscratch[k] if k < self.nstages - 1 else None
             prev = [self.dscratch[j] for j in range(k)]
@@ -173,7 +225,8 @@ class RKIntegrator:
         identical."""
         dts = torch.zeros(nsteps, dtype=torch.float64, device=self.d_dt.device)
         use = getattr(self.param, "hipgraph", True) if graph is None else graph
-        use = use and not stats and getattr(self.model, "capturable", True)
+        use = use and not stats and getattr(self.model, "capturable", True) \
+            and not self.forcings
         G = self.graph_steps
         n = 0
         while n < nsteps:
@@ -244,6 +297,117 @@ class RKIntegrator:
             e.graph_replayed(0, k)
         self._graphs[key] = g
         return g
+
+class LFRAIntegrator(RKIntegrator):
+    """Leap-Frog integrator with a Robert-Asselin filter (integrators.py:20-53):
+    the first iteration (ite == 0) is Euler forward, then
+      sa += 2 dt ds;  s += gamma sa + gamma sb - 2 gamma s;
+      (sa, s, sb) <- (s, sa, s)   (rightpermute, integrators.py:134-137)
+    with sa, sb the filtered previous state and ds = rhs(s).  The state's
+    prognostic arrays, the two levels and the tendency stay on the device;
+    the addto is f2d_addto (dt from the device)."""
+
+    def __init__(self, param, mesh, state):
+        param_integ = param.integrator
+        param.integrator = "ef"  # RKIntegrator's buffers: one stage
+        try:
+            super().__init__(param, mesh, state)
+        finally:
+            param.integrator = param_integ
+        self.gamma = float(param.RAgamma)
+        dev = mesh.device
+        specs = self.specs
+        self._sa = allocate_device(specs.prognostic, mesh.shape, self.dtype, dev)
+        self._sb = allocate_device(specs.prognostic, mesh.shape, self.dtype, dev)
+        self._first = True
+        self.__doc__ = LFRAIntegrator.__doc__
+
+    def _addto(self, y, terms, dtmask):
+        cs = [c for c, _ in terms] + [0.0] * (3 - len(terms))
+        xs = [x for _, x in terms] + [None] * (3 - len(terms))
+        _lib.call("f2d_addto", _lib.dtype_code(y.dtype), _lib.ptr(y), _lib.ptr(xs[0]),
+                  _lib.ptr(xs[1]), _lib.ptr(xs[2]), y.numel(), len(terms), float(cs[0]),
+                  float(cs[1]), float(cs[2]), _lib.ptr(self.d_dt), dtmask, _lib.stream_ptr())
+
+    def advance(self, stats=True):
+        p = self.param
+        s = self.dstate
+        dtn = self.d_dtn if self._wants_dtn() else None
+        ds = self.dscratch[0]
+        self.model.rhs(s, ds)
+        self._force(s, ds)
+        ys, dsl = self._prog(s), leaves(ds)
+        sa, sb = leaves(self._sa), leaves(self._sb)
+        if self._first:
+            for y, a, b, d in zip(ys, sa, sb, dsl):
+                b.copy_(y)
+                a.copy_(y)
+                self._addto(y, [(1.0, d)], 1)          # addto(s, dt, ds)
+            self._first = False
+        else:
+            g = self.gamma
+            for y, a, b, d in zip(ys, sa, sb, dsl):
+                self._addto(a, [(2.0, d)], 1)          # addto(sa, 2*dt, ds)
+                self._addto(y, [(g, a), (g, b), (-2 * g, y)], 0)
+                b.copy_(y)                             # rightpermute(sa, s, sb)
+                y.copy_(a)
+                a.copy_(b)
+        self.model.diag(s)
+        if dtn is not None:
+            op.cfl_dt(self.model.cfl_velocity(s), p.cfl, p.dtmax, dtn)
+        self._dtn_valid = dtn is not None
+
+    def device_step(self, time):
+        self._first = time.ite == 0
+        self.advance()
+        time.pushforward()
+
+    def device_steps(self, nsteps, stats=False, graph=None):
+        return super().device_steps(nsteps, stats=stats, graph=False)
+
+
+class _HostOperands(torch.Tensor):
+    """A device tensor whose arithmetic accepts numpy arrays as operands
+    (copied to the tensor's device first): forcing functions written for
+    the reference's numpy states, e.g. ``ds.pv[:, :] += curltau`` with
+    ``curltau`` a numpy array, run unchanged on the device state.  numpy
+    defers to these operators (``__array_ufunc__ = None``), so an array on
+    the left (``curl * s.q``) is converted as well."""
+
+    __array_ufunc__ = None
+
+    def _operand(self, other):
+        if isinstance(other, np.ndarray):
+            with torch._C.DisableTorchFunctionSubclass():
+                dev = self.device
+            return torch.as_tensor(other, device=dev)
+        return other
+
+    def __setitem__(self, index, value):
+        return super().__setitem__(index, self._operand(value))
+
+
+def _with_operand(name):
+    base = getattr(torch.Tensor, name)
+
+    def op(self, other):
+        return base(self, self._operand(other))
+    op.__name__ = name
+    return op
+
+
+for _name in ("add", "sub", "mul", "truediv", "pow"):
+    for _form in ("__%s__", "__r%s__", "__i%s__"):
+        _n = _form % _name
+        if hasattr(torch.Tensor, _n):
+            setattr(_HostOperands, _n, _with_operand(_n))
+
+
+def _host_operands(x):
+    if isinstance(x, torch.Tensor):
+        return x.as_subclass(_HostOperands)
+    return type(x)(*[_host_operands(v) for v in x])
+
 
 def _one(x):
     """The single prognostic array of a one-field prognostic namedtuple."""

@@ -24,6 +24,7 @@ class Model:
         self.state = State(param, self.mesh.shape)
         self.set_integrator()
         self.time = Time(param)
+        self.mesh.time = self.time  # forcing functions read mesh.time (forced_convection.py:21)
         self.io = IO(param, self.mesh, self.state, self.time)
         self.callbacks = []
         self.diags = []
@@ -116,4 +117,7 @@ class Model:
             diag()
 
     def add_forcing(self, forcing):
-        raise NotImplementedError("host forcing hooks are not on the GPU path")
+        """model.py:121-123: forcing(param, mesh, s, ds) added after every
+        rhs; it receives the device state and tendency (torch tensors, the
+        padded layout), see RKIntegrator.add_forcing."""
+        self.integrator.add_forcing(forcing)

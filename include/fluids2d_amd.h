@@ -142,6 +142,13 @@ int f2d_transport_stage_psi(int dtype, const f2d_grid* g, int method, const void
 int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,
                  const void* x2, int64_t n, int integrator, int stage,
                  const double* dt, void* stream);
+/* addto, integrators.py:154-196, with given coefficients (the LFRA
+ * integrator, integrators.py:38-53): over n entries y += ((c0 x0 + c1 x1) +
+ * c2 x2) for the first nterms terms, coefficient i multiplied by *dt (device
+ * float64) when bit i of dtmask is set. */
+int f2d_addto(int dtype, void* y, const void* x0, const void* x1, const void* x2,
+              int64_t n, int nterms, double c0, double c1, double c2, const double* dt,
+              int dtmask, void* stream);
 
 /* centerstovertices, operators.py:126-133 (addto=0/1) */
 int f2d_c2v(int dtype, const f2d_grid* g, const void* c, void* v,

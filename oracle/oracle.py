@@ -16,6 +16,8 @@ qg):
   factorised with scipy's SuperLU exactly as
   the reference does                             elliptic.py:71-203
 * SSP-RK3 / Euler forward / energy RK3 `addto`   integrators.py:82-196
+* leap-frog + Robert-Asselin filter (LFRA)       integrators.py:20-53, 134-137
+* forcing hook                                   model.py:121-123, equations.py:229-238
 * model closures                                 equations.py:73-157, 190-210
 * CFL time step and Kahan time                   model.py:71-87, timeline.py:14-35
 
@@ -588,6 +590,31 @@ def enrk3(s, dt, rhs, diag, scratch):
 INTEGRATORS = {"rk3": (rk3, 3), "ef": (ef, 1), "enrk3": (enrk3, 3)}
 
 
+def copyto(x, y):
+    """integrators.py:140-151: y[:] = x[:] over the leaves."""
+    for xx, yy in zip(_leaves(x), _leaves(y)):
+        yy[:] = xx[:]
+
+
+def lfra(s, dt, rhs, diag, scratch, first=False, gamma=0.1):
+    """integrators.py:40-53 with rightpermute (integrators.py:134-137):
+    Euler forward on the first iteration, then leap-frog and the
+    Robert-Asselin filter of the middle level."""
+    sb, sa, ds = scratch
+    rhs(s, ds)
+    if first:
+        copyto(s, sb)
+        copyto(s, sa)
+        addto(s, dt, ds)
+    else:
+        addto(sa, 2 * dt, ds)
+        addto(s, gamma, sa, gamma, sb, -2 * gamma, s)
+        copyto(s, sb)
+        copyto(sa, s)
+        copyto(sb, sa)
+    diag(s)
+
+
 def get_rhs_and_diag(param, mesh):
     """equations.py:73-157, 190-210 for the in-scope models."""
     m = param.model
@@ -704,7 +731,10 @@ class Model:
         self.mesh = Mesh(param, msk)
         names, prog = SPECS[param.model]
         self.state = allocate(names, self.mesh.shape)
-        caller, nst = INTEGRATORS[param.integrator]
+        if param.integrator == "LFRA":
+            caller, nst = lfra, 3
+        else:
+            caller, nst = INTEGRATORS[param.integrator]
         self._integ = caller
         self.scratch = [allocate(prog, self.mesh.shape) for _ in range(nst)]
         self.rhs, self.diag = get_rhs_and_diag(param, self.mesh)
@@ -725,10 +755,28 @@ class Model:
             maxU = np.max(np.abs(U.x)) + np.max(np.abs(U.y)) + 1e-99
         self.time.dt = min(p.cfl / maxU, p.dtmax)
 
+    def add_forcing(self, forcing):
+        """model.py:121-123 / equations.py:229-238: forcing(param, mesh, s,
+        ds) runs after every rhs.  mesh.time is the model's Time, as the
+        reference's forcing experiments set it (forced_convection.py:6)."""
+        rhs = self.rhs
+        self.mesh.time = self.time
+
+        def newrhs(s, ds):
+            rhs(s, ds)
+            forcing(self.param, self.mesh, s, ds)
+
+        self.rhs = newrhs
+
     def step(self):
-        """RKIntegrator.step, integrators.py:76-79"""
-        self._integ(_ProgView(self.state, self.prog), self.time.dt, self.rhs,
-                    self.diag, [_ProgView(x, self.prog) for x in self.scratch])
+        """RKIntegrator.step, integrators.py:76-79; LFRAintegrator.step,
+        integrators.py:32-36 (first = ite == 0)"""
+        args = (_ProgView(self.state, self.prog), self.time.dt, self.rhs,
+                self.diag, [_ProgView(x, self.prog) for x in self.scratch])
+        if self._integ is lfra:
+            self._integ(*args, first=self.time.ite == 0, gamma=self.param.RAgamma)
+        else:
+            self._integ(*args)
         self.time.pushforward()
 
     def run(self, nsteps=None):

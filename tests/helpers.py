@@ -11,7 +11,7 @@ class P:
                  yperiodic=False, halowidth=3, noslip=None, f0=10.0, beta=0.0, g=1,
                  H=1, tend=1.0, dt=0.0, maxite=100, dtmax=9e99, integrator="rk3",
                  cfl=0.9, compflux="weno", vortexforce="weno", innerproduct="weno",
-                 maxorder=6, nthreads=1)
+                 maxorder=6, nthreads=1, RAgamma=0.1)
         d.update(kw)
         for k, v in d.items():
             setattr(self, k, v)

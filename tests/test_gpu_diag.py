@@ -79,6 +79,7 @@ def test_bulk_matches_reference_formulas(dev, model, nx, ny, xper, yper, nisl, p
     b = _model(model, nx, ny, xper, yper, nisl, precision)
     k = 0
     for ite in range(1, 10):
+        b.set_dt()  # Model.run's CFL step (Model.step keeps time.dt)
         b.step()
         if ite % 3:
             continue

@@ -29,7 +29,8 @@ def _param(**kw):
 def _oracle_param(p):
     keys = ["model", "nx", "ny", "Lx", "Ly", "xperiodic", "yperiodic", "halowidth",
             "noslip", "f0", "beta", "g", "H", "tend", "dt", "maxite", "dtmax",
-            "integrator", "cfl", "compflux", "vortexforce", "innerproduct", "maxorder"]
+            "integrator", "cfl", "compflux", "vortexforce", "innerproduct", "maxorder",
+            "RAgamma"]
     return P(**{k: getattr(p, k) for k in keys})
 
 

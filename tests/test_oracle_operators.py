@@ -119,6 +119,70 @@ def test_integrator_amplification(name, poly):
         assert np.allclose(s[0], g * np.array([1.0, -2.0]), rtol=1e-14, atol=0)
 
 
+def _lfra_run(p, gamma, nsteps, y0):
+    """The oracle's lfra on the oscillation equation y' = i w y (y as the
+    pair re, im); returns the state after each step."""
+    s = (np.array(y0, dtype=float),)
+    scratch = [(np.zeros(2),) for _ in range(3)]
+
+    def rhs(x, dx):
+        dx[0][0] = -x[0][1]
+        dx[0][1] = x[0][0]
+
+    out = []
+    for n in range(nsteps):
+        O.lfra(s, p, rhs, lambda x: None, scratch, first=n == 0, gamma=gamma)
+        out.append(s[0][0] + 1j * s[0][1])
+    return np.array(out)
+
+
+def test_lfra_unfiltered_is_leapfrog():
+    """gamma = 0: Euler forward first, then y(n+1) = y(n-1) + 2 dt f(y(n))."""
+    dt, y0 = 0.3, 1.0 + 0.5j
+    ys = _lfra_run(dt, 0.0, 6, [y0.real, y0.imag])
+    ref = [y0, y0 + dt * 1j * y0]
+    for n in range(1, 6):
+        ref.append(ref[n - 1] + 2 * dt * 1j * ref[n])
+    assert np.allclose(ys, ref[1:], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("p,gamma", [(0.2, 0.1), (0.05, 0.2), (0.5, 0.06)])
+def test_lfra_physical_mode_amplification(p, gamma):
+    """Leap-frog with the Robert-Asselin filter on y' = i w y (p = w dt):
+    the physical mode's amplification factor is
+    A = gamma + i p + sqrt((1 - gamma)^2 - p^2) (Durran, Numerical Methods
+    for Fluid Dynamics, 2nd ed., sec. 2.3.5); the computational mode,
+    gamma + i p - sqrt(...), has |A| < 1 and has died out after a few
+    hundred steps, so the step-to-step ratio of the state is A."""
+    ys = _lfra_run(p, gamma, 600, [1.0, 0.0])
+    A = gamma + 1j * p + np.sqrt((1 - gamma) ** 2 - p * p)
+    ratio = ys[-1] / ys[-2]
+    assert abs(ratio - A) < 1e-12
+    # the filter damps the physical mode (plain leap-frog keeps |A| = 1)
+    assert 1 - abs(ratio) > 1e-4
+
+
+def test_forcing_runs_after_rhs():
+    """add_forcing (equations.py:229-238): advection with U = 0 and a
+    forcing adding c to the tendency on wet cells grows q as c t there, for
+    every integrator; mesh.time is the model's time during the forcing."""
+    for integ in ("rk3", "enrk3", "ef", "LFRA"):
+        p = P(model="advection", nx=12, ny=8, integrator=integ, dt=0.1, tend=1.0,
+              maxite=7)
+        m = O.Model(p)
+        seen = []
+
+        def forcing(param, mesh, s, ds):
+            ds.q[:] += 0.25 * mesh.msk
+            seen.append(mesh.time.t)
+
+        m.add_forcing(forcing)
+        m.run()
+        wet = m.mesh.msk == 1
+        assert np.allclose(m.state.q[wet], 0.25 * m.time.t, rtol=1e-13)
+        assert seen[0] == 0.0 and max(seen) < m.time.t
+
+
 def test_time_kahan_summation():
     p = P(dt=0.1, tend=1e9, maxite=10 ** 9)
     t = O.Time(p)
