@@ -123,6 +123,7 @@ struct TransportArgs {
   int ny, nx, P;  // interior sizes, padded row length
   int xper, yper;
   int tiles_x, ntiles;
+  int nhb;  // halo-ring blocks (block indices 0 .. nhb-1)
   int64_t nhalo;
   // U = perpgrad(psi) formed on the fly instead of read (psi non-NULL):
   // masks, contravariant factors and the periodic halo fill of k_perpgrad
@@ -202,6 +203,38 @@ __device__ __forceinline__ void stage_update(const TransportArgs<T>& a,
   a.qo[k] = qin + rk_combine<T>(nc, c, x0, x1, x2);
 }
 
+// halo ring cell h (0 .. nhalo-1): a periodic halo cell takes the
+// tendency of its wrapped interior source, a closed-direction one ds = 0
+template <class T, int M>
+__device__ __forceinline__ void transport_halo(const TransportArgs<T>& a, int64_t h) {
+  if (h >= a.nhalo) return;
+  const int P = a.P;
+  const int NXP = a.nx + 6;
+  int j, i;
+  if (h < 6 * (int64_t)NXP) {
+    int row = (int)(h / NXP);
+    i = (int)(h - (int64_t)row * NXP);
+    j = row < 3 ? row : a.ny + row;  // rows 0..2, ny+3..ny+5
+  } else {
+    int64_t u = h - 6 * (int64_t)NXP;
+    j = 3 + (int)(u / 6);
+    int c = (int)(u % 6);
+    i = c < 3 ? c : a.nx + c;
+  }
+  int js = j, is = i;
+  if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
+  if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
+  T ds = T(0);
+  if (js >= 3 && js < 3 + a.ny && is >= 3 && is < 3 + a.nx) {
+    int64_t ks = (int64_t)js * P + is;
+    T dfx = xface_global<T, M>(a, js, is + 1) - xface_global<T, M>(a, js, is);
+    T dfy = yface_global<T, M>(a, js + 1, is) - yface_global<T, M>(a, js, is);
+    ds = ((-dfx) - dfy) * T(a.msk[ks]);
+  }
+  int64_t k = (int64_t)j * P + i;
+  stage_update<T>(a, k, ds, a.q[k]);
+}
+
 template <class T, int M, int TY, int TX>
 __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
   constexpr int LW = TX + 6;
@@ -211,8 +244,11 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
   const int t = threadIdx.x;
   const int P = a.P;
 
-  if ((int)blockIdx.x < a.ntiles) {
-    const int by = blockIdx.x / a.tiles_x, bx = blockIdx.x - by * a.tiles_x;
+  // the halo-ring blocks come first: they start with the tiles instead of
+  // trailing them
+  if ((int)blockIdx.x >= a.nhb) {
+    const int tile = (int)blockIdx.x - a.nhb;
+    const int by = tile / a.tiles_x, bx = tile - by * a.tiles_x;
     const int j0 = 3 + by * TY, i0 = 3 + bx * TX;
     const int NYP = a.ny + 6, NXP = a.nx + 6;
     // Every global load of the tile is issued here, before any is used, so
@@ -328,33 +364,145 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
     return;
   }
 
-  // halo ring
-  int64_t h = (int64_t)(blockIdx.x - a.ntiles) * 256 + t;
-  if (h >= a.nhalo) return;
-  const int NXP = a.nx + 6;
-  int j, i;
-  if (h < 6 * (int64_t)NXP) {
-    int row = (int)(h / NXP);
-    i = (int)(h - (int64_t)row * NXP);
-    j = row < 3 ? row : a.ny + row;  // rows 0..2, ny+3..ny+5
-  } else {
-    int64_t u = h - 6 * (int64_t)NXP;
-    j = 3 + (int)(u / 6);
-    int c = (int)(u % 6);
-    i = c < 3 ? c : a.nx + c;
+  transport_halo<T, M>(a, (int64_t)blockIdx.x * 256 + t);
+}
+
+// The same stage for tiles that divide the interior (ny % TY == 0,
+// nx % TX == 0, TX a power of two dividing 256): every tile is whole, so
+// no load or store needs a bounds test, and thread t owns column t % TX of
+// rows t / TX + (256 / TX) i, whose offsets step by a constant.  Only the
+// tile's last x-face column and y-face row may lie on the wrapped periodic
+// edge (ux_at / uy_at there).  Per face and per cell the arithmetic is
+// that of k_transport.
+template <class T, int M, int TY, int TX>
+__global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a) {
+  constexpr int LW = TX + 6;
+  constexpr int RP = 256 / TX;             // rows per pass of the workgroup
+  constexpr int NQR = TY + 6;              // q rows of the tile
+  constexpr int NQ = (NQR + RP - 1) / RP;  // q rows per thread
+  constexpr int NR = TY / RP;              // cell (and face) rows per thread
+  static_assert(TX <= 256 && (TX & (TX - 1)) == 0 && TY % RP == 0, "tile shape");
+  __shared__ T sq[NQR * LW];
+  __shared__ T sfx[TY * (TX + 1)];
+  __shared__ T sfy[(TY + 1) * TX];
+  const int t = threadIdx.x;
+  if ((int)blockIdx.x < a.nhb) {  // halo-ring blocks first, as in k_transport
+    transport_halo<T, M>(a, (int64_t)blockIdx.x * 256 + t);
+    return;
   }
-  int js = j, is = i;
-  if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
-  if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
-  T ds = T(0);
-  if (js >= 3 && js < 3 + a.ny && is >= 3 && is < 3 + a.nx) {
-    int64_t ks = (int64_t)js * P + is;
-    T dfx = xface_global<T, M>(a, js, is + 1) - xface_global<T, M>(a, js, is);
-    T dfy = yface_global<T, M>(a, js + 1, is) - yface_global<T, M>(a, js, is);
-    ds = ((-dfx) - dfy) * T(a.msk[ks]);
+  const uint32_t P = (uint32_t)a.P;
+  const int tile = (int)blockIdx.x - a.nhb;
+  const int by = tile / a.tiles_x, bx = tile - by * a.tiles_x;
+  const int j0 = 3 + by * TY, i0 = 3 + bx * TX;
+  const int c = t & (TX - 1), r0 = t / TX;
+  const uint32_t oc = (uint32_t)(j0 + r0) * P + (uint32_t)(i0 + c);  // cell (r0, c)
+  constexpr uint32_t one = 1;
+  // every global load first (see k_transport)
+  T qv[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; i++) {
+    const int rr = r0 + RP * i;
+    qv[i] = (NQR % RP == 0 || rr < NQR) ? a.q[oc - 3 * P + (uint32_t)(RP * i) * P] : T(0);
   }
-  int64_t k = (int64_t)j * P + i;
-  stage_update<T>(a, k, ds, a.q[k]);
+  const bool hq = t < 6 * NQR;  // q halo columns: 3 left, 3 right of the tile
+  const int hr = t / 6, hcc = t - 6 * (t / 6);
+  const int hc = hcc < 3 ? hcc : TX + hcc;  // LDS column
+  T qh = T(0);
+  if (hq) qh = a.q[(uint32_t)(j0 - 3 + hr) * P + (uint32_t)(i0 - 3 + hc)];
+  int8_t ox[NR], oy[NR], mv[NR];
+  T ux[NR], uy[NR], d0[NR], d1[NR];
+  double cf[3] = {0.0, 0.0, 0.0};
+  const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, *a.dt, cf);
+#pragma unroll
+  for (int i = 0; i < NR; i++) {
+    const uint32_t k = oc + (uint32_t)(RP * i) * P;
+    ox[i] = a.ocx[k];
+    oy[i] = a.ocy[k];
+    mv[i] = a.msk[k];
+    if (a.psi) {
+      ux[i] = ((-(a.psi[k + P] - a.psi[k])) * T(a.pmx[k])) * T(a.psx);
+      uy[i] = ((a.psi[k + one] - a.psi[k]) * T(a.pmy[k])) * T(a.psy);
+    } else {
+      ux[i] = a.Ux[k];
+      uy[i] = a.Uy[k];
+    }
+    d0[i] = nc > 1 ? a.ds0[k] : T(0);
+    d1[i] = nc > 2 ? a.ds1[k] : T(0);
+  }
+  // the tile's last x-face column (rows t < TY) and y-face row (columns t < TX)
+  int8_t oxe = 0, oye = 0;
+  T uxe = T(0), uye = T(0);
+  if (t < TY) {
+    oxe = a.ocx[(uint32_t)(j0 + t) * P + (uint32_t)(i0 + TX)];
+    uxe = ux_at(a, j0 + t, i0 + TX);
+  }
+  if (t < TX) {
+    oye = a.ocy[(uint32_t)(j0 + TY) * P + (uint32_t)(i0 + t)];
+    uye = uy_at(a, j0 + TY, i0 + t);
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; i++) {
+    const int rr = r0 + RP * i;
+    if (NQR % RP == 0 || rr < NQR) sq[rr * LW + c + 3] = qv[i];
+  }
+  if (hq) sq[hr * LW + hc] = qh;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NR; i++) {
+    const int r = r0 + RP * i;
+    T v = T(0);
+    const int o = ox[i];
+    if (o > 0) {
+      const T u = ux[i];
+      const T* sp = sq + (r + 3) * LW + c;  // sp[0] = q[gi-3]
+      v = flux_by_order<T, M>(o, u, sp[0], sp[1], sp[2], sp[3], sp[4], sp[5]) * u;
+    }
+    sfx[r * (TX + 1) + c] = v;
+    T w = T(0);
+    const int oo = oy[i];
+    if (oo > 0) {
+      const T u = uy[i];
+      const T* sp = sq + r * LW + (c + 3);  // sp[0] = q[gj-3]
+      w = flux_by_order<T, M>(oo, u, sp[0], sp[LW], sp[2 * LW], sp[3 * LW], sp[4 * LW],
+                              sp[5 * LW]) *
+          u;
+    }
+    sfy[r * TX + c] = w;
+  }
+  if (t < TY) {
+    T v = T(0);
+    if (oxe > 0) {
+      const T* sp = sq + (t + 3) * LW + TX;
+      v = flux_by_order<T, M>(oxe, uxe, sp[0], sp[1], sp[2], sp[3], sp[4], sp[5]) * uxe;
+    }
+    sfx[t * (TX + 1) + TX] = v;
+  }
+  if (t < TX) {
+    T w = T(0);
+    if (oye > 0) {
+      const T* sp = sq + TY * LW + (t + 3);
+      w = flux_by_order<T, M>(oye, uye, sp[0], sp[LW], sp[2 * LW], sp[3 * LW], sp[4 * LW],
+                              sp[5 * LW]) *
+          uye;
+    }
+    sfy[TY * TX + t] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NR; i++) {
+    const int r = r0 + RP * i;
+    const uint32_t k = oc + (uint32_t)(RP * i) * P;
+    const T dfx = sfx[r * (TX + 1) + c + 1] - sfx[r * (TX + 1) + c];
+    const T dfy = sfy[(r + 1) * TX + c] - sfy[r * TX + c];
+    const T ds = ((-dfx) - dfy) * T(mv[i]);
+    if (a.dso) a.dso[k] = ds;
+    if (nc > 0) {
+      const T x0 = (nc == 1) ? ds : d0[i];
+      const T x1 = (nc == 2) ? ds : (nc > 2 ? d1[i] : T(0));
+      const T x2 = (nc == 3) ? ds : T(0);
+      a.qo[k] = sq[(r + 3) * LW + c + 3] + rk_combine<T>(nc, cf, x0, x1, x2);
+    }
+  }
 }
 
 template <class T, int M, int TY, int TX>
@@ -364,8 +512,15 @@ void launch_transport_tile(const TransportArgs<T>& a0, hipStream_t s) {
   int tiles_y = (a.ny + TY - 1) / TY;
   a.ntiles = a.tiles_x * tiles_y;
   a.nhalo = 6 * (int64_t)(a.nx + 6) + 6 * (int64_t)a.ny;
-  int nb = a.ntiles + (int)((a.nhalo + 255) / 256);
-  k_transport<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
+  a.nhb = (int)((a.nhalo + 255) / 256);
+  int nb = a.ntiles + a.nhb;
+  // whole tiles, and every padded offset within 32 bits
+  const bool whole = a.ny % TY == 0 && a.nx % TX == 0 &&
+                     (int64_t)(a.ny + 6) * (int64_t)a.P < ((int64_t)1 << 31);
+  if (whole)
+    k_transport_full<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
+  else
+    k_transport<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
 }
 
 // tile shape by code (f2d_grid.tile): the arithmetic per cell is the same

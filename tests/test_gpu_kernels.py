@@ -419,6 +419,52 @@ def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean):
     assert errs[2][1] < (1e-9 if nref > 1 else 1e-4), errs
 
 
+@pytest.mark.parametrize("ny,nx,mode,maindiag", [
+    (1024, 1024, "f64", 0.0), (1024, 1024, "refine", 0.0), (2048, 256, "f64", 0.5),
+    (256, 2048, "refine", 0.0), (1024, 2048, "f64", 0.0), (2048, 1024, "refine", 0.3)])
+def test_poisson_fft_large_vs_dft(dev, L, ny, nx, mode, maindiag):
+    """Large direct solves (the wavefront row/column kernels, 256..2048
+    points a transform) against numpy's 2-D DFT closed form of the same
+    operator (oracle.fft_poisson_periodic)."""
+    import torch
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True,
+          Lx=nx / 512.0, Ly=ny / 700.0)
+    m = O.default_mask(ny, nx, 3, True, True)
+    unk = np.zeros(m.shape, dtype=np.int8)
+    unk[3:-3, 3:-3] = 1
+    rng = np.random.default_rng(17)
+    b = np.zeros(m.shape)
+    b[3:-3, 3:-3] = rng.standard_normal((ny, nx)) + 0.25
+    g = L.make_grid(p)
+    opts = L.PoissonOpts()
+    opts.mode = L.POISSON_F64 if mode == "f64" else L.POISSON_F32_REFINE
+    opts.solver = L.SOLVERS["fft"]
+    h = ctypes.c_void_p()
+    L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+           maindiag, ctypes.byref(opts), ctypes.byref(h))
+    dt = torch.float64 if mode == "f64" else torch.float32
+    dx = torch.zeros(m.shape, dtype=dt, device=dev)
+    it, rr = ctypes.c_int(), ctypes.c_double()
+    L.call("f2d_poisson_solve", h, L.ptr(T(b, dev).to(dt)), L.ptr(dx), 1e-10, 6,
+           ctypes.byref(it), ctypes.byref(rr), L.stream_ptr())
+    x = N(dx).astype(np.float64)
+    L.call("f2d_poisson_destroy", h)
+    bi = b[3:-3, 3:-3]
+    if maindiag == 0.0:
+        bi = bi - bi.mean()
+    xr = O.fft_poisson_periodic(bi, p.Lx / nx, p.Ly / ny, maindiag)
+    xi = x[3:-3, 3:-3]
+    if maindiag == 0.0:
+        xi = xi - xi.mean()
+    err = np.linalg.norm(xi - xr) / np.linalg.norm(xr)
+    print(f"\nfft {mode} {ny}x{nx} md={maindiag}: relres={rr.value:.2e} err={err:.2e}")
+    assert err <= (1e-11 if mode == "f64" else 1e-6)
+    assert rr.value <= (1e-11 if mode == "f64" else 1e-9)
+    assert np.array_equal(x[:3, 3:-3], x[-6:-3, 3:-3])
+    assert np.array_equal(x[3:-3, :3], x[3:-3, -6:-3])
+    assert np.array_equal(x[:3, :3], x[-6:-3, -6:-3])
+
+
 def test_poisson_fft_matches_mg(dev, L):
     p = P(model="eulerpsi", nx=256, ny=256, xperiodic=True, yperiodic=True)
     m = O.default_mask(256, 256, 3, True, True)
