@@ -332,6 +332,14 @@ def _lib_tiles():
     return _lib.TILES
 
 
+def _traffic_key(kernel, tile):
+    """The transport's traffic depends on its LDS tile (halo re-reads): its
+    summary row is keyed 'k_transport <rows>x<columns>'."""
+    if kernel.startswith("k_transport") and tile is not None:
+        return "k_transport " + "x".join(map(str, _lib_tiles()[tile]))
+    return kernel
+
+
 def traffic_of(kernel, nx, prec):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary (profiles/traffic_<workload>.txt, JSON: FETCH_SIZE +
@@ -470,7 +478,8 @@ def main():
                    "nx": nx, "ny": nx},
         "roofline": {"bound": "hbm", "achieved": prof["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": prof["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": traffic_of(prof["kernel"], nx, head_prec),
+                     "traffic": traffic_of(_traffic_key(prof["kernel"], tiles.get(head_prec)),
+                                           nx, head_prec),
                      "kernel": prof["kernel"],
                      "bytes_per_launch": prof["bytes"], "avg_launch_us": prof["us"],
                      "kernels": {k: {"us": round(v["us"], 2), "per_step": v["per_step"],
