@@ -10,8 +10,9 @@ The averages are FP64 sums over the padded arrays, formed on the device from
 the model's device state (`f2d_bulk_diagnostics`, also in FP32 runs), so a
 run with diagnostics never copies the state to the host: records stay in
 device memory until ``write``.  eulerpsi, whose state has no ke, takes the
-cell-centred 1/2 u.U of its velocity.  The netCDF file needs netCDF4 (as in
-the reference); ``ncfile=None`` keeps the records in memory only.
+cell-centred 1/2 u.U of its velocity.  The netCDF file is written with
+netCDF4 when importable, else as NetCDF-3 classic through scipy
+(fluids2d_amd.ncfile); ``ncfile=None`` keeps the records in memory only.
 """
 from collections import namedtuple
 
@@ -77,7 +78,7 @@ class Bulk:
         cols = [self.t[:n].copy()] + [rec[:, q] for q in range(4)]
         self.history = [np.concatenate([h, c]) for h, c in zip(self.history, cols)]
         if self.ncfile is not None:
-            from netCDF4 import Dataset
+            from .ncfile import Dataset
             idx = slice(self.k0, self.k0 + n)
             with Dataset(self.ncfile, "r+") as nc:
                 for name, x in zip(VARIABLES, cols):
@@ -87,7 +88,7 @@ class Bulk:
 
     def read(self):
         if self.ncfile is not None:
-            from netCDF4 import Dataset
+            from .ncfile import Dataset
             with Dataset(self.ncfile, "r") as nc:
                 return BulkData(*[nc.variables[name][:] for name in VARIABLES])
         pending = self.rec[:self.kt].cpu().numpy()
@@ -97,7 +98,7 @@ class Bulk:
 
 def create_file(ncfile, param, variables):
     """diagnostics.py:79-92"""
-    from netCDF4 import Dataset
+    from .ncfile import Dataset
     from .io import get_atts_from_param
     with Dataset(ncfile, "w", format="NETCDF4") as nc:
         nc.setncatts(get_atts_from_param(param))
@@ -112,6 +113,6 @@ def get_number_of_records(ncfile):
     import os
     if not os.path.isfile(ncfile):
         return 0
-    from netCDF4 import Dataset
+    from .ncfile import Dataset
     with Dataset(ncfile) as nc:
         return len(nc.dimensions["t"])

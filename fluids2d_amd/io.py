@@ -1,6 +1,8 @@
-"""NetCDF history output (io.py:1-92), unchanged in format: all Param fields
-as global attributes, float32 variables on (t, y, x).  netCDF4 is imported
-only when a history file is requested (nhis > 0)."""
+"""NetCDF history output (io.py:1-92), unchanged in layout: all Param fields
+as global attributes, float32 variables on (t, y, x).  The file is NETCDF4
+through netCDF4 when that is importable, else NetCDF-3 classic through
+scipy (fluids2d_amd.ncfile); it is opened only when a history file is
+requested (nhis > 0)."""
 from .states import vectors
 
 
@@ -12,7 +14,7 @@ class IO:
             create_file(param, mesh, state, time)
 
     def write(self, state, time):
-        from netCDF4 import Dataset
+        from .ncfile import Dataset
         with Dataset(self.param.outputfile, "r+") as nc:
             nc.variables["t"][self.kt] = time.t
             nc.variables["ite"][self.kt] = time.ite
@@ -52,7 +54,7 @@ def get_atts_from_param(param):
 
 
 def create_file(param, mesh, state, time):
-    from netCDF4 import Dataset
+    from .ncfile import Dataset
     atts = get_atts_from_param(param)
     dimensions = {"x": mesh.shape[1], "y": mesh.shape[0], "t": None}
     timevar = ["t", "ite", "dt"]

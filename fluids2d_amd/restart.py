@@ -3,11 +3,12 @@
 The reference restarts from its netCDF history file: ``read_param``,
 ``set_state_and_time_from_file`` (the last record of ux, uy, t, ite), then
 ``integrator.diag``; ``get_model_from_restart`` does both.  The same
-functions are here, with netCDF4 imported only when called, and they read
-whichever prognostic variables the history stores (``var_to_store``), so
-eulerpsi (omega), qg (pv), rsw/qgrsw (u, h) restart too.
+functions are here (files through fluids2d_amd.ncfile: netCDF4 when
+importable, else NetCDF-3 classic through scipy), and they read whichever
+prognostic variables the history stores (``var_to_store``), so eulerpsi
+(omega), qg (pv), rsw/qgrsw (u, h) restart too.
 
-``save_checkpoint`` / ``load_checkpoint`` need no netCDF4: an .npz holding
+``save_checkpoint`` / ``load_checkpoint`` need no NetCDF: an .npz holding
 every state array (float64, which holds an FP32 run's values exactly), the
 Param (JSON) and the time with its Kahan compensation.  The streamfunction
 and velocities are stored with the prognostic fields, so the Poisson solves
@@ -31,7 +32,7 @@ def get_keys_from_param(param):
 
 def read_param(ncfile):
     """restart.py:12-19"""
-    from netCDF4 import Dataset
+    from .ncfile import Dataset
     from .param import Param
     param = Param()
     keys = get_keys_from_param(param)
@@ -39,7 +40,13 @@ def read_param(ncfile):
         for key in keys:
             if key in nc.ncattrs():
                 val = nc.getncattr(key)
-                setattr(param, key, None if isinstance(val, str) and val == "None" else val)
+                if isinstance(val, str) and val == "None":
+                    val = None
+                elif isinstance(val, str) and (key == "noslip" or
+                                               isinstance(getattr(param, key), (list, tuple))):
+                    # a NetCDF-3 file holds a list of names as one string
+                    val = [v for v in val.split(",") if v]
+                setattr(param, key, val)
     return param
 
 
@@ -52,7 +59,7 @@ def check_two_params_are_same(param1, param2):
 def set_state_and_time_from_file(ncfile, model):
     """restart.py:32-44: the last record of the stored prognostic variables,
     t and ite; then diag."""
-    from netCDF4 import Dataset
+    from .ncfile import Dataset
     from .states import get_specs
     s, t = model.state, model.time
     prog = get_specs(model.param).prognostic
