@@ -325,9 +325,8 @@ __device__ __forceinline__ T rk_combine(int nc, const double c[3], T x0, T x1,
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic grid reductions: every block writes its partial, the last
-// block to arrive (device-scope counter) reduces the partials in a fixed
-// order and runs a finaliser.  Counters reset themselves.
+// Block reduction of the grid reductions (every block writes its partial;
+// a following kernel or host step reduces the partials in a fixed order).
 // ---------------------------------------------------------------------------
 template <int BS>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -343,63 +342,5 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   __syncthreads();
   return r;
 }
-
-template <int BS>
-__device__ __forceinline__ double block_max(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-#pragma unroll
-  for (int s = BS / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
-    __syncthreads();
-  }
-  double r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-__device__ __forceinline__ void store_partial(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_partial(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Returns true in the last block (all threads of it).  `nvals` partial
-// values per block are stored at partials[k * nblocks + block].
-template <int BS>
-__device__ __forceinline__ bool last_block_arrive(unsigned* counter,
-                                                  int* sh_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    *sh_flag = (prev == nb - 1);
-    if (prev == nb - 1)
-      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  bool last = *sh_flag != 0;
-  if (last) __threadfence();
-  return last;
-}
-
-// Sum (or max) of nb partials in a fixed order by one block.
-template <int BS>
-__device__ __forceinline__ double reduce_partials(const double* part, int nb,
-                                                  double* sh, bool is_max) {
-  double acc = is_max ? 0.0 : 0.0;
-  for (int i = threadIdx.x; i < nb; i += BS) {
-    double v = load_partial(part + i);
-    acc = is_max ? fmax(acc, v) : acc + v;
-  }
-  return is_max ? block_max<BS>(acc, sh) : block_sum<BS>(acc, sh);
-}
-
-inline int block_count(dim3 g) { return (int)(g.x * g.y * g.z); }
 
 }  // namespace f2d

@@ -95,6 +95,8 @@ class Ensemble:
     def _ready(self):
         if not all(getattr(m.integrator.model, "capturable", True) for m in self.models):
             return False  # a member's steps check on the host: eager only
+        if any(getattr(m.integrator, "forcings", None) for m in self.models):
+            return False  # host forcing code runs at every step: eager only
         return self._eager >= 2 and all(
             m.integrator._dtn_valid or not m.integrator._wants_dtn() for m in self.models)
 
