@@ -19,7 +19,7 @@ EXPORTS = [
     "f2d_abi_version", "f2d_last_error", "f2d_device_synchronize",
     "f2d_set_masks", "f2d_set_order", "f2d_slipcoef", "f2d_gindex",
     "f2d_compflux", "f2d_vortexforce", "f2d_innerproduct",
-    "f2d_transport_stage", "f2d_transport_stage_psi", "f2d_rk_addto", "f2d_addto", "f2d_c2v", "f2d_perpgrad",
+    "f2d_transport_stage", "f2d_transport_stage_log", "f2d_transport_stage_psi", "f2d_rk_addto", "f2d_addto", "f2d_c2v", "f2d_perpgrad",
     "f2d_fill_halo", "f2d_cfl_dt", "f2d_set_scalar", "f2d_copy_f64",
     "f2d_reserve_stream_scratch",
     "f2d_poisson_create", "f2d_poisson_destroy", "f2d_poisson_solve",
@@ -89,6 +89,8 @@ def lib():
         "f2d_innerproduct": (i32, [i32, vp, vp, vp, vp, i64, i64, i32, vp]),
         "f2d_transport_stage": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
                                       vp, vp, vp, i32, i32, vp, vp]),
+        "f2d_transport_stage_log": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
+                                          vp, vp, vp, i32, i32, vp, vp, vp]),
         "f2d_transport_stage_psi": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, d, d, i32, vp,
                                           vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "f2d_rk_addto": (i32, [i32, vp, vp, vp, vp, i64, i32, i32, vp, vp]),

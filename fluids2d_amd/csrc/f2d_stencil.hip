@@ -124,6 +124,7 @@ struct TransportArgs {
   int xper, yper;
   int tiles_x, ntiles;
   int nhb;  // halo-ring blocks (block indices 0 .. nhb-1)
+  double* dt_log;  // nullable: *dt copied there (the step's time step)
   int64_t nhalo;
   // U = perpgrad(psi) formed on the fly instead of read (psi non-NULL):
   // masks, contravariant factors and the periodic halo fill of k_perpgrad
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
   __shared__ T sfy[(TY + 1) * TX];
   const int t = threadIdx.x;
   const int P = a.P;
+  if (a.dt_log && blockIdx.x == 0 && t == 0) *a.dt_log = *a.dt;
 
   // the halo-ring blocks come first: they start with the tiles instead of
   // trailing them
@@ -386,6 +388,7 @@ __global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a) {
   __shared__ T sfx[TY * (TX + 1)];
   __shared__ T sfy[(TY + 1) * TX];
   const int t = threadIdx.x;
+  if (a.dt_log && blockIdx.x == 0 && t == 0) *a.dt_log = *a.dt;
   if ((int)blockIdx.x < a.nhb) {  // halo-ring blocks first, as in k_transport
     transport_halo<T, M>(a, (int64_t)blockIdx.x * 256 + t);
     return;
@@ -873,10 +876,12 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
                                 const int8_t* msky, double sx, double sy, int fill,
                                 const int8_t* ocx, const int8_t* ocy, const int8_t* msk,
                                 const void* ds_prev0, const void* ds_prev1, void* ds_out,
-                                int integrator, int stage, const double* dt, void* stream) {
+                                int integrator, int stage, const double* dt, void* stream,
+                                double* dt_log = nullptr) {
   return guarded([&] {
     check_grid(g);
     check_method(method);
+    F2D_REQUIRE(!dt_log || dt, "dt_log needs dt");
     if (psi) {
       F2D_REQUIRE(mskx && msky, "mskx and msky are required with psi");
     } else {
@@ -913,6 +918,7 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
       a.ds0 = (const T*)ds_prev0;
       a.ds1 = (const T*)ds_prev1;
       a.dso = (T*)ds_out;
+      a.dt_log = dt_log;
       a.dt = dt;
       a.integ = integrator;
       a.stage = stage;
@@ -952,6 +958,16 @@ int f2d_transport_stage_psi(int dtype, const f2d_grid* g, int method, const void
   return transport_stage_impl(dtype, g, method, q_in, q_out, nullptr, nullptr, psi, mskx,
                               msky, sx, sy, fill, ocx, ocy, msk, ds_prev0, ds_prev1, ds_out,
                               integrator, stage, dt, stream);
+}
+
+int f2d_transport_stage_log(int dtype, const f2d_grid* g, int method, const void* q_in,
+                            void* q_out, const void* Ux, const void* Uy, const int8_t* ocx,
+                            const int8_t* ocy, const int8_t* msk, const void* ds_prev0,
+                            const void* ds_prev1, void* ds_out, int integrator, int stage,
+                            const double* dt, double* dt_log, void* stream) {
+  return transport_stage_impl(dtype, g, method, q_in, q_out, Ux, Uy, nullptr, nullptr,
+                              nullptr, 0.0, 0.0, 0, ocx, ocy, msk, ds_prev0, ds_prev1,
+                              ds_out, integrator, stage, dt, stream, dt_log);
 }
 
 int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,

@@ -62,8 +62,7 @@ class Ensemble:
     def _member_step(self, i, dst):
         integ = self.models[i].integrator
         integ._select_dt()
-        _lib.call("f2d_copy_f64", _lib.ptr(dst), _lib.ptr(integ.d_dt), 1, _lib.stream_ptr())
-        integ.advance(stats=False)
+        integ.advance(stats=False, dt_log=dst)
 
     def _fork(self, body):
         main = torch.cuda.current_stream()

@@ -31,6 +31,7 @@ class _Base:
 
 class EulerPsi(_Base):
     """Euler in vorticity-streamfunction form (equations.py:269-282)."""
+    logs_dt = True  # stage(..., dt_log=) records the step's dt
     prognostic = ("omega",)
 
     def rhs(self, s, ds):
@@ -51,19 +52,20 @@ class EulerPsi(_Base):
         op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
                     cfl=p.cfl, dtmax=p.dtmax)
 
-    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None):
         """One RK stage: transport + update, then the diag.  U is stored
         only by the step's last stage (it feeds the CFL step and the state):
         a later stage forms U = perpgrad(psi) inside its transport kernel,
         bit for bit the same velocity, and the earlier stages skip the
-        perpgrad pass."""
+        perpgrad pass.  dt_log (stage 0): the transport kernel also copies
+        the step's dt there."""
         m, p = self.mesh, self.param
         if k > 0:
             op.transport_stage_psi(m, p.compflux, s.omega, alt.omega, s.psi, True, True,
                                    ds_prev, ds_out, p.integrator, k, dt)
         else:
             op.transport_stage(m, p.compflux, s.omega, alt.omega, s.U, ds_prev, ds_out,
-                               p.integrator, k, dt)
+                               p.integrator, k, dt, dt_log)
         s, alt = s._replace(omega=alt.omega), alt._replace(omega=s.omega)
         m.poisson_vertices.solve_c2v(s.omega, s.psi, stats=stats)
         if k == _lib.NSTAGES[p.integrator] - 1:
@@ -77,6 +79,7 @@ class EulerPsi(_Base):
 
 class Advection(_Base):
     """Tracer advection by a fixed velocity (equations.py:356-366)."""
+    logs_dt = True  # stage(..., dt_log=) records the step's dt
     prognostic = ("q",)
 
     def rhs(self, s, ds):
@@ -88,10 +91,10 @@ class Advection(_Base):
     def diag(self, s):
         pass
 
-    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None):
         p = self.param
         op.transport_stage(self.mesh, p.compflux, s.q, alt.q, s.U, ds_prev, ds_out,
-                           p.integrator, k, dt)
+                           p.integrator, k, dt, dt_log if k == 0 else None)
         s, alt = s._replace(q=alt.q), alt._replace(q=s.q)
         if dt_next is not None:
             op.cfl_dt(s.U, p.cfl, p.dtmax, dt_next)
@@ -100,6 +103,7 @@ class Advection(_Base):
 
 class QG(_Base):
     """Quasi-geostrophic PV transport (equations.py:285-299)."""
+    logs_dt = True  # stage(..., dt_log=) records the step's dt
     prognostic = ("pv",)
 
     def __init__(self, param, mesh):
@@ -120,7 +124,7 @@ class QG(_Base):
         m.qg_helmholtz.solve(s.work, s.psi)
         op.perpgrad(m, s.psi, s.U, contravariant=True)
 
-    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None):
         """As EulerPsi.stage: U stored by the last stage only, formed from psi
         inside the later stages' transport."""
         m, p = self.mesh, self.param
@@ -129,7 +133,7 @@ class QG(_Base):
                                    ds_prev, ds_out, p.integrator, k, dt)
         else:
             op.transport_stage(m, p.compflux, s.pv, alt.pv, s.U, ds_prev, ds_out,
-                               p.integrator, k, dt)
+                               p.integrator, k, dt, dt_log)
         s, alt = s._replace(pv=alt.pv), alt._replace(pv=s.pv)
         m.qg_helmholtz.solve_c2v(s.pv, s.psi, stats=stats)
         if k == _lib.NSTAGES[p.integrator] - 1:

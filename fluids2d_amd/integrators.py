@@ -177,22 +177,34 @@ class RKIntegrator:
         p = self.param
         return not (p.dt > 0 or p.model == "rsw")
 
-    def advance(self, stats=True):
-        """One RK step on the device with the dt already in d_dt."""
+    def _log_dt(self, dt_log):
+        """The step's dt into dt_log by a kernel of its own (models whose
+        first stage does not log it)."""
+        _lib.call("f2d_copy_f64", _lib.ptr(dt_log), _lib.ptr(self.d_dt), 1, _lib.stream_ptr())
+
+    def advance(self, stats=True, dt_log=None):
+        """One RK step on the device with the dt already in d_dt; dt_log (a
+        one-element float64 device tensor) receives that dt."""
         s, alt = self.dstate, self.dalt
         dtn = self.d_dtn if self._wants_dtn() else None
         if self.forcings:
+            if dt_log is not None:
+                self._log_dt(dt_log)
             for k in range(self.nstages):
                 self._forced_stage(s, k, dtn if k == self.nstages - 1 else None)
             self._dtn_valid = dtn is not None
             return
+        logs = getattr(self.model, "logs_dt", False)
+        if dt_log is not None and not logs:
+            self._log_dt(dt_log)
         for k in range(self.nstages):
             ds_out = self.dscratch[k] if k < self.nstages - 1 else None
             prev = [self.dscratch[j] for j in range(k)]
             last = k == self.nstages - 1
+            kw = {"dt_log": dt_log} if (k == 0 and logs and dt_log is not None) else {}
             s, alt = self.model.stage(s, alt, [_one(x) for x in prev],
                                       _one(ds_out) if ds_out is not None else None,
-                                      k, self.d_dt, dtn if last else None, stats)
+                                      k, self.d_dt, dtn if last else None, stats, **kw)
         self.dstate, self.dalt = s, alt
         self._dtn_valid = dtn is not None
 
@@ -241,8 +253,7 @@ class RKIntegrator:
                 n += G
                 continue
             self._select_dt()
-            dts[n:n + 1].copy_(self.d_dt)
-            self.advance(stats=stats)
+            self.advance(stats=stats, dt_log=dts[n:n + 1])
             self._eager_steps += 1
             n += 1
         return dts
@@ -286,9 +297,7 @@ class RKIntegrator:
         with _lib.capture(g):
             for i in range(G):
                 self._select_dt()
-                _lib.call("f2d_copy_f64", _lib.ptr(self._glog[i:i + 1]), _lib.ptr(self.d_dt), 1,
-                          _lib.stream_ptr())
-                self.advance(stats=False)
+                self.advance(stats=False, dt_log=self._glog[i:i + 1])
         if self._graph_key() != key:
             raise RuntimeError("graph capture did not return the state buffers to "
                                "their roles (odd number of buffer swaps)")
@@ -329,10 +338,12 @@ class LFRAIntegrator(RKIntegrator):
                   _lib.ptr(xs[1]), _lib.ptr(xs[2]), y.numel(), len(terms), float(cs[0]),
                   float(cs[1]), float(cs[2]), _lib.ptr(self.d_dt), dtmask, _lib.stream_ptr())
 
-    def advance(self, stats=True):
+    def advance(self, stats=True, dt_log=None):
         p = self.param
         s = self.dstate
         dtn = self.d_dtn if self._wants_dtn() else None
+        if dt_log is not None:
+            self._log_dt(dt_log)
         ds = self.dscratch[0]
         self.model.rhs(s, ds)
         self._force(s, ds)

@@ -41,14 +41,20 @@ def transport_rhs(mesh, method, q, U, dq):
 
 
 def transport_stage(mesh, method, q_in, q_out, U, ds_prev, ds_out, integrator, stage,
-                    dt_dev):
-    """Fused rhs + fill + RK addto of one stage (see f2d_transport_stage)."""
+                    dt_dev, dt_log=None):
+    """Fused rhs + fill + RK addto of one stage (see f2d_transport_stage);
+    with dt_log (a one-element float64 device tensor) the kernel also
+    copies dt there (f2d_transport_stage_log)."""
     d0 = ds_prev[0] if len(ds_prev) > 0 else None
     d1 = ds_prev[1] if len(ds_prev) > 1 else None
-    call("f2d_transport_stage", _dt(q_in), mesh.grid_ref(), _lib.METHODS[method],
-         ptr(q_in), ptr(q_out), ptr(U.x), ptr(U.y), ptr(mesh.d_oc.x), ptr(mesh.d_oc.y),
-         ptr(mesh.d_msk), ptr(d0), ptr(d1), ptr(ds_out), _lib.INTEGRATORS[integrator],
-         stage, ptr(dt_dev), stream_ptr())
+    args = (_dt(q_in), mesh.grid_ref(), _lib.METHODS[method],
+            ptr(q_in), ptr(q_out), ptr(U.x), ptr(U.y), ptr(mesh.d_oc.x), ptr(mesh.d_oc.y),
+            ptr(mesh.d_msk), ptr(d0), ptr(d1), ptr(ds_out), _lib.INTEGRATORS[integrator],
+            stage, ptr(dt_dev))
+    if dt_log is None:
+        call("f2d_transport_stage", *args, stream_ptr())
+    else:
+        call("f2d_transport_stage_log", *args, ptr(dt_log), stream_ptr())
 
 
 def transport_stage_psi(mesh, method, q_in, q_out, psi, contravariant, fill, ds_prev, ds_out,

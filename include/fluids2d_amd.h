@@ -124,6 +124,15 @@ int f2d_transport_stage(int dtype, const f2d_grid* g, int method,
                         const void* ds_prev1, void* ds_out, int integrator,
                         int stage, const double* dt, void* stream);
 
+/* f2d_transport_stage that also copies *dt to *dt_log (one double).
+ * This has no counterpart in the reference: it lets a step record its time
+ * step without a launch of its own (a captured step's dt log). */
+int f2d_transport_stage_log(int dtype, const f2d_grid* g, int method, const void* q_in,
+                            void* q_out, const void* Ux, const void* Uy, const int8_t* ocx,
+                            const int8_t* ocy, const int8_t* msk, const void* ds_prev0,
+                            const void* ds_prev1, void* ds_out, int integrator, int stage,
+                            const double* dt, double* dt_log, void* stream);
+
 /* The same stage with U = perpgrad(psi) formed on the fly rather than read:
  * perpgrad, operators.py:144-149 (Ux = -(psi[j+1] - psi[j]) mskx sx,
  * Uy = (psi[i+1] - psi[i]) msky sy, with fill != 0 the periodic halo fill
