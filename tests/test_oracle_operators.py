@@ -232,6 +232,29 @@ def euler_dipole(O, p):
     return m, omega
 
 
+def test_euler_kat_as_written_takes_one_step():
+    """tests/test_models.py:8-15 literally: no diag before run, so the first
+    CFL step (model.py:84-87) sees U = 0 and dt = min(0.9 / 1e-99, dtmax) =
+    9e98; that one step ends the run.  The current source cannot give the
+    test's 25 iterations (an older version's count)."""
+    p = P(model="euler", tend=10)
+    m = O.Model(p)
+    mesh = m.mesh
+    x, y = mesh.xy("v")
+
+    def g(x0, y0, r):
+        return np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * r ** 2))
+    omega = (g(0.55, 0.5, 0.05) - g(0.45, 0.5, 0.05)) * mesh.mskv * mesh.area
+    psi = np.zeros(mesh.shape)
+    mesh.poisson_vertices.solve(omega, psi)
+    O.perpgrad(mesh, psi, m.state.u)
+    m.state.omega[...] = omega
+    assert not m.state.U.x.any() and not m.state.U.y.any()
+    with np.errstate(all="ignore"):
+        m.run()
+    assert m.time.ite == 1 and m.time.dt == 0.9 / 1e-99
+
+
 def test_euler_dipole_iterations():
     """The case of the reference's KAT test_euler (tests/test_models.py:8-15:
     default Param, euler, the vortex.py dipole, tend = 10).  That test
