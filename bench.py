@@ -390,7 +390,8 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--accuracy-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=16,
+                    help="RK3 steps of the CPU baseline sample (about 10 s at 1024^2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--no-adjoint", action="store_true",
