@@ -61,6 +61,8 @@ class PoissonOpts(ctypes.Structure):
 
 SOLVERS = {"auto": 0, "mg": 1, "fft": 2}
 
+ABI_VERSION = 5  # f2d_abi_version() of the library these bindings describe
+
 
 _lib = None
 
@@ -135,6 +137,9 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
+    if L.f2d_abi_version() != ABI_VERSION:
+        raise F2DError(f"{LIBPATH} has ABI version {L.f2d_abi_version()}, this package "
+                       f"binds version {ABI_VERSION}: rebuild it")
     _lib = L
     return L
 
