@@ -62,6 +62,12 @@ struct FftArgs {
   double cx, cy, d;
   int singular;
   int pad;
+  // Closed directions (ex, ey): the problem's onx x ony interior is solved
+  // as the odd extension on the periodic doubled grid (nx = 2 onx when ex,
+  // ny = 2 ony when ey); onx / ony are the problem's sizes, NXP its padded
+  // row length.
+  int ex, ey;
+  int onx, ony;
 };
 
 // Forward DFTs of `nf` contiguous length-N sequences held in LDS buffer `a`
@@ -170,6 +176,79 @@ __device__ __forceinline__ double rval32(const FftArgs& a, double g,
   return (g - mb) - Ax;
 }
 
+// Closed directions.  The Dirichlet problem on a closed direction of
+// length on (unknowns at points 1 .. on-1, zero at 0 and on) is the
+// restriction of the periodic problem on the doubled length 2 on to the
+// odd extension of its data: doubled point n is the problem's point n for
+// n < on, the point 2 on - n with the opposite sign above on, and 0 at 0
+// and on (the 5-point operator commutes with the reflection, and the
+// extension vanishes on the boundary lines).  ext_of returns that sign (0
+// on a boundary line) and the problem's point.
+__device__ __forceinline__ int ext_of(const FftArgs& a, int y, int n, int& yo, int& xo) {
+  int sg = 1;
+  yo = y;
+  xo = n;
+  if (a.ey) {
+    if (y == 0 || y == a.ony) return 0;
+    if (y > a.ony) {
+      yo = 2 * a.ony - y;
+      sg = -sg;
+    }
+  }
+  if (a.ex) {
+    if (n == 0 || n == a.onx) return 0;
+    if (n > a.onx) {
+      xo = 2 * a.onx - n;
+      sg = -sg;
+    }
+  }
+  return sg;
+}
+
+// x at the problem's point (yo, xo) of an array of the padded layout, as
+// the stencil of an unknown reads it: 0 on the boundary lines of a closed
+// direction, wrapped in a periodic one (the halo is not read)
+template <class TX>
+__device__ __forceinline__ double xat(const FftArgs& a, const TX* __restrict__ x, int yo,
+                                      int xo) {
+  if (a.ex) {
+    if (xo <= 0 || xo >= a.onx) return 0.0;
+  } else {
+    xo = xo < 0 ? xo + a.onx : (xo >= a.onx ? xo - a.onx : xo);
+  }
+  if (a.ey) {
+    if (yo <= 0 || yo >= a.ony) return 0.0;
+  } else {
+    yo = yo < 0 ? yo + a.ony : (yo >= a.ony ? yo - a.ony : yo);
+  }
+  return (double)x[(int64_t)(yo + 3) * a.NXP + xo + 3];
+}
+
+// A x at the problem's unknown (yo, xo), x read through xat
+template <class TX>
+__device__ __forceinline__ double ax_ext(const FftArgs& a, const TX* __restrict__ x, int yo,
+                                         int xo) {
+  return a.cx * (xat(a, x, yo, xo - 1) + xat(a, x, yo, xo + 1)) +
+         a.cy * (xat(a, x, yo - 1, xo) + xat(a, x, yo + 1, xo)) - a.d * xat(a, x, yo, xo);
+}
+
+// value v at the problem's unknown (y, n) and its periodic halo image
+// (closed directions have no image; at most one direction is periodic)
+template <class TX>
+__device__ __forceinline__ void put_ext(TX* x, int y, int n, const FftArgs& a, TX v) {
+  const int64_t P = a.NXP;
+  const int64_t k = (int64_t)(y + 3) * P + n + 3;
+  x[k] = v;
+  if (!a.ex) {
+    const int64_t d = n < 3 ? a.onx : (n >= a.onx - 3 ? -a.onx : 0);
+    if (d) x[k + d] = v;
+  }
+  if (!a.ey) {
+    const int64_t d = y < 3 ? (int64_t)a.ony * P : (y >= a.ony - 3 ? -(int64_t)a.ony * P : 0);
+    if (d) x[k + d] = v;
+  }
+}
+
 // Rows forward.  Without x64/x32: the rows of b (or of c2v(b)); with bpart
 // non-NULL each workgroup also writes the FP64 sum of its values of b (the
 // mean of b for the singular operator).  With x64 (or x32): the rows of
@@ -205,7 +284,28 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
     const int p = w >> lg, n = w & (nx - 1);
     const int64_t ka = (int64_t)(y0 + 2 * p + 3) * a.NXP + n + 3;
     double va, vb;
-    if (x32) {
+    if (a.ex | a.ey) {
+      // odd extension of a closed direction: the value (or residual) at the
+      // problem's point, with the reflection's sign (no mean: not singular)
+      double v2[2];
+      for (int h = 0; h < 2; h++) {
+        int yo, xo;
+        const int sg = ext_of(a, y0 + 2 * p + h, n, yo, xo);
+        double v = 0.0;
+        if (sg) {
+          const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
+          v = gval(b, k, a.NXP, gmode);
+          if (x32)
+            v = v - ax_ext<float>(a, x32, yo, xo);
+          else if (x64)
+            v = v - ax_ext<double>(a, x64, yo, xo);
+          if (sg < 0) v = -v;
+        }
+        v2[h] = v;
+      }
+      va = v2[0];
+      vb = v2[1];
+    } else if (x32) {
       const double ga = gval(b, ka, a.NXP, gmode), gb = gval(b, ka + a.NXP, a.NXP, gmode);
       sum += ga + gb;
       va = rval32(a, ga, x32, y0 + 2 * p, n, mb);
@@ -341,6 +441,28 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
     const int ya = y0 + 2 * p;
     const cplx<T> v = r[w];
     const double va = (double)v.x, vb = -(double)v.y;
+    if (a.ex | a.ey) {
+      // closed direction: only the problem's unknowns are written
+      for (int h = 0; h < 2; h++) {
+        const int y = ya + h;
+        if ((a.ey && (y == 0 || y >= a.ony)) || (a.ex && (n == 0 || n >= a.onx))) continue;
+        const double val = h ? vb : va;
+        const int64_t k = (int64_t)(y + 3) * a.NXP + n + 3;
+        if (MODE == 3) {
+          const double nv = (double)x[k] + val;
+          if (x64) put_ext<double>(x64, y, n, a, nv);
+          put_ext<TX>(x, y, n, a, TX(nv));
+        } else if (MODE == 2) {
+          const double nv = x64[k] + val;
+          put_ext<double>(x64, y, n, a, nv);
+          put_ext<TX>(x, y, n, a, TX(nv));
+        } else {
+          put_ext<TX>(x, y, n, a, TX(val));
+          if (MODE == 1) put_ext<double>(x64, y, n, a, val);
+        }
+      }
+      continue;
+    }
     if (MODE == 3) {
       const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
       const double na = (double)x[ka] + va, nb = (double)x[ka + a.NXP] + vb;
@@ -413,15 +535,21 @@ __global__ __launch_bounds__(256) void k_fft_res(FftArgs a, const TB* __restrict
     const int y = (int)(idx >> a.lgx), q = (int)(idx & (a.nx - 1));
     const int64_t P = a.NXP;
     const int64_t k = (int64_t)(y + 3) * P + q + 3;
-    const double bv = gval(b, k, a.NXP, gmode) - m0;
-    const double Ax = a.cx * (x[k - 1] + x[k + 1]) + a.cy * (x[k - P] + x[k + P]) -
-                      a.d * x[k];
-    const double rv = bv - Ax;
-    if (r) r[k] = rv;
-    rr = rv * rv;
-    rs = rv;
-    bb = bv * bv;
-    bs = bv;
+    // closed directions: the problem's unknowns only (the rest of the
+    // doubled grid is the extension)
+    const bool unk = !((a.ey && (y == 0 || y >= a.ony)) || (a.ex && (q == 0 || q >= a.onx)));
+    if (unk) {
+      const double bv = gval(b, k, a.NXP, gmode) - m0;
+      const double Ax = (a.ex | a.ey) ? ax_ext<double>(a, x, y, q)
+                                      : a.cx * (x[k - 1] + x[k + 1]) +
+                                            a.cy * (x[k - P] + x[k + P]) - a.d * x[k];
+      const double rv = bv - Ax;
+      if (r) r[k] = rv;
+      rr = rv * rv;
+      rs = rv;
+      bb = bv * bv;
+      bs = bv;
+    }
   }
   rr = bsum256(rr, sh);
   rs = bsum256(rs, sh);
@@ -514,8 +642,9 @@ static int ilog2(int64_t v) {
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
 bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet) {
-  return xper && yper && pow2(nx) && pow2(ny) && nx >= 8 && ny >= 8 &&
-         nx <= 4096 && ny <= 4096 && nwet == nx * ny;
+  const int ex = xper ? 0 : 1, ey = yper ? 0 : 1;
+  return pow2(nx) && pow2(ny) && nx >= 8 && ny >= 8 && (nx << ex) <= 4096 &&
+         (ny << ey) <= 4096 && nwet == (nx - ex) * (ny - ey);
 }
 
 template <class P>
@@ -589,20 +718,29 @@ static void set_lds(size_t lr, size_t lc) {
 }
 
 FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
-                      double maindiag, bool f64, bool f32) {
+                      double maindiag, bool f64, bool f32, int xper, int yper) {
   FftSolver* f = new FftSolver();
   try {
     FftArgs& a = f->a;
+    a.ex = xper ? 0 : 1;
+    a.ey = yper ? 0 : 1;
+    a.onx = (int)nx;
+    a.ony = (int)ny;
+    const int64_t onx = nx, ony = ny;
+    nx = onx << a.ex;  // the transformed (doubled when closed) sizes
+    ny = ony << a.ey;
     a.nx = (int)nx;
     a.ny = (int)ny;
     a.lgx = ilog2(nx);
     a.lgy = ilog2(ny);
     a.nk = (int)(nx / 2 + 1);
-    a.NXP = (int)(nx + 6);
+    a.NXP = (int)(onx + 6);
     a.cx = dy / dx;
     a.cy = dx / dy;
     a.d = 2.0 * (a.cx + a.cy) + maindiag;
-    a.singular = maindiag == 0.0 ? 1 : 0;
+    // a closed direction makes the problem non-singular (its odd extension
+    // has no mean)
+    a.singular = (maindiag == 0.0 && !a.ex && !a.ey) ? 1 : 0;
     const long double pi = 3.14159265358979323846264338327950288L;
     std::vector<double> lx(a.nk), ly(ny);
     for (int k = 0; k < a.nk; k++)
@@ -614,7 +752,7 @@ FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
     F2D_HIP(hipMemcpy(f->lamx, lx.data(), sizeof(double) * a.nk, hipMemcpyHostToDevice));
     F2D_HIP(hipMemcpy(f->lamy, ly.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
     const size_t nspec = (size_t)a.nk * ny;
-    const size_t npad = (size_t)(ny + 6) * (nx + 6);
+    const size_t npad = (size_t)(ony + 6) * (onx + 6);
     f->nbres = (int)((nx * ny + 255) / 256);
     f->part = falloc<double>(f, sizeof(double) * 4 * f->nbres);
     f->part_b = falloc<double>(f, sizeof(double) * f->nbres);
@@ -768,7 +906,7 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
               "FFT solver has no buffers of this precision");
   F2D_REQUIRE(f64 ? (kind == 0 || kind == 2 || kind == 3) : (kind >= 0 && kind <= 6),
               "FFT bench kind");
-  const size_t npad = (size_t)(f->a.ny + 6) * (f->a.nx + 6);
+  const size_t npad = (size_t)(f->a.ony + 6) * (f->a.onx + 6);
   if (!f->bench_buf) f->bench_buf = falloc<double>(f, sizeof(double) * npad);
   FftArgs a = f->a;
   a.P = f64 ? f->P64 : f->P32;

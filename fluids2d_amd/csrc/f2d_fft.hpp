@@ -16,12 +16,14 @@ namespace f2d {
 
 struct FftSolver;
 
-// nx, ny powers of two in [8, 4096], all interior vertices unknown, doubly
-// periodic
+// nx, ny powers of two >= 8, every vertex strictly inside the domain
+// unknown: all nx x ny of a doubly periodic one; in a closed direction the
+// boundary lines are Dirichlet (nx - 1 unknowns) and the problem is solved
+// as its odd extension on the periodic doubled grid (2 nx <= 4096 then)
 bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet);
 
 FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
-                      double maindiag, bool f64, bool f32);
+                      double maindiag, bool f64, bool f32, int xper, int yper);
 void fft_destroy(FftSolver* f);
 
 // x = A^-1 b on the padded (ny+6) x (nx+6) layout, halo filled.  gmode 0:

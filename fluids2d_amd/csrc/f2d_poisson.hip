@@ -2311,13 +2311,13 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
     }
     const bool fft_ok = fft_applicable(h->ny, h->nx, h->xper, h->yper, nwet);
     F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT || fft_ok,
-                "FFT solver requested but the operator is not fully wet, "
-                "doubly periodic with power-of-two sizes <= 4096");
+                "FFT solver requested but the operator is not fully wet with "
+                "power-of-two sizes (<= 4096, <= 2048 in a closed direction)");
     F2D_REQUIRE(h->opts.solver >= 0 && h->opts.solver <= 2, "opts.solver unknown");
     if (fft_ok && h->opts.solver != F2D_SOLVER_MG)
       h->fft = fft_create(h->ny, h->nx, g->dx, g->dy, maindiag,
                           h->opts.mode == F2D_POISSON_F64,
-                          h->opts.mode != F2D_POISSON_F64);
+                          h->opts.mode != F2D_POISSON_F64, h->xper, h->yper);
     F2D_HIP(hipDeviceSynchronize());
     *out = h;
   });

@@ -342,6 +342,37 @@ def test_poisson_fft_f64_vs_superlu(dev, L, ny, nx, Ly, maindiag, c2v):
     assert np.array_equal(xx, x)
 
 
+@pytest.mark.parametrize("ny,nx,xper,yper,maindiag,mode,c2v", [
+    (64, 64, False, False, 0.0, "f64", False), (128, 64, False, False, 0.7, "f64", True),
+    (64, 256, True, False, 0.0, "f64", False), (256, 32, False, True, 0.3, "f64", False),
+    (512, 512, False, False, 0.0, "f64", True), (256, 256, False, False, 0.0, "refine", False),
+    (128, 512, True, False, 0.0, "refine", True)])
+def test_poisson_fft_closed_vs_superlu(dev, L, ny, nx, xper, yper, maindiag, mode, c2v):
+    """Closed directions in the direct solve (odd extension on the doubled
+    periodic grid) against SuperLU on the same masked operator: closed
+    rectangles, channels, Helmholtz, the c2v right-hand side, FP64 and FP32
+    with refinement.  Entries other than the unknowns are left alone, and a
+    periodic direction gets its halo images."""
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper,
+          Lx=nx / 64.0, Ly=ny / 80.0)
+    m = O.default_mask(ny, nx, 3, xper, yper)
+    md = L.POISSON_F64 if mode == "f64" else L.POISSON_F32_REFINE
+    err, it, rr, x, xr, unk = _poisson_case(dev, L, p, m, maindiag, md, solver="fft",
+                                            c2v=c2v)
+    print(f"\nfft closed {ny}x{nx} xper={xper} yper={yper} md={maindiag} {mode} c2v={c2v}: "
+          f"relres={rr:.2e} err={err:.2e}")
+    if mode == "f64":
+        assert rr <= 1e-12 and err <= 1e-11
+    else:
+        assert rr <= 1e-9 and err < 1e-6
+    inner = np.zeros(unk.shape, dtype=bool)
+    inner[3:-3, 3:-3] = True
+    assert np.all(x[inner & (unk == 0)] == 0)
+    xx = x.copy()
+    O.fill(p, xx)
+    assert np.array_equal(xx, x)
+
+
 @pytest.mark.parametrize("mode,ny,nx", [("refine", 512, 512), ("refine", 64, 128),
                                         ("f32", 256, 256)])
 def test_poisson_fft_f32_modes(dev, L, mode, ny, nx):
@@ -357,10 +388,10 @@ def test_poisson_fft_f32_modes(dev, L, mode, ny, nx):
         assert err < 1e-5
 
 
-@pytest.mark.parametrize("ny,nx,maindiag,nref,bmean", [
-    (256, 256, 0.0, 1, 0.0), (128, 512, 0.0, 1, 3.0), (256, 128, 0.4, 1, 0.0),
-    (512, 512, 0.0, 2, 0.5)])
-def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean):
+@pytest.mark.parametrize("ny,nx,maindiag,nref,bmean,per", [
+    (256, 256, 0.0, 1, 0.0, True), (128, 512, 0.0, 1, 3.0, True), (256, 128, 0.4, 1, 0.0, True),
+    (512, 512, 0.0, 2, 0.5, True), (256, 256, 0.0, 1, 0.0, False), (128, 256, 0.2, 2, 1.0, False)])
+def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean, per):
     """Warm-started FP32 refinement (opts.fft_warm): from the handle's
     second solve on, the first refinement starts from the x passed in.
     Solve b1 (cold), then b2 = b1 + 1% noise from the first solution (halo
@@ -368,9 +399,9 @@ def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean):
     each against SuperLU on the same b; b may carry a mean, which the
     singular operator drops."""
     import torch
-    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True,
+    p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=per, yperiodic=per,
           Lx=nx / 256.0, Ly=ny / 256.0)
-    m = O.default_mask(ny, nx, 3, True, True)
+    m = O.default_mask(ny, nx, 3, per, per)
     mesh = O.Mesh(p, msk=m)
     unk = O.unknown_mask(p, mesh.mskv)
     rng = np.random.default_rng(9)
@@ -388,7 +419,7 @@ def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean):
     b3 = (rng.standard_normal(m.shape) + bmean) * mesh.mskv
     errs = []
     for k, b in enumerate((b1, b2, b3)):
-        if k == 1:
+        if k == 1 and per:
             dx[:3, :] = 1e6
             dx[:, -3:] = -1e6
         x_ref = np.zeros(m.shape)
@@ -398,15 +429,15 @@ def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean):
                ctypes.byref(it), ctypes.byref(rr), L.stream_ptr())
         x = N(dx).astype(np.float64)
         xi, xr = x[3:-3, 3:-3], x_ref[3:-3, 3:-3]
-        if maindiag == 0.0:  # the singular operator's solution is up to a constant
+        if maindiag == 0.0 and per:  # the singular operator's solution is up to a constant
             xi, xr = xi - xi.mean(), xr - xr.mean()
         err = np.linalg.norm(xi - xr) / np.linalg.norm(xr)
         errs.append((err, rr.value))
         print(f"\nwarm {ny}x{nx} md={maindiag} nref={nref} solve {k}: relres={rr.value:.2e} "
               f"err={err:.2e}")
-        # halo images of the periodic solution
-        assert np.array_equal(x[:3, 3:-3], x[-6:-3, 3:-3])
-        assert np.array_equal(x[3:-3, :3], x[3:-3, -6:-3])
+        if per:  # halo images of the periodic solution
+            assert np.array_equal(x[:3, 3:-3], x[-6:-3, 3:-3])
+            assert np.array_equal(x[3:-3, :3], x[3:-3, -6:-3])
     L.call("f2d_poisson_destroy", h)
     # k=0: cold FP32 solve + nref FP64-residual refinements; k=1: from the
     # nearby solution the refinement's FP32 error is 1% of a cold solve's;

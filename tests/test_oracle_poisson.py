@@ -112,3 +112,43 @@ def test_singular_doubly_periodic_matches_dft():
     want = O.fft_poisson_periodic(b[3:-3, 3:-3], mesh.dx, mesh.dy)
     assert np.allclose(x[3:-3, 3:-3], want, rtol=0, atol=1e-10 * np.abs(want).max())
     assert abs(x[3:-3, 3:-3].mean()) < 1e-12
+
+
+def _odd_extension_solve(b, dx, dy, maindiag, xper, yper):
+    """The Dirichlet problem of a closed direction as the periodic problem
+    on the doubled grid for the odd extension of b (the direct solver's
+    method), by numpy's FFT.  b: (ny, nx) vertex values, index 0 (and the
+    closed directions' boundary lines) not unknowns."""
+    ny, nx = b.shape
+    e = b.copy()
+    if not xper:
+        e[:, 0] = 0
+        e = np.concatenate([e, np.zeros((ny, 1)), -e[:, :0:-1]], axis=1)
+    if not yper:
+        e[0, :] = 0
+        e = np.concatenate([e, np.zeros((1, e.shape[1])), -e[:0:-1, :]], axis=0)
+    x = O.fft_poisson_periodic(e, dx, dy, maindiag)
+    return x[:ny, :nx]
+
+
+@pytest.mark.parametrize("xper,yper,maindiag", [(False, False, 0.0), (False, False, 0.7),
+                                                (True, False, 0.0), (False, True, 0.3)])
+def test_closed_directions_by_odd_extension(xper, yper, maindiag):
+    """SuperLU on the masked operator of an all-wet closed (or channel)
+    domain equals the periodic solve of the odd extension on the doubled
+    grid: the reduction the direct GPU solver uses for closed domains."""
+    nx, ny = 16, 8
+    p = P(nx=nx, ny=ny, Lx=1.0, Ly=0.6, xperiodic=xper, yperiodic=yper)
+    mesh = O.Mesh(p)
+    ps = O.Poisson2D(p, mesh, "v", maindiag=maindiag)
+    unk = O.unknown_mask(p, mesh.mskv)
+    assert unk.sum() == (nx - (not xper)) * (ny - (not yper))
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(mesh.shape) * unk
+    x = np.zeros(mesh.shape)
+    ps.solve(b, x)
+    want = _odd_extension_solve(b[3:-3, 3:-3], mesh.dx, mesh.dy, maindiag, xper, yper)
+    got = x[3:-3, 3:-3]
+    if maindiag == 0.0 and xper and yper:
+        got = got - got.mean()
+    assert np.allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
