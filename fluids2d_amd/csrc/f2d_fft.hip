@@ -341,7 +341,16 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
       const cplx<T> u = csub<T>(C, D);
       X = mk<T>(T(0.5) * u.y, T(-0.5) * u.x);
     }
-    S[(int64_t)k * a.ny + y0 + 2 * p + ab] = X;
+    const int y = y0 + 2 * p + ab;
+    S[(int64_t)k * a.ny + y] = X;
+    if (a.ey) {
+      // closed y: the workgroups cover rows 0 .. ony-1; row 2 ony - y is the
+      // reflection of row y with the opposite sign, row ony a boundary line
+      if (y > 0)
+        S[(int64_t)k * a.ny + 2 * a.ony - y] = mk<T>(-X.x, -X.y);
+      else
+        S[(int64_t)k * a.ny + a.ony] = mk<T>(T(0), T(0));
+    }
   }
 }
 
@@ -691,7 +700,10 @@ template <class T>
 static void plan_blocks(const FftArgs& a, int& P, int& Q, size_t& lr, size_t& lc) {
   const size_t c = sizeof(T) * 2;
   P = a.nx >= 1024 ? 1 : 1024 / a.nx;
-  while (P > 1 && ((a.ny / 2) % P || (2 * P + 1) * a.nx * c > LDS_CAP)) P >>= 1;
+  // (closed y: the row passes cover the first half of the doubled rows)
+  while (P > 1 && ((a.ny / 2) % P || (a.ey && (a.ny / 4) % P) ||
+                   (2 * P + 1) * a.nx * c > LDS_CAP))
+    P >>= 1;
   Q = a.ny >= 1024 ? 1 : 1024 / a.ny;
   while (Q > 1 && (2 * Q + 1) * a.ny * c > LDS_CAP) Q >>= 1;
   lr = (2 * (size_t)P + 1) * a.nx * c;
@@ -810,7 +822,8 @@ static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
   cplx<T>* S = d ? (cplx<T>*)f->S64 : (cplx<T>*)f->S32;
   const cplx<T>* twx = d ? (const cplx<T>*)f->twx64 : (const cplx<T>*)f->twx32;
   const cplx<T>* twy = d ? (const cplx<T>*)f->twy64 : (const cplx<T>*)f->twy32;
-  const int gr = a.ny / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  // closed y: rows 0 .. ony-1 only (the rest mirror them, see k_fft_rows_fwd)
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
   const double* bpin = ((xres || xres32) && a.singular) ? rs_in : nullptr;
   k_fft_rows_fwd<T, TB><<<gr, 256, lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
                                             xres32);
@@ -912,7 +925,8 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
   a.P = f64 ? f->P64 : f->P32;
   a.Q = f64 ? f->Q64 : f->Q32;
   const size_t lr = f64 ? f->lds_r64 : f->lds_r32, lc = f64 ? f->lds_c64 : f->lds_c32;
-  const int gr = a.ny / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  // closed y: rows 0 .. ony-1 only (the rest mirror them, see k_fft_rows_fwd)
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
   const double N = (double)a.nx * a.ny, NS = (double)a.nk * a.ny;
   const double w = f64 ? 8.0 : 4.0;  // real word
   hipEvent_t e0, e1;
