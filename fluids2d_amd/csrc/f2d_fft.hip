@@ -62,10 +62,11 @@ struct FftArgs {
   double cx, cy, d;
   int singular;
   int pad;
-  // Closed directions (ex, ey): the problem's onx x ony interior is solved
-  // as the odd extension on the periodic doubled grid (nx = 2 onx when ex,
-  // ny = 2 ony when ey); onx / ony are the problem's sizes, NXP its padded
-  // row length.
+  // Closed directions (ex, ey nonzero): the problem's onx x ony interior is
+  // solved as its extension on the periodic doubled grid (nx = 2 onx when
+  // ex, ny = 2 ony when ey): 1 odd (vertex unknowns, Dirichlet boundary
+  // lines), 2 even (cell unknowns, Neumann walls); onx / ony are the
+  // problem's sizes, NXP its padded row length.
   int ex, ey;
   int onx, ony;
 };
@@ -176,48 +177,69 @@ __device__ __forceinline__ double rval32(const FftArgs& a, double g,
   return (g - mb) - Ax;
 }
 
-// Closed directions.  The Dirichlet problem on a closed direction of
-// length on (unknowns at points 1 .. on-1, zero at 0 and on) is the
-// restriction of the periodic problem on the doubled length 2 on to the
-// odd extension of its data: doubled point n is the problem's point n for
-// n < on, the point 2 on - n with the opposite sign above on, and 0 at 0
-// and on (the 5-point operator commutes with the reflection, and the
-// extension vanishes on the boundary lines).  ext_of returns that sign (0
-// on a boundary line) and the problem's point.
+// Closed directions.  Odd (1): the Dirichlet problem on a closed direction
+// of length on (vertex unknowns at points 1 .. on-1, zero at 0 and on) is
+// the restriction of the periodic problem on the doubled length 2 on to
+// the odd extension of its data: doubled point n is the problem's point n
+// for n < on, the point 2 on - n with the opposite sign above on, and 0 at
+// 0 and on.  Even (2): the Neumann problem on on cells (elliptic.py:186-190:
+// a wall drops its neighbour from the row and from the diagonal) is the
+// restriction of the periodic problem to the even extension about the
+// walls: doubled point n is the problem's cell n for n < on and cell
+// 2 on - 1 - n above, with the same sign.  The 5-point operator commutes
+// with both reflections.  ext_of returns the sign (0 on a boundary line)
+// and the problem's point.
 __device__ __forceinline__ int ext_of(const FftArgs& a, int y, int n, int& yo, int& xo) {
   int sg = 1;
   yo = y;
   xo = n;
-  if (a.ey) {
+  if (a.ey == 1) {
     if (y == 0 || y == a.ony) return 0;
     if (y > a.ony) {
       yo = 2 * a.ony - y;
       sg = -sg;
     }
+  } else if (a.ey == 2 && y >= a.ony) {
+    yo = 2 * a.ony - 1 - y;
   }
-  if (a.ex) {
+  if (a.ex == 1) {
     if (n == 0 || n == a.onx) return 0;
     if (n > a.onx) {
       xo = 2 * a.onx - n;
       sg = -sg;
     }
+  } else if (a.ex == 2 && n >= a.onx) {
+    xo = 2 * a.onx - 1 - n;
   }
   return sg;
 }
 
+// the problem's unknowns among the doubled grid's points: the first
+// quadrant, less the boundary lines of an odd direction
+__device__ __forceinline__ bool ext_unknown(const FftArgs& a, int y, int n) {
+  if (a.ey && (y >= a.ony || (a.ey == 1 && y == 0))) return false;
+  if (a.ex && (n >= a.onx || (a.ex == 1 && n == 0))) return false;
+  return true;
+}
+
 // x at the problem's point (yo, xo) of an array of the padded layout, as
-// the stencil of an unknown reads it: 0 on the boundary lines of a closed
-// direction, wrapped in a periodic one (the halo is not read)
+// the stencil of an unknown reads it: 0 on the boundary lines of an odd
+// direction, the mirror cell across the wall of an even one, wrapped in a
+// periodic one (the halo is not read)
 template <class TX>
 __device__ __forceinline__ double xat(const FftArgs& a, const TX* __restrict__ x, int yo,
                                       int xo) {
-  if (a.ex) {
+  if (a.ex == 1) {
     if (xo <= 0 || xo >= a.onx) return 0.0;
+  } else if (a.ex == 2) {
+    xo = xo < 0 ? -1 - xo : (xo >= a.onx ? 2 * a.onx - 1 - xo : xo);
   } else {
     xo = xo < 0 ? xo + a.onx : (xo >= a.onx ? xo - a.onx : xo);
   }
-  if (a.ey) {
+  if (a.ey == 1) {
     if (yo <= 0 || yo >= a.ony) return 0.0;
+  } else if (a.ey == 2) {
+    yo = yo < 0 ? -1 - yo : (yo >= a.ony ? 2 * a.ony - 1 - yo : yo);
   } else {
     yo = yo < 0 ? yo + a.ony : (yo >= a.ony ? yo - a.ony : yo);
   }
@@ -277,7 +299,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
   if ((x64 || x32) && bpin) {
     double acc = 0.0;
     for (int i = threadIdx.x; i < nbp; i += blockDim.x) acc += bpin[i];
-    mb = bsum256(acc, sh) / ((double)a.nx * (double)a.ny);
+    mb = bsum256(acc, sh) / ((double)a.onx * (double)a.ony);
   }
   double sum = 0.0;
   for (int w = threadIdx.x; w < P * nx; w += blockDim.x) {
@@ -285,8 +307,8 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
     const int64_t ka = (int64_t)(y0 + 2 * p + 3) * a.NXP + n + 3;
     double va, vb;
     if (a.ex | a.ey) {
-      // odd extension of a closed direction: the value (or residual) at the
-      // problem's point, with the reflection's sign (no mean: not singular)
+      // extension of a closed direction: the value (or residual) at the
+      // problem's point, with the reflection's sign
       double v2[2];
       for (int h = 0; h < 2; h++) {
         int yo, xo;
@@ -294,11 +316,14 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
         double v = 0.0;
         if (sg) {
           const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
-          v = gval(b, k, a.NXP, gmode);
+          const double g = gval(b, k, a.NXP, gmode);
+          // sums of b over the problem's points (the singular even case)
+          if (ext_unknown(a, y0 + 2 * p + h, n)) sum += g;
+          v = g;
           if (x32)
-            v = v - ax_ext<float>(a, x32, yo, xo);
+            v = (g - mb) - ax_ext<float>(a, x32, yo, xo);
           else if (x64)
-            v = v - ax_ext<double>(a, x64, yo, xo);
+            v = (g - mb) - ax_ext<double>(a, x64, yo, xo);
           if (sg < 0) v = -v;
         }
         v2[h] = v;
@@ -343,13 +368,15 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
     }
     const int y = y0 + 2 * p + ab;
     S[(int64_t)k * a.ny + y] = X;
-    if (a.ey) {
+    if (a.ey == 1) {
       // closed y: the workgroups cover rows 0 .. ony-1; row 2 ony - y is the
       // reflection of row y with the opposite sign, row ony a boundary line
       if (y > 0)
         S[(int64_t)k * a.ny + 2 * a.ony - y] = mk<T>(-X.x, -X.y);
       else
         S[(int64_t)k * a.ny + a.ony] = mk<T>(T(0), T(0));
+    } else if (a.ey == 2) {
+      S[(int64_t)k * a.ny + 2 * a.ony - 1 - y] = X;  // the even reflection
     }
   }
 }
@@ -454,7 +481,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
       // closed direction: only the problem's unknowns are written
       for (int h = 0; h < 2; h++) {
         const int y = ya + h;
-        if ((a.ey && (y == 0 || y >= a.ony)) || (a.ex && (n == 0 || n >= a.onx))) continue;
+        if (!ext_unknown(a, y, n)) continue;
         const double val = h ? vb : va;
         const int64_t k = (int64_t)(y + 3) * a.NXP + n + 3;
         if (MODE == 3) {
@@ -509,7 +536,7 @@ __global__ __launch_bounds__(256) void k_fft_bsum(FftArgs a, const TB* __restric
   double v = 0.0;
   if (idx < (int64_t)a.nx * a.ny) {
     const int y = (int)(idx >> a.lgx), q = (int)(idx & (a.nx - 1));
-    v = gval(b, (int64_t)(y + 3) * a.NXP + q + 3, a.NXP, gmode);
+    if (ext_unknown(a, y, q)) v = gval(b, (int64_t)(y + 3) * a.NXP + q + 3, a.NXP, gmode);
   }
   v = bsum256(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = v;
@@ -546,7 +573,7 @@ __global__ __launch_bounds__(256) void k_fft_res(FftArgs a, const TB* __restrict
     const int64_t k = (int64_t)(y + 3) * P + q + 3;
     // closed directions: the problem's unknowns only (the rest of the
     // doubled grid is the extension)
-    const bool unk = !((a.ey && (y == 0 || y >= a.ony)) || (a.ex && (q == 0 || q >= a.onx)));
+    const bool unk = ext_unknown(a, y, q);
     if (unk) {
       const double bv = gval(b, k, a.NXP, gmode) - m0;
       const double Ax = (a.ex | a.ey) ? ax_ext<double>(a, x, y, q)
@@ -650,6 +677,12 @@ static int ilog2(int64_t v) {
 
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
+bool fft_applicable_cells(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet) {
+  const int ex = xper ? 0 : 1, ey = yper ? 0 : 1;
+  return pow2(nx) && pow2(ny) && nx >= 8 && ny >= 8 && (nx << ex) <= 4096 &&
+         (ny << ey) <= 4096 && nwet == nx * ny;
+}
+
 bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet) {
   const int ex = xper ? 0 : 1, ey = yper ? 0 : 1;
   return pow2(nx) && pow2(ny) && nx >= 8 && ny >= 8 && (nx << ex) <= 4096 &&
@@ -730,17 +763,19 @@ static void set_lds(size_t lr, size_t lc) {
 }
 
 FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
-                      double maindiag, bool f64, bool f32, int xper, int yper) {
+                      double maindiag, bool f64, bool f32, int xper, int yper,
+                      bool cells) {
   FftSolver* f = new FftSolver();
   try {
     FftArgs& a = f->a;
-    a.ex = xper ? 0 : 1;
-    a.ey = yper ? 0 : 1;
+    const int closed = cells ? 2 : 1;
+    a.ex = xper ? 0 : closed;
+    a.ey = yper ? 0 : closed;
     a.onx = (int)nx;
     a.ony = (int)ny;
     const int64_t onx = nx, ony = ny;
-    nx = onx << a.ex;  // the transformed (doubled when closed) sizes
-    ny = ony << a.ey;
+    nx = a.ex ? 2 * onx : onx;  // the transformed (doubled when closed) sizes
+    ny = a.ey ? 2 * ony : ony;
     a.nx = (int)nx;
     a.ny = (int)ny;
     a.lgx = ilog2(nx);
@@ -750,9 +785,9 @@ FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
     a.cx = dy / dx;
     a.cy = dx / dy;
     a.d = 2.0 * (a.cx + a.cy) + maindiag;
-    // a closed direction makes the problem non-singular (its odd extension
-    // has no mean)
-    a.singular = (maindiag == 0.0 && !a.ex && !a.ey) ? 1 : 0;
+    // a Dirichlet direction makes the problem non-singular (its odd
+    // extension has no mean); an even one keeps the constant null vector
+    a.singular = (maindiag == 0.0 && a.ex != 1 && a.ey != 1) ? 1 : 0;
     const long double pi = 3.14159265358979323846264338327950288L;
     std::vector<double> lx(a.nk), ly(ny);
     for (int k = 0; k < a.nk; k++)
@@ -844,7 +879,7 @@ template <class TB>
 static const double* mean_of(FftSolver* f, const TB* b, int gmode, hipStream_t s) {
   if (!f->a.singular) return nullptr;
   k_fft_bsum<TB><<<f->nbres, 256, 0, s>>>(f->a, b, gmode, f->part_b);
-  k_fft_mean<<<1, 256, 0, s>>>(f->part_b, f->nbres, (double)f->a.nx * f->a.ny,
+  k_fft_mean<<<1, 256, 0, s>>>(f->part_b, f->nbres, (double)f->a.onx * f->a.ony,
                                f->mean_b);
   return f->mean_b;
 }
@@ -855,7 +890,7 @@ void fft_solve_f64(FftSolver* f, const double* b, int gmode, double* x,
   spectral<double, double, double, 0>(f, b, gmode, x, nullptr, s);
   if (measure) residual<double>(f, b, gmode, x, nullptr, s, mean_of(f, b, gmode, s));
   if (stats && measure)
-    k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.nx * f->a.ny,
+    k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.onx * f->a.ony,
                                   f->a.singular, rtol, 0, measure ? 1 : 0, stats);
 }
 
@@ -899,7 +934,7 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
   }
   f->solved = true;
   if (stats && measure)
-    k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.nx * f->a.ny,
+    k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.onx * f->a.ony,
                                   f->a.singular, rtol, nref, measure ? 1 : 0, stats);
 }
 

@@ -1,5 +1,7 @@
-// f2d_fft.hpp -- direct spectral solve of the doubly periodic 5-point
-// Poisson/Helmholtz operator (internal interface of the library).
+// f2d_fft.hpp -- direct spectral solve of the 5-point Poisson/Helmholtz
+// operator of fully wet domains (internal interface of the library): doubly
+// periodic as below, closed directions through odd (vertex) or even (cell)
+// extensions on the doubled periodic grid.
 //
 // When every interior vertex is an unknown and both directions are periodic
 // (elliptic.py:114-195 with the y-periodic extension), the operator
@@ -22,8 +24,16 @@ struct FftSolver;
 // as its odd extension on the periodic doubled grid (2 nx <= 4096 then)
 bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet);
 
+// the Neumann cell operator (elliptic.py:186-190) of an all-wet domain:
+// every cell unknown, nx, ny powers of two >= 8 (<= 2048 when closed); a
+// closed direction is solved as the even extension about its walls
+bool fft_applicable_cells(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet);
+
+// cells: the Neumann cell operator (even extensions), else the vertex
+// operator (odd extensions of closed directions)
 FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
-                      double maindiag, bool f64, bool f32, int xper, int yper);
+                      double maindiag, bool f64, bool f32, int xper, int yper,
+                      bool cells = false);
 void fft_destroy(FftSolver* f);
 
 // x = A^-1 b on the padded (ny+6) x (nx+6) layout, halo filled.  gmode 0:

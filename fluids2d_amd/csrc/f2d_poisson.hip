@@ -2265,9 +2265,14 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
         nwet += v;
       }
     h->nwet = (double)nwet;
-    if (h->opts.neumann) {
+    // the Neumann cell operator: the direct solve when every cell is wet
+    // (closed directions as even extensions), Jacobi-PCG otherwise
+    const bool cells_fft = h->opts.neumann &&
+                           fft_applicable_cells(h->ny, h->nx, h->xper, h->yper, nwet);
+    if (h->opts.neumann && !(cells_fft && h->opts.solver != F2D_SOLVER_MG)) {
       F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT,
-                  "the Neumann operator is solved by Jacobi-PCG, not the FFT");
+                  "FFT solver requested for a Neumann operator with dry cells or sizes "
+                  "that are not powers of two");
       h->neu = neu_create(m.data(), h->ny, h->nx, h->xper, h->yper, g->dx, g->dy, maindiag);
       *out = h;
       return;
@@ -2309,7 +2314,8 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
         h->e32 = (float*)dalloc(h, N * sizeof(float));
       }
     }
-    const bool fft_ok = fft_applicable(h->ny, h->nx, h->xper, h->yper, nwet);
+    const bool fft_ok = h->opts.neumann ? cells_fft
+                                        : fft_applicable(h->ny, h->nx, h->xper, h->yper, nwet);
     F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT || fft_ok,
                 "FFT solver requested but the operator is not fully wet with "
                 "power-of-two sizes (<= 4096, <= 2048 in a closed direction)");
@@ -2317,7 +2323,8 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
     if (fft_ok && h->opts.solver != F2D_SOLVER_MG)
       h->fft = fft_create(h->ny, h->nx, g->dx, g->dy, maindiag,
                           h->opts.mode == F2D_POISSON_F64,
-                          h->opts.mode != F2D_POISSON_F64, h->xper, h->yper);
+                          h->opts.mode != F2D_POISSON_F64, h->xper, h->yper,
+                          h->opts.neumann != 0);
     F2D_HIP(hipDeviceSynchronize());
     *out = h;
   });
@@ -2352,8 +2359,9 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
     hipStream_t s = S(stream);
     bool sync = iters != nullptr || relres != nullptr;
     const int mode = h->opts.mode;
+    F2D_REQUIRE(!h->opts.neumann || gmode == G_FROM_B,
+                "the Neumann operator takes b on the cells");
     if (h->neu) {
-      F2D_REQUIRE(gmode == G_FROM_B, "the Neumann operator takes b on the cells");
       int it = 0;
       double rr = 0.0;
       neu_solve(h->neu, mode == F2D_POISSON_F64 ? F2D_F64 : F2D_F32, b, x, rtol, maxit, sync,

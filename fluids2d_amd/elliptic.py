@@ -63,10 +63,9 @@ class Poisson2D:
         # direct FFT solve in FP32: refine from the previous solution held in
         # x (the state's streamfunction) instead of a cold FP32 solve
         opts.fft_warm = int(bool(getattr(p, "poisson_warm", True)))
-        # "c": the Neumann cell operator (elliptic.py:186-190), Jacobi-PCG in FP64
+        # "c": the Neumann cell operator (elliptic.py:186-190): the direct
+        # solve when every cell is wet, else Jacobi-PCG in FP64
         opts.neumann = int(location == "c")
-        if location == "c":
-            opts.solver = _lib.SOLVERS["mg"] if opts.solver == _lib.SOLVERS["fft"] else opts.solver
         tuning = getattr(p, "poisson_opts", None) if tuning is None else tuning
         self.tuning = dict(tuning or {})
         for k, v in self.tuning.items():

@@ -306,16 +306,19 @@ typedef struct f2d_poisson_opts {
   int32_t neumann;       /* 1: the Neumann ("c") operator of
                             get_Laplacian_sparse (elliptic.py:186-190,
                             diagonal minus the sum of the unknown
-                            neighbours' coefficients minus maindiag), solved
-                            by Jacobi-preconditioned CG in FP64; b must be
-                            consistent when singular, x keeps the constant
-                            of its initial guess                           */
+                            neighbours' coefficients minus maindiag).  Every
+                            cell wet and power-of-two sizes: the direct
+                            solve (even extensions about closed walls; zero
+                            mean when singular) unless solver is MG; else
+                            Jacobi-preconditioned CG in FP64 (x keeps the
+                            constant of its initial guess)                 */
 } f2d_poisson_opts;
 
 /* Solver selection.  AUTO takes the direct spectral (FFT) solve when every
- * interior vertex is an unknown, both directions are periodic and nx, ny
- * are powers of two in [8, 4096] (the operator is then diagonalised by the
- * 2-D DFT), and the multigrid-preconditioned CG otherwise. */
+ * vertex strictly inside the domain is an unknown and nx, ny are powers of
+ * two in [8, 4096] ([8, 2048] in a closed direction, solved as the odd
+ * extension on the doubled periodic grid; the operator is then diagonalised
+ * by the 2-D DFT), and the multigrid-preconditioned CG otherwise. */
 #define F2D_SOLVER_AUTO 0
 #define F2D_SOLVER_MG 1
 #define F2D_SOLVER_FFT 2

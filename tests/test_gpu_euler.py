@@ -1,6 +1,7 @@
 """The euler model (momentum-pressure, equations.py:9-24) on the device:
 vortex force and kinetic-energy gradient, the pressure projection with the
-Neumann cell Poisson solve (Jacobi-PCG, FP64) against the oracle's SuperLU.
+Neumann cell Poisson solve (direct on all-wet power-of-two domains,
+Jacobi-PCG in FP64 otherwise) against the oracle's SuperLU.
 The pressure is defined up to a constant (singular Neumann operator), so it
 is compared after removing its mean; everything else directly."""
 import numpy as np

@@ -373,6 +373,90 @@ def test_poisson_fft_closed_vs_superlu(dev, L, ny, nx, xper, yper, maindiag, mod
     assert np.array_equal(xx, x)
 
 
+@pytest.mark.parametrize("ny,nx,xper,yper,maindiag,mode,solver", [
+    (64, 64, False, False, 0.0, "f64", "fft"), (128, 64, False, False, 0.7, "f64", "auto"),
+    (64, 256, True, False, 0.0, "f64", "fft"), (256, 32, False, True, 0.3, "f64", "fft"),
+    (128, 128, True, True, 0.0, "f64", "fft"), (512, 512, False, False, 0.0, "f64", "auto"),
+    (256, 256, False, False, 0.0, "refine", "fft"), (128, 512, True, False, 0.0, "refine", "fft"),
+    (64, 64, False, False, 0.0, "f64", "mg")])
+def test_poisson_cells_fft_vs_superlu(dev, L, ny, nx, xper, yper, maindiag, mode, solver):
+    """The Neumann cell operator (elliptic.py:186-190, the euler model's
+    pressure) of an all-wet domain by the direct solve, closed directions as
+    even extensions about the walls, against SuperLU; "mg" keeps the
+    Jacobi-PCG solver.  Without a mass term the operator is singular: the
+    right-hand side is given a nonzero mean (projected out by both) and the
+    solutions are compared after removing their means."""
+    import torch
+    p = P(model="euler", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper,
+          Lx=nx / 64.0, Ly=ny / 80.0)
+    m = O.default_mask(ny, nx, 3, xper, yper)
+    mesh = O.Mesh(p, msk=m)
+    unk = O.unknown_mask(p, mesh.msk)  # the cells, without periodic halo images
+    rng = np.random.default_rng(11)
+    b = (rng.standard_normal(m.shape) + 0.5) * unk
+    inner = unk == 1
+    bref = b.copy()
+    if maindiag == 0.0:
+        bref[inner] -= bref[inner].mean()
+    x_ref = np.zeros(m.shape)
+    O.Poisson2D(p, mesh, "c", maindiag=maindiag).solve(bref, x_ref)
+    g = L.make_grid(p)
+    opts = L.PoissonOpts()
+    opts.mode = L.POISSON_F64 if mode == "f64" else L.POISSON_F32_REFINE
+    opts.nu, opts.neumann = 2, 1
+    opts.solver = L.SOLVERS[solver]
+    h = ctypes.c_void_p()
+    L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+           maindiag, ctypes.byref(opts), ctypes.byref(h))
+    try:
+        method = L.lib().f2d_poisson_method(h)
+        assert method == (3 if solver == "mg" else 2), method
+        dt = torch.float64 if mode == "f64" else torch.float32
+        db = T(b, dev).to(dt)
+        dx = torch.zeros(m.shape, dtype=dt, device=dev)
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        L.call("f2d_poisson_solve", h, L.ptr(db), L.ptr(dx), 1e-13, 2000, ctypes.byref(it),
+               ctypes.byref(rr), L.stream_ptr())
+        x = N(dx).astype(np.float64)
+    finally:
+        L.call("f2d_poisson_destroy", h)
+    got, want = x[inner], x_ref[inner]
+    if maindiag == 0.0:
+        got, want = got - got.mean(), want - want.mean()
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print(f"\ncells {solver} {ny}x{nx} xper={xper} yper={yper} md={maindiag} {mode}: "
+          f"relres={rr.value:.2e} err={err:.2e}")
+    if mode == "f64":
+        # the singular Neumann operator's condition number grows as n^2:
+        # 512^2 closed reaches 1.5e-11 against SuperLU at relres 6e-14
+        assert rr.value <= 1e-12 and err <= 1e-10
+    else:
+        assert rr.value <= 1e-9 and err < 1e-6
+    # periodic directions get their halo images, closed ones are left alone
+    xx = x.copy()
+    O.fill(p, xx)
+    if xper and yper:
+        assert np.array_equal(xx, x)
+    elif xper:
+        assert np.array_equal(xx[3:-3], x[3:-3])
+    elif yper:
+        assert np.array_equal(xx[:, 3:-3], x[:, 3:-3])
+
+
+def test_poisson_cells_fft_refuses_dry_cells(dev, L):
+    p = P(model="euler", nx=64, ny=64)
+    m = random_mask(64, 64, 3, False, False, nisl=3)
+    mesh = O.Mesh(p, msk=m)
+    unk = O.unknown_mask(p, mesh.msk)
+    g = L.make_grid(p)
+    opts = L.PoissonOpts()
+    opts.mode, opts.nu, opts.neumann, opts.solver = L.POISSON_F64, 2, 1, L.SOLVERS["fft"]
+    h = ctypes.c_void_p()
+    with pytest.raises(L.F2DError, match="dry cells"):
+        L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+               0.0, ctypes.byref(opts), ctypes.byref(h))
+
+
 @pytest.mark.parametrize("mode,ny,nx", [("refine", 512, 512), ("refine", 64, 128),
                                         ("f32", 256, 256)])
 def test_poisson_fft_f32_modes(dev, L, mode, ny, nx):
