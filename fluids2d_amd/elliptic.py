@@ -64,8 +64,12 @@ class Poisson2D:
         # x (the state's streamfunction) instead of a cold FP32 solve
         opts.fft_warm = int(bool(getattr(p, "poisson_warm", True)))
         # "c": the Neumann cell operator (elliptic.py:186-190): the direct
-        # solve when every cell is wet, else Jacobi-PCG in FP64
+        # solve when every cell is wet, else Jacobi-PCG in FP64.  Its
+        # right-hand side (the divergence a stage's update left) is not
+        # close to the previous stage's: cold FP32 solves, refined once
         opts.neumann = int(location == "c")
+        if location == "c":
+            opts.fft_warm = 0
         tuning = getattr(p, "poisson_opts", None) if tuning is None else tuning
         self.tuning = dict(tuning or {})
         for k, v in self.tuning.items():

@@ -274,7 +274,7 @@ class RKIntegrator:
     def _solvers(self):
         m = self.mesh
         out = []
-        for name in ("poisson_vertices", "qg_helmholtz"):
+        for name in ("poisson_vertices", "poisson_centers", "qg_helmholtz"):
             e = m.__dict__.get(name)
             if e is not None:
                 out.append(e)

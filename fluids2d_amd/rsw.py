@@ -148,10 +148,15 @@ _Du = namedtuple("prognostic", ("u",))
 class Euler(_SW):
     """Euler in momentum-pressure form (equations.py:9-24): vortex force and
     kinetic-energy gradient, then the pressure projection
-    (operators.py:114-119) with the Neumann cell Poisson solve.  The solve
-    checks convergence on the host, so steps are not graph-captured."""
+    (operators.py:114-119) with the Neumann cell Poisson solve.  With the
+    direct solve (all-wet domains, power-of-two sizes) stages may skip the
+    host check and steps are graph-captured; the Jacobi-PCG solve always
+    checks convergence on the host, so its steps are not."""
     prognostic = ("u",)
-    capturable = False
+
+    @property
+    def capturable(self):
+        return self.mesh.poisson_centers.method == "fft"
 
     def rhs(self, s, ds):
         m, p = self.mesh, self.param
@@ -173,7 +178,8 @@ class Euler(_SW):
         _ops("f2d_div", dt, g, _lib.ptr(s.U.x), _lib.ptr(s.U.y), _lib.ptr(m.d_msk),
              _lib.ptr(s.div))
         rhs = s.div * (-m.area)
-        m.poisson_centers.solve(rhs, s.p)
+        pc = m.poisson_centers
+        pc.solve(rhs, s.p, stats=self._stats if pc.method == "fft" else True)
         _ops("f2d_addgrad", dt, g, _lib.ptr(s.p), _lib.ptr(m.d_mskx), _lib.ptr(m.d_msky),
              _lib.ptr(s.u.x), _lib.ptr(s.u.y))
         op.fill(m, s.u.x)
@@ -203,7 +209,11 @@ class Euler(_SW):
         prev = [leaves(x.u if hasattr(x, "u") else x) for x in ds_prev]
         for n, y in enumerate(ys):
             op.rk_addto(y, [pv[n] for pv in prev] + [dsl[n]], p.integrator, k, dt)
-        self.diag(s)
+        self._stats = stats
+        try:
+            self.diag(s)
+        finally:
+            self._stats = True
         if dt_next is not None:
             op.cfl_dt(s.U, p.cfl, p.dtmax, dt_next)
         return s, alt

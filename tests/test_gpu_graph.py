@@ -1,7 +1,8 @@
 """Captured HIP graphs of device steps replay exactly the eager launches:
 the state after a run with graphs equals, bit for bit, the state after the
 same run without them (eulerpsi with the direct FFT solve and with the
-masked multigrid CG, qg, qgrsw; FP64 and FP32+refinement).  The graph is
+masked multigrid CG, qg, qgrsw, euler with the direct Neumann solve; FP64
+and FP32+refinement).  The graph is
 replayed several times with eager steps and host reads between replays,
 and every replay's dt log is checked against the eager run's."""
 import numpy as np
@@ -35,6 +36,10 @@ def _model(model, nx, ny, xper, yper, nisl, precision, seed, hipgraph):
         s.h[...] = (1 + 0.05 * rng.standard_normal(m.mesh.shape)) * m.mesh.area * m.mesh.msk
         s.u.x[...] = 1e-3 * rng.standard_normal(m.mesh.shape) * m.mesh.mskx
         s.u.y[...] = 1e-3 * rng.standard_normal(m.mesh.shape) * m.mesh.msky
+    elif model == "euler":
+        s = m.state
+        s.u.x[...] = 0.05 * rng.standard_normal(m.mesh.shape) * m.mesh.mskx
+        s.u.y[...] = 0.05 * rng.standard_normal(m.mesh.shape) * m.mesh.msky
     else:
         x, y = m.mesh.xy("c")
         w = np.exp(-((x - 0.4 * p.Lx) ** 2 + (y - 0.5) ** 2) / 0.02) - \
@@ -67,6 +72,8 @@ CASES = [
     ("eulerpsi", 48, 40, True, False, 2, "fp32"),
     ("qg", 48, 48, False, False, 2, "fp64"),
     ("qgrsw", 48, 48, False, False, 0, "fp32"),
+    ("euler", 64, 32, False, False, 0, "fp64"),
+    ("euler", 32, 32, True, False, 0, "fp32"),
 ]
 
 
@@ -81,11 +88,12 @@ def test_graph_replay_matches_eager(dev, model, nx, ny, xper, yper, nisl, precis
     lb = _run(b, chunks)
     assert len(a.integrator._graphs) >= 1
     assert np.array_equal(la, lb), (la, lb)
-    names = ["h", "u"] if model == "qgrsw" else (["pv"] if model == "qg" else ["omega"])
+    names = {"qgrsw": ["h", "u"], "qg": ["pv"], "euler": ["u", "p"]}.get(model, ["omega"])
     for name in names:
         x, y = getattr(a.state, name), getattr(b.state, name)
         for u, v in (zip(x, y) if name == "u" else [(x, y)]):
             assert np.array_equal(np.nan_to_num(u), np.nan_to_num(v)), name
-    pa = a.mesh.poisson_vertices.read_stats()
-    pb = b.mesh.poisson_vertices.read_stats()
+    solver = "poisson_centers" if model == "euler" else "poisson_vertices"
+    pa = getattr(a.mesh, solver).read_stats()
+    pb = getattr(b.mesh, solver).read_stats()
     assert pa["failed"] == 0 and pb["failed"] == 0
