@@ -236,6 +236,16 @@ __device__ __forceinline__ void transport_halo(const TransportArgs<T>& a, int64_
   stage_update<T>(a, k, ds, a.q[k]);
 }
 
+// Tile of workgroup b (counted after the halo-ring blocks, whose number is
+// a multiple of 8).  Workgroups are dispatched round-robin over the 8 XCDs,
+// workgroup b to XCD b % 8; with nt % 8 == 0 workgroup b takes tile
+// (b % 8) * (nt / 8) + b / 8, so each XCD runs one contiguous band of
+// tile rows and the halo rows and boundary lines that neighbouring tiles
+// share are fetched once into that XCD's L2.
+__device__ __forceinline__ int tile_of(int b, int nt) {
+  return (nt & 7) ? b : (b & 7) * (nt >> 3) + (b >> 3);
+}
+
 template <class T, int M, int TY, int TX>
 __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
   constexpr int LW = TX + 6;
@@ -249,7 +259,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
   // the halo-ring blocks come first: they start with the tiles instead of
   // trailing them
   if ((int)blockIdx.x >= a.nhb) {
-    const int tile = (int)blockIdx.x - a.nhb;
+    const int tile = tile_of((int)blockIdx.x - a.nhb, a.ntiles);
     const int by = tile / a.tiles_x, bx = tile - by * a.tiles_x;
     const int j0 = 3 + by * TY, i0 = 3 + bx * TX;
     const int NYP = a.ny + 6, NXP = a.nx + 6;
@@ -394,7 +404,7 @@ __global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a) {
     return;
   }
   const uint32_t P = (uint32_t)a.P;
-  const int tile = (int)blockIdx.x - a.nhb;
+  const int tile = tile_of((int)blockIdx.x - a.nhb, a.ntiles);
   const int by = tile / a.tiles_x, bx = tile - by * a.tiles_x;
   const int j0 = 3 + by * TY, i0 = 3 + bx * TX;
   const int c = t & (TX - 1), r0 = t / TX;
@@ -515,7 +525,9 @@ void launch_transport_tile(const TransportArgs<T>& a0, hipStream_t s) {
   int tiles_y = (a.ny + TY - 1) / TY;
   a.ntiles = a.tiles_x * tiles_y;
   a.nhalo = 6 * (int64_t)(a.nx + 6) + 6 * (int64_t)a.ny;
-  a.nhb = (int)((a.nhalo + 255) / 256);
+  // a multiple of 8 halo-ring blocks (the surplus returns at once): tile
+  // workgroup b then runs on XCD b % 8 (tile_of)
+  a.nhb = (int)(((a.nhalo + 255) / 256 + 7) & ~(int64_t)7);
   int nb = a.ntiles + a.nhb;
   // whole tiles, and every padded offset within 32 bits
   const bool whole = a.ny % TY == 0 && a.nx % TX == 0 &&
@@ -620,16 +632,40 @@ __device__ __forceinline__ double nanmax(double a, double b) {
 
 constexpr int PG_PTS = 4;  // points per thread of k_perpgrad
 
+// block max of v over the 256 threads (NaN-propagating, order-free):
+// butterflies within each 64-lane wave, then the 4 wave results
+__device__ __forceinline__ double block_nanmax256(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
+  __syncthreads();
+  return r;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
-  __shared__ double sh[256];
+  __shared__ double sh[4];
   const int NY = a.ny + 6, NX = a.nx + 6;
   const int64_t n = (int64_t)NY * NX;
+  // chunk of this workgroup: XCD b % 8 takes a contiguous run of chunks
+  // (the grid is a multiple of 8), so the row below a chunk, which its
+  // x-face differences read, is mostly in the same L2
+  const int G = gridDim.x, b = blockIdx.x;
+  const int64_t chunk = (b & 7) * (G >> 3) + (b >> 3);
   double mxv = 0.0, myv = 0.0;
   for (int r = 0; r < PG_PTS; r++) {
-    int64_t k = ((int64_t)blockIdx.x * PG_PTS + r) * 256 + threadIdx.x;
+    int64_t k = (chunk * PG_PTS + r) * 256 + threadIdx.x;
     if (k >= n) break;
-    int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+    int j, i;
+    if (n < ((int64_t)1 << 31)) {
+      j = (int)((uint32_t)k / (uint32_t)NX);
+      i = (int)((uint32_t)k - (uint32_t)j * (uint32_t)NX);
+    } else {
+      j = (int)(k / NX);
+      i = (int)(k - (int64_t)j * NX);
+    }
     int js = j, is = i;
     if (a.fill) {
       if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
@@ -651,24 +687,11 @@ __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
     myv = nanmax(myv, fabs((double)uy));
   }
   if (!a.partials) return;
-  const int t = threadIdx.x;
-  sh[t] = mxv;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
-    __syncthreads();
-  }
-  double bx = sh[0];
-  __syncthreads();
-  sh[t] = myv;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) sh[t] = nanmax(sh[t], sh[t + s]);
-    __syncthreads();
-  }
-  if (t == 0) {
+  const double bx = block_nanmax256(mxv, sh);
+  const double by = block_nanmax256(myv, sh);
+  if (threadIdx.x == 0) {
     a.partials[blockIdx.x] = bx;
-    a.partials[gridDim.x + blockIdx.x] = sh[0];
+    a.partials[gridDim.x + blockIdx.x] = by;
   }
 }
 
@@ -1013,7 +1036,9 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
   return guarded([&] {
     check_grid(g);
     int64_t n = (g->ny + 6) * (g->nx + 6);
-    int nb = grid1d(n, 256 * PG_PTS);
+    // a multiple of 8 workgroups (k_perpgrad's chunk order); the surplus
+    // chunks are empty and report maxima of 0
+    int nb = (grid1d(n, 256 * PG_PTS) + 7) & ~7;
     double* part = (maxabs || dt) ? red_scratch(nb, stream) : nullptr;
     F2D_DTYPE_DISPATCH(dtype, {
       PerpArgs<T> a{};
