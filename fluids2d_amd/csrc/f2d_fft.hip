@@ -135,15 +135,15 @@ __device__ __forceinline__ double gval(const TB* __restrict__ b, int64_t k,
                   (double)b[k - NXP]) + (double)b[k]);
 }
 
+// sum of v over the 256 threads of the workgroup, the same on every
+// thread: butterflies within each 64-lane wave, then the 4 wave sums in a
+// fixed order (deterministic); sh holds at least 4 doubles
 __device__ __forceinline__ double bsum256(double v, double* sh) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
 #pragma unroll
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  const double r = sh[0];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
   __syncthreads();
   return r;
 }
