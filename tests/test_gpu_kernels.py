@@ -159,16 +159,21 @@ def test_flat_loops_fp32(dev, L, method):
         assert_same(N(d), du, f"vortexforce fp32 s={s}", rtol=3e-6)
 
 
-@pytest.mark.parametrize("xper,yper,method,integ", [
-    (False, False, "weno", "rk3"), (True, False, "weno", "rk3"),
-    (True, True, "weno", "enrk3"), (False, True, "upwind", "rk3"),
-    (True, False, "centered", "ef"), (False, False, "cweno", "rk3")])
-def test_transport_stage_bitexact(dev, L, xper, yper, method, integ):
+@pytest.mark.parametrize("xper,yper,method,integ,ny,nx,tile", [
+    (False, False, "weno", "rk3", 37, 70, 0), (True, False, "weno", "rk3", 37, 70, 0),
+    (True, True, "weno", "enrk3", 37, 70, 0), (False, True, "upwind", "rk3", 37, 70, 0),
+    (True, False, "centered", "ef", 37, 70, 0), (False, False, "cweno", "rk3", 37, 70, 0),
+    (True, True, "weno", "rk3", 64, 256, 4), (False, False, "weno", "rk3", 64, 256, 0),
+    (True, False, "weno", "rk3", 128, 128, 3)])
+def test_transport_stage_bitexact(dev, L, xper, yper, method, integ, ny, nx, tile):
+    """37x70: partial tiles in both directions (k_transport).  The other
+    shapes are whole tiles (k_transport_full) whose count is a multiple of 8,
+    so the tiles are handed out in XCD order: 16 tiles of 8x128, 16x64 or
+    32x32."""
     import torch
-    ny, nx = 37, 70  # partial tiles in both directions
     m = random_mask(ny, nx, 11, xper, yper, nisl=3)
     p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=xper, yperiodic=yper,
-          compflux=method)
+          compflux=method, tile=tile)
     mesh = O.Mesh(p, msk=m)
     rng = np.random.default_rng(4)
     q = rng.standard_normal(m.shape)
