@@ -677,6 +677,14 @@ static int ilog2(int64_t v) {
 
 static bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
+bool fft_fits(int64_t ny, int64_t nx, int xper, int yper, bool f64) {
+  // one row (column) per workgroup: two FFT buffers and the twiddles in LDS
+  // (plan_blocks)
+  const size_t c = f64 ? 16 : 8;
+  const int64_t tx = xper ? nx : 2 * nx, ty = yper ? ny : 2 * ny;
+  return 3 * (size_t)tx * c <= 160 * 1024 && 3 * (size_t)ty * c <= 160 * 1024;
+}
+
 bool fft_applicable_cells(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet) {
   const int ex = xper ? 0 : 1, ey = yper ? 0 : 1;
   return pow2(nx) && pow2(ny) && nx >= 8 && ny >= 8 && (nx << ex) <= 4096 &&

@@ -29,6 +29,10 @@ bool fft_applicable(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet);
 // closed direction is solved as the even extension about its walls
 bool fft_applicable_cells(int64_t ny, int64_t nx, int xper, int yper, int64_t nwet);
 
+// the transforms of a row and of a column fit in a workgroup's LDS
+// (FP64: transformed sizes up to 2048; FP32: up to 4096)
+bool fft_fits(int64_t ny, int64_t nx, int xper, int yper, bool f64);
+
 // cells: the Neumann cell operator (even extensions), else the vertex
 // operator (odd extensions of closed directions)
 FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,

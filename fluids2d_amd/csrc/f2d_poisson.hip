@@ -2267,7 +2267,9 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
     h->nwet = (double)nwet;
     // the Neumann cell operator: the direct solve when every cell is wet
     // (closed directions as even extensions), Jacobi-PCG otherwise
-    const bool cells_fft = h->opts.neumann &&
+    const bool fits = fft_fits(h->ny, h->nx, h->xper, h->yper,
+                               h->opts.mode == F2D_POISSON_F64);
+    const bool cells_fft = h->opts.neumann && fits &&
                            fft_applicable_cells(h->ny, h->nx, h->xper, h->yper, nwet);
     if (h->opts.neumann && !(cells_fft && h->opts.solver != F2D_SOLVER_MG)) {
       F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT,
@@ -2314,11 +2316,13 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
         h->e32 = (float*)dalloc(h, N * sizeof(float));
       }
     }
-    const bool fft_ok = h->opts.neumann ? cells_fft
-                                        : fft_applicable(h->ny, h->nx, h->xper, h->yper, nwet);
+    const bool fft_ok =
+        h->opts.neumann ? cells_fft
+                        : fits && fft_applicable(h->ny, h->nx, h->xper, h->yper, nwet);
     F2D_REQUIRE(h->opts.solver != F2D_SOLVER_FFT || fft_ok,
                 "FFT solver requested but the operator is not fully wet with "
-                "power-of-two sizes (<= 4096, <= 2048 in a closed direction)");
+                "power-of-two sizes (<= 4096, <= 2048 in a closed direction; "
+                "FP64: transformed sizes <= 2048)");
     F2D_REQUIRE(h->opts.solver >= 0 && h->opts.solver <= 2, "opts.solver unknown");
     if (fft_ok && h->opts.solver != F2D_SOLVER_MG)
       h->fft = fft_create(h->ny, h->nx, g->dx, g->dy, maindiag,

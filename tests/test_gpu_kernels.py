@@ -448,6 +448,32 @@ def test_poisson_cells_fft_vs_superlu(dev, L, ny, nx, xper, yper, maindiag, mode
         assert np.array_equal(xx[:, 3:-3], x[:, 3:-3])
 
 
+def test_poisson_fft_lds_limit(dev, L):
+    """A closed 64 x 2048 basin transforms rows of 4096: FP32 fits the
+    workgroup's LDS, FP64 does not, so "auto" takes multigrid in FP64 and an
+    explicit "fft" request is refused."""
+    p = P(model="eulerpsi", nx=2048, ny=64, Lx=32.0, Ly=1.0)
+    m = O.default_mask(64, 2048, 3, False, False)
+    mesh = O.Mesh(p, msk=m)
+    unk = O.unknown_mask(p, mesh.mskv)
+    g = L.make_grid(p)
+
+    def create(mode, solver):
+        opts = L.PoissonOpts()
+        opts.mode, opts.nu, opts.solver = mode, 2, L.SOLVERS[solver]
+        h = ctypes.c_void_p()
+        L.call("f2d_poisson_create", unk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(g),
+               0.0, ctypes.byref(opts), ctypes.byref(h))
+        method = L.lib().f2d_poisson_method(h)
+        L.call("f2d_poisson_destroy", h)
+        return method
+
+    assert create(L.POISSON_F32_REFINE, "auto") == 2
+    assert create(L.POISSON_F64, "auto") == 1
+    with pytest.raises(L.F2DError, match="FP64"):
+        create(L.POISSON_F64, "fft")
+
+
 def test_poisson_cells_fft_refuses_dry_cells(dev, L):
     p = P(model="euler", nx=64, ny=64)
     m = random_mask(64, 64, 3, False, False, nisl=3)
