@@ -1,7 +1,7 @@
 """Whole-model parity on the GPU: the drop-in Model against the oracle's
 FP64 restatement of the reference, on identical seeded inputs.
 
-Tolerances (north-star parity bar): relative L2 of the prognostic field
+Tolerances (the FP64 parity bar): relative L2 of the prognostic field
 <= 1e-12 after 100 FP64 steps; the reference's own known-answer test
 (tests/test_models.py:17-26, advection body rotation: 37 iterations) exact.
 """
