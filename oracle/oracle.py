@@ -358,14 +358,33 @@ class Mesh:
         self.ov = S(ovx.astype(np.int8), ovy.astype(np.int8))
         self.ok = S(set_order(self.mskx, -self.xshift, mo),
                     set_order(self.msky, -self.yshift, mo))
-        self.poisson_vertices = Poisson2D(p, self, "v")
-        # meshes.py:45-47: the Neumann cell operator of the euler projection
-        self.poisson_centers = Poisson2D(p, self, "c") if p.model == "euler" else None
+        # the solvers (meshes.py:41-47) are factorised on first use: the
+        # reference builds them here, but a run that never solves (rsw) need
+        # not wait for SuperLU on a large grid; results are the same
+        for name in self._SOLVERS:
+            self.__dict__.pop(name, None)
         if p.model in ("qg", "qgrsw", "rsw"):
-            maindiag = self.area * p.f0 ** 2 / (p.g * p.H)
-            self.qg_helmholtz = Poisson2D(p, self, "v", maindiag=maindiag)
             self.hb = 0
             self.qgcoef = p.f0 / +p.H
+
+    _SOLVERS = ("poisson_vertices", "poisson_centers", "qg_helmholtz")
+
+    def __getattr__(self, name):
+        if name not in Mesh._SOLVERS or "param" not in self.__dict__:
+            raise AttributeError(name)
+        p = self.param
+        if name == "poisson_vertices":
+            v = Poisson2D(p, self, "v")
+        elif name == "poisson_centers":
+            # meshes.py:45-47: the Neumann cell operator of the euler projection
+            v = Poisson2D(p, self, "c") if p.model == "euler" else None
+        elif p.model in ("qg", "qgrsw", "rsw"):
+            maindiag = self.area * p.f0 ** 2 / (p.g * p.H)
+            v = Poisson2D(p, self, "v", maindiag=maindiag)
+        else:
+            raise AttributeError(name)
+        self.__dict__[name] = v
+        return v
 
     def xy(self, which="c"):
         """meshes.py:57-68"""

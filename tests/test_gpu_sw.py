@@ -158,3 +158,78 @@ def test_qgrsw_nan_cells_like_reference(dev):
     assert np.array_equal(np.isnan(ds.h), np.isnan(ds_ref.h))
     ok = ~np.isnan(ds_ref.h)
     assert _rel(ds.h[ok], ds_ref.h[ok]) < 1e-11
+
+
+def _c3_mask(nx, ny, seed, nisl=12):
+    """12 seeded circular islands, radius U{20..60} cells, centres at least
+    r + 3 cells from the walls (x periodic, walls in y)."""
+    m = O.default_mask(ny, nx, 3, True, False)
+    rng = np.random.default_rng(seed)
+    jj, ii = np.mgrid[0:ny + 6, 0:nx + 6]
+    for _ in range(nisl):
+        r = int(rng.integers(20, 61))
+        cj = 3 + rng.uniform(r + 3, ny - r - 3)
+        ci = 3 + rng.uniform(0, nx)
+        d2 = (jj - cj) ** 2 + np.minimum(np.abs(ii - ci), nx - np.abs(ii - ci)) ** 2
+        m[d2 < r * r] = 0
+    m[:, :3] = m[:, nx:nx + 3]  # periodic halo columns
+    m[:, nx + 3:] = m[:, 3:6]
+    return m
+
+
+def test_rsw_2048x1024_islands(dev):
+    """The 2048 x 1024 rsw workload (Lx = 2, Ly = 1, x periodic, walls in y,
+    12 islands, WENO5 everywhere, f0 = g = H = 1, CFL 0.9): bit-exact
+    against the oracle for 3 steps at full size; then 300 device steps in
+    FP64 and in FP32, which conserve the wet-cell mass of h (flux form,
+    closed walls) and stay within 1e-5 of each other."""
+    import time as _time
+    import torch
+    nx, ny = 2048, 1024
+    msk = _c3_mask(nx, ny, 31)
+    kw = dict(model="rsw", nx=nx, ny=ny, Lx=2.0, Ly=1.0, xperiodic=True, yperiodic=False,
+              f0=1.0, g=1.0, H=1.0, cfl=0.9, tend=1e9, compflux="weno",
+              vortexforce="weno", innerproduct="weno")
+    print(f"\nrsw {nx}x{ny}: {int((msk[3:-3, 3:-3] == 0).sum())} dry cells", flush=True)
+    s, r = _run_pair(_param(**kw), msk, 3, seed=5)
+    assert_same(s.h, r.h, "h")
+    assert_same(s.u.x, r.u.x, "u.x")
+    assert_same(s.u.y, r.u.y, "u.y")
+    del r
+    wet = msk == 1
+    wet[:, :3] = False
+    wet[:, -3:] = False
+    out = {}
+    for prec in ("fp64", "fp32"):
+        import fluids2d_amd as f2d
+        p = _param(precision=prec, **kw)
+        model = f2d.Model(p)
+        model.mesh.msk[...] = msk
+        model.mesh.finalize()
+        model.integrator.model.mesh = model.mesh
+        h0 = _bumps(model.mesh, 5)
+        # periodic halo columns equal to their interior sources, as a fill
+        # leaves them: with another halo the fluxes through the periodic
+        # boundary differ at its two ends at every step, and the mass drifts
+        h0[:, :3] = h0[:, nx:nx + 3]
+        h0[:, nx + 3:] = h0[:, 3:6]
+        model.state.h[...] = h0
+        m0 = model.state.h[wet].sum()
+        model.integrator.diag(model.state)
+        p.maxite = 300
+        torch.cuda.synchronize()
+        t0 = _time.perf_counter()
+        model.run()
+        torch.cuda.synchronize()
+        ms = (_time.perf_counter() - t0) / 300 * 1e3
+        h = model.state.h.astype(np.float64)
+        dm = abs(h[wet].sum() - m0) / m0
+        print(f"  {prec}: 300 steps, {ms:.3f} ms/step, mass drift {dm:.2e}, "
+              f"t = {model.time.t:.5f}", flush=True)
+        assert np.isfinite(h).all()
+        assert dm <= (1e-13 if prec == "fp64" else 1e-6), (prec, dm)
+        out[prec] = (h, model.time.t)
+    e = _rel(out["fp32"][0], out["fp64"][0])
+    print(f"  FP32 vs FP64 after 300 steps: rel L2(h) {e:.2e}")
+    assert abs(out["fp32"][1] - out["fp64"][1]) <= 1e-5 * out["fp64"][1]
+    assert e <= 1e-5
