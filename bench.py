@@ -373,6 +373,7 @@ def cpu_baseline(nx, seed, nsteps):
         setattr(b, k, getattr(p, k))
     t0 = time.perf_counter()
     m = O.Model(b)
+    m.mesh.poisson_vertices  # the SuperLU factorisation (made on first use)
     tfact = time.perf_counter() - t0
     m.state.omega[...] = initial_omega(m.mesh, seed)
     m.diag(m.state)
@@ -380,6 +381,92 @@ def cpu_baseline(nx, seed, nsteps):
     m.run(nsteps)
     el = time.perf_counter() - t0
     return nx * nx * nsteps / el, el, tfact
+
+
+CLOSED_NX = 256
+
+
+def closed_param(precision):
+    """eulerpsi 256^2 closed square, WENO5, SSP-RK3, CFL 0.8 (the workload the
+    reference runs on a CPU)."""
+    p = make_param(CLOSED_NX, precision)
+    p.xperiodic = p.yperiodic = False
+    p.cfl = 0.8
+    return p
+
+
+def vortex_omega(mesh, seed):
+    """Gaussian vortex (width 0.1, centred) plus 1e-3 uniform noise at the
+    cell centres, times msk * area (vortex.py:23-26's convention)."""
+    x, y = mesh.xy("c")
+    rng = np.random.default_rng(seed)
+    w = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / (2 * 0.1 ** 2))
+    w = w + 1e-3 * rng.uniform(-1, 1, w.shape)
+    return w * mesh.msk * mesh.area
+
+
+def closed_bench(seed, steps):
+    """GPU step times of the closed 256^2 workload, FP64 and FP32 + refinement
+    (direct solve through the odd extension)."""
+    import torch
+    from fluids2d_amd import Model
+    out = {"workload": f"eulerpsi {CLOSED_NX}x{CLOSED_NX} closed square, WENO5, SSP-RK3, "
+                       f"CFL 0.8, Gaussian vortex + 1e-3 noise (seed {seed})",
+           "steps": steps}
+    for prec in ("fp64", "fp32"):
+        m = Model(closed_param(prec))
+        m.state.omega[...] = vortex_omega(m.mesh, seed)
+        m.integrator.diag(m.state)
+        m.integrator.begin(m.state)
+        calibrate(m, 3)
+        ms, wall, st, dts = timed(m, steps, 20)
+        out[f"{prec}_ms_per_step"] = ms
+        out[f"{prec}_cell_steps_per_s"] = CLOSED_NX ** 2 / (ms * 1e-3)
+        del m
+        torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline_closed(seed, steps):
+    """The oracle (C loops + SuperLU, one thread) on the closed 256^2 workload
+    for `steps` steps, and the device FP64 run of the same steps from the
+    same state checked against it (relative L2 of the vorticity)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from fluids2d_amd import Model
+    p = closed_param("fp64")
+    p.maxite = steps
+    m = Model(p)
+    w0 = vortex_omega(m.mesh, seed)
+    m.state.omega[...] = w0
+    m.integrator.diag(m.state)
+    m.run()
+
+    class B:
+        pass
+    b = B()
+    for k in ["model", "nx", "ny", "Lx", "Ly", "xperiodic", "yperiodic", "halowidth",
+              "noslip", "f0", "beta", "g", "H", "tend", "dt", "maxite", "dtmax",
+              "integrator", "cfl", "compflux", "vortexforce", "innerproduct", "maxorder"]:
+        setattr(b, k, getattr(p, k))
+    t0 = time.perf_counter()
+    ref = O.Model(b)
+    ref.mesh.poisson_vertices  # the SuperLU factorisation (made on first use)
+    tfact = time.perf_counter() - t0
+    ref.state.omega[...] = w0
+    ref.diag(ref.state)
+    t0 = time.perf_counter()
+    ref.run(steps)
+    el = time.perf_counter() - t0
+    err = float(np.linalg.norm(m.state.omega - ref.state.omega) /
+                np.linalg.norm(ref.state.omega))
+    return {"value": CLOSED_NX ** 2 * steps / el, "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} RK3 steps of the closed {CLOSED_NX}^2 workload through the "
+                      f"oracle, one thread, {el:.1f} s timed; SuperLU factorisation "
+                      f"({tfact:.1f} s) excluded",
+            "gpu_fp64_vs_oracle_rel_l2_omega": err,
+            "iterations": [int(m.time.ite), int(ref.time.ite)]}
 
 
 def main():
@@ -393,6 +480,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=16,
                     help="RK3 steps of the CPU baseline sample (about 10 s at 1024^2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--closed-steps", type=int, default=200,
+                    help="steps of the closed 256^2 workload (GPU timing, and the oracle "
+                         "with the CPU baseline); 0: skip")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint cost measurement")
@@ -507,6 +597,10 @@ def main():
                                       "checkpoints every 20, gradient of a final misfit "
                                       "w.r.t. the initial vorticity",
                           "forward_s": tfw, "forward_plus_adjoint_s": tfb}
+    if a.closed_steps > 0:
+        out["closed256"] = closed_bench(a.seed, a.closed_steps)
+        if cpu is not None:
+            cpu["closed256"] = cpu_baseline_closed(a.seed, a.closed_steps)
     if a.ensemble > 0:
         out["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps,
                                          log=lambda m: print(m, file=sys.stderr, flush=True))
