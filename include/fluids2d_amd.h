@@ -318,7 +318,9 @@ typedef struct f2d_poisson_opts {
  * vertex strictly inside the domain is an unknown and nx, ny are powers of
  * two in [8, 4096] ([8, 2048] in a closed direction, solved as the odd
  * extension on the doubled periodic grid; the operator is then diagonalised
- * by the 2-D DFT), and the multigrid-preconditioned CG otherwise. */
+ * by the 2-D DFT) and, in F2D_POISSON_F64, the transformed (doubled when
+ * closed) sizes are at most 2048 (one row in a workgroup's LDS); the
+ * multigrid-preconditioned CG otherwise. */
 #define F2D_SOLVER_AUTO 0
 #define F2D_SOLVER_MG 1
 #define F2D_SOLVER_FFT 2
