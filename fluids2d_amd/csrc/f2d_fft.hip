@@ -302,48 +302,62 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
     mb = bsum256(acc, sh) / ((double)a.onx * (double)a.ony);
   }
   double sum = 0.0;
-  for (int w = threadIdx.x; w < P * nx; w += blockDim.x) {
-    const int p = w >> lg, n = w & (nx - 1);
-    const int64_t ka = (int64_t)(y0 + 2 * p + 3) * a.NXP + n + 3;
-    double va, vb;
-    if (a.ex | a.ey) {
-      // extension of a closed direction: the value (or residual) at the
-      // problem's point, with the reflection's sign
-      double v2[2];
-      for (int h = 0; h < 2; h++) {
-        int yo, xo;
-        const int sg = ext_of(a, y0 + 2 * p + h, n, yo, xo);
-        double v = 0.0;
-        if (sg) {
-          const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
-          const double g = gval(b, k, a.NXP, gmode);
-          // sums of b over the problem's points (the singular even case)
-          if (ext_unknown(a, y0 + 2 * p + h, n)) sum += g;
-          v = g;
-          if (x32)
-            v = (g - mb) - ax_ext<float>(a, x32, yo, xo);
-          else if (x64)
-            v = (g - mb) - ax_ext<double>(a, x64, yo, xo);
-          if (sg < 0) v = -v;
-        }
-        v2[h] = v;
+  if (a.ex | a.ey) {
+    // extension of a closed direction: the value (or residual) at each of
+    // the problem's points once, written to its doubled-grid point and,
+    // in a closed x, to the mirror point with the reflection's sign (0 on
+    // an odd extension's boundary lines).  Component h of the complex row
+    // pair holds row y0 + 2p + h.
+    const int nh = a.ex ? a.onx : nx;
+    T* bs = reinterpret_cast<T*>(b0);
+    for (int w = threadIdx.x; w < P * 2 * nh; w += blockDim.x) {
+      const int p = w / (2 * nh), r = w - p * (2 * nh);
+      const int h = r >= nh ? 1 : 0, n = r - h * nh;
+      const int y = y0 + 2 * p + h;
+      int yo, xo;
+      const int sg = ext_of(a, y, n, yo, xo);
+      double v = 0.0;
+      if (sg) {
+        const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
+        const double g = gval(b, k, a.NXP, gmode);
+        // sums of b over the problem's points (the singular even case)
+        if (ext_unknown(a, y, n)) sum += g;
+        v = g;
+        if (x32)
+          v = (g - mb) - ax_ext<float>(a, x32, yo, xo);
+        else if (x64)
+          v = (g - mb) - ax_ext<double>(a, x64, yo, xo);
+        if (sg < 0) v = -v;
       }
-      va = v2[0];
-      vb = v2[1];
-    } else if (x32) {
-      const double ga = gval(b, ka, a.NXP, gmode), gb = gval(b, ka + a.NXP, a.NXP, gmode);
-      sum += ga + gb;
-      va = rval32(a, ga, x32, y0 + 2 * p, n, mb);
-      vb = rval32(a, gb, x32, y0 + 2 * p + 1, n, mb);
-    } else if (x64) {
-      va = rval(a, b, gmode, x64, ka, mb);
-      vb = rval(a, b, gmode, x64, ka + a.NXP, mb);
-    } else {
-      va = gval(b, ka, a.NXP, gmode);
-      vb = gval(b, ka + a.NXP, a.NXP, gmode);
-      sum += va + vb;
+      const int q = p * nx;
+      bs[2 * (q + n) + h] = T(v);
+      if (a.ex == 1) {
+        if (n > 0) bs[2 * (q + 2 * a.onx - n) + h] = T(-v);
+        else bs[2 * (q + a.onx) + h] = T(0);  // the boundary line
+      } else if (a.ex == 2) {
+        bs[2 * (q + 2 * a.onx - 1 - n) + h] = T(v);
+      }
     }
-    b0[w] = mk<T>(T(va), T(vb));
+  } else {
+    for (int w = threadIdx.x; w < P * nx; w += blockDim.x) {
+      const int p = w >> lg, n = w & (nx - 1);
+      const int64_t ka = (int64_t)(y0 + 2 * p + 3) * a.NXP + n + 3;
+      double va, vb;
+      if (x32) {
+        const double ga = gval(b, ka, a.NXP, gmode), gb = gval(b, ka + a.NXP, a.NXP, gmode);
+        sum += ga + gb;
+        va = rval32(a, ga, x32, y0 + 2 * p, n, mb);
+        vb = rval32(a, gb, x32, y0 + 2 * p + 1, n, mb);
+      } else if (x64) {
+        va = rval(a, b, gmode, x64, ka, mb);
+        vb = rval(a, b, gmode, x64, ka + a.NXP, mb);
+      } else {
+        va = gval(b, ka, a.NXP, gmode);
+        vb = gval(b, ka + a.NXP, a.NXP, gmode);
+        sum += va + vb;
+      }
+      b0[w] = mk<T>(T(va), T(vb));
+    }
   }
   if (bpart) {
     sum = bsum256(sum, sh);
