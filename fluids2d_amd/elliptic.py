@@ -117,13 +117,19 @@ class Poisson2D:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            self._h = None
+        if h is None or not h.value:
+            return
+        self._h = None
+        try:
             if _lib.capturing():
                 # freeing device memory would invalidate the capture: later
                 _PENDING.append(h)
                 return
             _destroy(h)
+        except TypeError:
+            # at interpreter shutdown the module's globals may already be
+            # gone; the process releases the device memory itself
+            pass
 
 
     def handle(self):
