@@ -135,17 +135,32 @@ __device__ __forceinline__ double gval(const TB* __restrict__ b, int64_t k,
                   (double)b[k - NXP]) + (double)b[k]);
 }
 
-// sum of v over the 256 threads of the workgroup, the same on every
-// thread: butterflies within each 64-lane wave, then the 4 wave sums in a
-// fixed order (deterministic); sh holds at least 4 doubles
+// sum of v over the threads of the workgroup (a multiple of 64, at most
+// 1024), the same on every thread: butterflies within each 64-lane wave,
+// then the wave sums in a fixed order (deterministic); sh holds at least
+// 16 doubles
 __device__ __forceinline__ double bsum256(double v, double* sh) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  const int nw = blockDim.x >> 6;
+  double r;
+  if (nw == 4) {
+    r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  } else {
+    r = 0.0;
+    for (int i = 0; i < nw; i++) r += sh[i];
+  }
   __syncthreads();
   return r;
+}
+
+// workgroup size of the FFT passes: 256 threads up to 1024 points per
+// workgroup (4 per thread), more for longer rows and columns, so that a CU
+// holding one or two such workgroups (their LDS) still has waves to switch
+static inline int fft_threads(int points) {
+  return points >= 4096 ? 1024 : (points >= 2048 ? 512 : 256);
 }
 
 // right-hand side of a refinement: (b - mb) - A x64 at padded index k
@@ -277,7 +292,7 @@ __device__ __forceinline__ void put_ext(TX* x, int y, int n, const FftArgs& a, T
 // the residual (b - mb) - A x, mb = (sum of the nbp partials bpin) / (nx ny);
 // bpart, if given, still receives the sums of b.
 template <class T, class TB>
-__global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
+__global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
                                                       const TB* __restrict__ b,
                                                       int gmode,
                                                       cplx<T>* __restrict__ S,
@@ -396,7 +411,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(FftArgs a,
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fft_cols(FftArgs a,
+__global__ __launch_bounds__(1024) void k_fft_cols(FftArgs a,
                                                   cplx<T>* __restrict__ S,
                                                   const cplx<T>* __restrict__ tw,
                                                   const double* __restrict__ lamx,
@@ -449,7 +464,7 @@ __device__ __forceinline__ void put_images(TX* x, int y, int n, const FftArgs& a
 // when given (each interior point is read and written by one thread; the
 // halo images written are never read)
 template <class T, class TX, int MODE>
-__global__ __launch_bounds__(256) void k_fft_rows_inv(FftArgs a,
+__global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
                                                       const cplx<T>* __restrict__ S,
                                                       const cplx<T>* __restrict__ tw,
                                                       TX* __restrict__ x,
@@ -882,10 +897,10 @@ static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
   // closed y: rows 0 .. ony-1 only (the rest mirror them, see k_fft_rows_fwd)
   const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
   const double* bpin = ((xres || xres32) && a.singular) ? rs_in : nullptr;
-  k_fft_rows_fwd<T, TB><<<gr, 256, lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
+  k_fft_rows_fwd<T, TB><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
                                             xres32);
-  k_fft_cols<T><<<gc, 256, lc, s>>>(a, S, twy, f->lamx, f->lamy);
-  k_fft_rows_inv<T, TX, MODE><<<gr, 256, lr, s>>>(a, S, twx, x, x64,
+  k_fft_cols<T><<<gc, fft_threads(a.Q * a.ny), lc, s>>>(a, S, twy, f->lamx, f->lamy);
+  k_fft_rows_inv<T, TX, MODE><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, S, twx, x, x64,
                                                   keep_sums ? f->rowsum[1] : nullptr,
                                                   keep_sums ? f->rowsum[0] : nullptr);
 }
@@ -993,34 +1008,34 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
     if (f64) {
       double* buf = f->bench_buf;
       if (kind == 0)
-        k_fft_rows_fwd<double, double><<<gr, 256, lr, s>>>(a, buf, 0, f->S64, f->twx64,
+        k_fft_rows_fwd<double, double><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, buf, 0, f->S64, f->twx64,
                                                            nullptr, nullptr, nullptr, gr,
                                                            nullptr);
       else if (kind == 2)
-        k_fft_cols<double><<<gc, 256, lc, s>>>(a, f->S64, f->twy64, f->lamx, f->lamy);
+        k_fft_cols<double><<<gc, fft_threads(a.Q * a.ny), lc, s>>>(a, f->S64, f->twy64, f->lamx, f->lamy);
       else
-        k_fft_rows_inv<double, double, 0><<<gr, 256, lr, s>>>(a, f->S64, f->twx64, buf,
+        k_fft_rows_inv<double, double, 0><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S64, f->twx64, buf,
                                                               nullptr);
     } else {
       float* buf = reinterpret_cast<float*>(f->bench_buf);
       if (kind == 0 || kind == 1)
-        k_fft_rows_fwd<float, float><<<gr, 256, lr, s>>>(
+        k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), lr, s>>>(
             a, buf, 0, f->S32, f->twx32, nullptr, kind == 1 ? f->x64 : nullptr,
             nullptr, gr, nullptr);
       else if (kind == 2)
-        k_fft_cols<float><<<gc, 256, lc, s>>>(a, f->S32, f->twy32, f->lamx, f->lamy);
+        k_fft_cols<float><<<gc, fft_threads(a.Q * a.ny), lc, s>>>(a, f->S32, f->twy32, f->lamx, f->lamy);
       else if (kind == 3)
-        k_fft_rows_inv<float, float, 0><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
+        k_fft_rows_inv<float, float, 0><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
                                                             nullptr);
       else if (kind == 4)
-        k_fft_rows_inv<float, float, 2><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
+        k_fft_rows_inv<float, float, 2><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
                                                             f->x64);
       else if (kind == 5)
-        k_fft_rows_fwd<float, float><<<gr, 256, lr, s>>>(
+        k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), lr, s>>>(
             a, buf, 0, f->S32, f->twx32, f->rowsum[1], nullptr,
             a.singular ? f->rowsum[0] : nullptr, gr, buf);
       else
-        k_fft_rows_inv<float, float, 3><<<gr, 256, lr, s>>>(a, f->S32, f->twx32, buf,
+        k_fft_rows_inv<float, float, 3><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
                                                             nullptr);
     }
   };

@@ -567,11 +567,12 @@ def test_poisson_fft_f32_warm(dev, L, ny, nx, maindiag, nref, bmean, per):
 
 @pytest.mark.parametrize("ny,nx,mode,maindiag", [
     (1024, 1024, "f64", 0.0), (1024, 1024, "refine", 0.0), (2048, 256, "f64", 0.5),
-    (256, 2048, "refine", 0.0), (1024, 2048, "f64", 0.0), (2048, 1024, "refine", 0.3)])
+    (256, 2048, "refine", 0.0), (1024, 2048, "f64", 0.0), (2048, 1024, "refine", 0.3),
+    (256, 4096, "refine", 0.0), (4096, 512, "refine", 0.2)])
 def test_poisson_fft_large_vs_dft(dev, L, ny, nx, mode, maindiag):
-    """Large direct solves (the wavefront row/column kernels, 256..2048
-    points a transform) against numpy's 2-D DFT closed form of the same
-    operator (oracle.fft_poisson_periodic)."""
+    """Large direct solves (256..4096 points a transform; 512 threads a
+    workgroup from 2048 points, 1024 from 4096) against numpy's 2-D DFT
+    closed form of the same operator (oracle.fft_poisson_periodic)."""
     import torch
     p = P(model="eulerpsi", nx=nx, ny=ny, xperiodic=True, yperiodic=True,
           Lx=nx / 512.0, Ly=ny / 700.0)
