@@ -58,6 +58,7 @@ def initial_omega(mesh, seed):
 
 
 HIPGRAPH = True  # --no-graph: eager launches (for per-dispatch counter passes)
+WINDOWS = 2  # timed windows per precision, interleaved
 
 
 def make_param(nx, precision, dt=0.0):
@@ -98,13 +99,28 @@ def calibrate(model, nsteps=3):
         model.time.pushforward()
 
 
-def timed(model, steps, warmup):
+def warm(model, warmup):
+    """`warmup` untimed steps, then the step graph of the buffer assignment
+    a timed window starts from is captured (if it is not yet) and replayed
+    once, untimed: a timed window replays graphs and captures none."""
+    import torch
+    integ = model.integrator
+    if warmup:
+        integ.device_steps(warmup)
+    integ.prepare_graph()
+    integ.device_steps(integ.graph_steps)
+    torch.cuda.synchronize()
+
+
+def timed_window(model, steps):
+    """Exactly `steps` steps between two HIP events, bracketed by device
+    synchronisations.  Raises if a step graph was captured inside."""
     import torch
     integ = model.integrator
     pois = model.mesh.poisson_vertices
-    if warmup:
-        integ.device_steps(warmup)
+    integ.prepare_graph()  # the buffer assignment this window starts from
     pois.reset_stats()
+    ngraphs = integ.graph_count()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -113,9 +129,15 @@ def timed(model, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1)
-    st = pois.read_stats()
-    return ms / steps, wall, st, dts
+    if integ.graph_count() != ngraphs:
+        raise RuntimeError("a step graph was captured inside the timed window")
+    return e0.elapsed_time(e1) / steps, wall, pois.read_stats(), dts
+
+
+def timed(model, steps, warmup):
+    """warm(), then one timed window."""
+    warm(model, warmup)
+    return timed_window(model, steps)
 
 
 def accuracy(nx, seed, nsteps):
@@ -356,9 +378,37 @@ def traffic_of(kernel, nx, prec):
     return None
 
 
-def cpu_baseline(nx, seed, nsteps):
-    """The oracle (C loops + SuperLU, the reference's algorithm) on the host,
-    single thread, a few steps of the same workload after factorisation."""
+def host_cores():
+    """The host threads this process may use (its CPU affinity), the machine's
+    cpu_count and the CPU model name (lscpu's "Model name")."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return usable, os.cpu_count(), model
+
+
+# at most this many host threads for the baseline: the CPU share one GPU's
+# process is given on the measurement machines
+MAX_CPU_THREADS = 16
+
+
+def cpu_baseline(nx, seed, nsteps, threads=1):
+    """The oracle (the reference's algorithm: C restatement of the numba loops,
+    -O3, OpenMP over the flat index as the analogue of prange, numpy
+    operators, scipy SuperLU) on the host, a few steps of the same workload
+    after factorisation, with `threads` threads in the loops (SuperLU and
+    numpy run on one)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     p = make_param(nx, "fp64")
@@ -371,6 +421,7 @@ def cpu_baseline(nx, seed, nsteps):
     b = B()
     for k in keys:
         setattr(b, k, getattr(p, k))
+    used = O.set_threads(threads)
     t0 = time.perf_counter()
     m = O.Model(b)
     m.mesh.poisson_vertices  # the SuperLU factorisation (made on first use)
@@ -380,7 +431,7 @@ def cpu_baseline(nx, seed, nsteps):
     t0 = time.perf_counter()
     m.run(nsteps)
     el = time.perf_counter() - t0
-    return nx * nx * nsteps / el, el, tfact
+    return nx * nx * nsteps / el, el, tfact, used
 
 
 CLOSED_NX = 256
@@ -427,10 +478,11 @@ def closed_bench(seed, steps):
     return out
 
 
-def cpu_baseline_closed(seed, steps):
-    """The oracle (C loops + SuperLU, one thread) on the closed 256^2 workload
-    for `steps` steps, and the device FP64 run of the same steps from the
-    same state checked against it (relative L2 of the vorticity)."""
+def cpu_baseline_closed(seed, steps, threads=(1,)):
+    """The oracle (C loops + SuperLU; the loops on each thread count of
+    `threads`) on the closed 256^2 workload for `steps` steps, and the device
+    FP64 run of the same steps from the same state checked against it
+    (relative L2 of the vorticity)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from fluids2d_amd import Model
@@ -449,24 +501,33 @@ def cpu_baseline_closed(seed, steps):
               "noslip", "f0", "beta", "g", "H", "tend", "dt", "maxite", "dtmax",
               "integrator", "cfl", "compflux", "vortexforce", "innerproduct", "maxorder"]:
         setattr(b, k, getattr(p, k))
-    t0 = time.perf_counter()
-    ref = O.Model(b)
-    ref.mesh.poisson_vertices  # the SuperLU factorisation (made on first use)
-    tfact = time.perf_counter() - t0
-    ref.state.omega[...] = w0
-    ref.diag(ref.state)
-    t0 = time.perf_counter()
-    ref.run(steps)
-    el = time.perf_counter() - t0
-    err = float(np.linalg.norm(m.state.omega - ref.state.omega) /
-                np.linalg.norm(ref.state.omega))
-    return {"value": CLOSED_NX ** 2 * steps / el, "unit": "cell-steps/s", "cores": 1,
+    runs = {}
+    for thr in threads:
+        used = O.set_threads(thr)
+        t0 = time.perf_counter()
+        ref = O.Model(b)
+        ref.mesh.poisson_vertices  # the SuperLU factorisation (made on first use)
+        tfact = time.perf_counter() - t0
+        ref.state.omega[...] = w0
+        ref.diag(ref.state)
+        t0 = time.perf_counter()
+        ref.run(steps)
+        el = time.perf_counter() - t0
+        err = float(np.linalg.norm(m.state.omega - ref.state.omega) /
+                    np.linalg.norm(ref.state.omega))
+        runs[used] = dict(value=CLOSED_NX ** 2 * steps / el, seconds=el, fact=tfact, err=err,
+                          ite=int(ref.time.ite))
+    best = max(runs, key=lambda t: runs[t]["value"])
+    r = runs[best]
+    return {"value": r["value"], "unit": "cell-steps/s", "cores": best,
             "kind": "port",
             "sample": f"{steps} RK3 steps of the closed {CLOSED_NX}^2 workload through the "
-                      f"oracle, one thread, {el:.1f} s timed; SuperLU factorisation "
-                      f"({tfact:.1f} s) excluded",
-            "gpu_fp64_vs_oracle_rel_l2_omega": err,
-            "iterations": [int(m.time.ite), int(ref.time.ite)]}
+                      f"oracle on {sorted(runs)} threads (value: the faster, "
+                      f"{r['seconds']:.1f} s); SuperLU factorisation excluded",
+            "by_threads": {str(t): {"value": v["value"], "seconds": round(v["seconds"], 2)}
+                           for t, v in runs.items()},
+            "gpu_fp64_vs_oracle_rel_l2_omega": max(v["err"] for v in runs.values()),
+            "iterations": [int(m.time.ite)] + [v["ite"] for v in runs.values()]}
 
 
 def main():
@@ -518,21 +579,35 @@ def main():
     nx = a.nx
     res = {}
     tiles = {}
-    for prec in (["fp64", "fp32"] if a.only == "both" else [a.only]):
+    precs = ["fp64", "fp32"] if a.only == "both" else [a.only]
+    models = {}
+    for prec in precs:
         m = build(nx, prec, a.seed)
         calibrate(m, 3)
         # the transport tile by measurement (results do not depend on it)
         tiles[prec] = autotune.tune_tile(m, cache=False)
-        msps, wall, st, dts = timed(m, a.steps, a.warmup)
-        if st["failed"] > 0:
-            raise SystemExit(f"{prec}: {st['failed']} Poisson solves did not converge "
-                             f"within the enqueued iterations")
+        warm(m, a.warmup)
+        models[prec] = m
+    # the precisions' windows interleaved, each exactly `steps` steps: a
+    # slower first measurement in the process cannot favour one precision
+    windows = {prec: [] for prec in precs}
+    for _ in range(WINDOWS):
+        for prec in precs:
+            windows[prec].append(timed_window(models[prec], a.steps))
+    for prec in precs:
+        m = models[prec]
+        ws = windows[prec]
+        for w in ws:
+            if w[2]["failed"] > 0:
+                raise SystemExit(f"{prec}: {w[2]['failed']} Poisson solves did not converge "
+                                 f"within the enqueued iterations")
+        msps = sum(w[0] for w in ws) / len(ws)
         prof = rl.profile_dominant(m, reps=20)
-        res[prec] = dict(ms=msps, wall=wall, stats=st, prof=prof,
-                         csps=nx * nx / (msps * 1e-3),
-                         t_end=float(dts.sum().item()))
-        del m
-        torch.cuda.empty_cache()
+        res[prec] = dict(ms=msps, windows_ms=[w[0] for w in ws], wall=sum(w[1] for w in ws),
+                         stats=ws[-1][2], prof=prof, csps=nx * nx / (msps * 1e-3),
+                         t_end=float(ws[-1][3].sum().item()))
+    del models, m
+    torch.cuda.empty_cache()
 
     acc_err = acc_diag = None
     if a.only == "both" and a.accuracy_steps > 0:
@@ -542,12 +617,26 @@ def main():
     head_prec = "fp32" if "fp32" in res else "fp64"
     cpu = None
     if not a.no_cpu_baseline:
-        cps, el, tf = cpu_baseline(nx, a.seed, a.cpu_steps)
-        cpu = {"value": cps, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+        usable, ncpu, cpu_model = host_cores()
+        nthr = max(1, min(usable, MAX_CPU_THREADS))
+        runs = {}
+        for thr in sorted({1, nthr}):
+            cps, el, tf, used = cpu_baseline(nx, a.seed, a.cpu_steps, thr)
+            runs[thr] = dict(value=cps, seconds=el, threads=used, factorisation_s=tf)
+        best = max(runs, key=lambda t: runs[t]["value"])
+        cpu = {"value": runs[best]["value"], "unit": "cell-steps/s",
+               "cores": runs[best]["threads"], "kind": "port",
                "sample": f"{a.cpu_steps} RK3 steps of the same {nx}^2 doubly periodic "
-                         f"Euler workload through the oracle (C restatement of the numba "
-                         f"loops + scipy SuperLU, one thread), {el:.1f} s timed; the "
-                         f"one-time SuperLU factorisation ({tf:.1f} s) excluded"}
+                         f"Euler workload through the oracle (the reference's algorithm: C "
+                         f"restatement of the numba loops at -O3 with OpenMP over the flat "
+                         f"index, numpy operators, scipy SuperLU), timed on 1 thread and on "
+                         f"{nthr}; value is the faster; the one-time SuperLU "
+                         f"factorisation excluded",
+               "by_threads": {str(t): {"value": r["value"], "seconds": round(r["seconds"], 2),
+                                       "factorisation_s": round(r["factorisation_s"], 2)}
+                              for t, r in runs.items()},
+               "host": {"usable_cpus": usable, "os_cpu_count": ncpu, "model_name": cpu_model,
+                        "thread_cap": MAX_CPU_THREADS}}
     prof = head["prof"]
     out = {
         "metric": "cell-steps/s at 1024² Euler, FP32+refine vs FP64; fwd+adjoint/fwd cost ratio",
@@ -578,6 +667,11 @@ def main():
                                  for k, v in prof["all"].items()}},
         "cpu_baseline": cpu,
     }
+    out["timing"] = {"windows": WINDOWS, "steps_per_window": a.steps,
+                     "note": "value and ms_per_step are the mean over the precision's "
+                             "windows; FP64 and FP32 windows alternate; every window "
+                             "replays step graphs captured before it",
+                     "windows_ms_per_step": {k: v["windows_ms"] for k, v in res.items()}}
     if "fp64" in res:
         out["fp64_cell_steps_per_s"] = res["fp64"]["csps"]
         out["fp64_ms_per_step"] = res["fp64"]["ms"]
@@ -600,7 +694,8 @@ def main():
     if a.closed_steps > 0:
         out["closed256"] = closed_bench(a.seed, a.closed_steps)
         if cpu is not None:
-            cpu["closed256"] = cpu_baseline_closed(a.seed, a.closed_steps)
+            cpu["closed256"] = cpu_baseline_closed(a.seed, a.closed_steps,
+                                                   sorted({1, cpu["cores"]} | {int(t) for t in cpu["by_threads"]}))
     if a.ensemble > 0:
         out["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps,
                                          log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -102,14 +102,45 @@ class Advection(_Base):
 
 
 class QG(_Base):
-    """Quasi-geostrophic PV transport (equations.py:285-299)."""
+    """Quasi-geostrophic PV transport (equations.py:285-299).
+
+    With beta != 0 the inversion subtracts the planetary vorticity
+    ``mesh.f``, a vertex array the caller sets on the mesh, as
+    experiments/inpreparation/gyre.py:24-29 does (operators.py:189-190).
+    It is copied to the device the first time a solve needs it, and again
+    whenever ``mesh.f`` is rebound to another array."""
     logs_dt = True  # stage(..., dt_log=) records the step's dt
     prognostic = ("pv",)
 
     def __init__(self, param, mesh):
         super().__init__(param, mesh)
-        if param.beta != 0:
-            raise NotImplementedError("qg with beta != 0 needs mesh.f on the device")
+        self._f_src = None
+        self._f_dev = None
+
+    def _planetary(self, like):
+        f = getattr(self.mesh, "f", None)
+        if f is None:
+            raise AttributeError("qg with beta != 0 needs mesh.f (the planetary "
+                                 "vorticity at the vertices, times the cell area)")
+        if f is not self._f_src or self._f_dev is None or self._f_dev.dtype != like.dtype:
+            import numpy as np
+            import torch
+            if isinstance(f, torch.Tensor):
+                d = f.to(device=like.device, dtype=like.dtype)
+            else:
+                a = np.broadcast_to(np.asarray(f, dtype=np.float64), tuple(like.shape))
+                d = torch.from_numpy(np.ascontiguousarray(a)).to(like.device).to(like.dtype)
+            self._f_src, self._f_dev = f, d.contiguous()
+        return self._f_dev
+
+    def _inversion_rhs(self, s):
+        """work = centerstovertices(pv - hb*qgcoef) - f (operators.py:186-190;
+        hb = 0 on the device, so pv - 0 is pv bit for bit)."""
+        m = self.mesh
+        op.centerstovertices(m, s.pv, s.work)
+        if self.param.beta != 0:
+            # work + (-1 * f) is work - f exactly
+            op.addto_terms(s.work, [self._planetary(s.work)], [-1.0])
 
     def rhs(self, s, ds):
         m, p = self.mesh, self.param
@@ -118,15 +149,16 @@ class QG(_Base):
         op.transport_rhs(m, p.compflux, s.pv, s.U, ds.pv)
 
     def diag(self, s):
-        # qg_inversion (operators.py:186-191) with hb = 0, beta = 0
+        # qg_inversion (operators.py:186-191) with hb = 0, then perpgrad
         m = self.mesh
-        op.centerstovertices(m, s.pv, s.work)
+        self._inversion_rhs(s)
         m.qg_helmholtz.solve(s.work, s.psi)
         op.perpgrad(m, s.psi, s.U, contravariant=True)
 
     def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None):
         """As EulerPsi.stage: U stored by the last stage only, formed from psi
-        inside the later stages' transport."""
+        inside the later stages' transport.  With beta = 0 the solve forms
+        its right-hand side itself (centerstovertices on the fly)."""
         m, p = self.mesh, self.param
         if k > 0:
             op.transport_stage_psi(m, p.compflux, s.pv, alt.pv, s.psi, True, False,
@@ -135,14 +167,18 @@ class QG(_Base):
             op.transport_stage(m, p.compflux, s.pv, alt.pv, s.U, ds_prev, ds_out,
                                p.integrator, k, dt, dt_log)
         s, alt = s._replace(pv=alt.pv), alt._replace(pv=s.pv)
-        m.qg_helmholtz.solve_c2v(s.pv, s.psi, stats=stats)
+        if p.beta != 0:
+            self._inversion_rhs(s)
+            m.qg_helmholtz.solve(s.work, s.psi, stats=stats)
+        else:
+            m.qg_helmholtz.solve_c2v(s.pv, s.psi, stats=stats)
         if k == _lib.NSTAGES[p.integrator] - 1:
             op.perpgrad(m, s.psi, s.U, contravariant=True, fill=False, dt_dev=dt_next,
                         cfl=p.cfl, dtmax=p.dtmax)
         return s, alt
 
     def materialize(self, s):
-        op.centerstovertices(self.mesh, s.pv, s.work)
+        self._inversion_rhs(s)
 
 
 MODELS = {"eulerpsi": EulerPsi, "advection": Advection, "qg": QG}

@@ -258,6 +258,27 @@ class RKIntegrator:
             n += 1
         return dts
 
+    def prepare_graph(self):
+        """Run the eager steps a capture needs and capture the step graph of
+        the current buffer assignment now, so that a following
+        ``device_steps`` replays only (nothing is captured inside it when its
+        step count is a multiple of ``graph_steps``).  Returns the number of
+        steps taken (0 to 2 eager steps; the capture itself advances
+        nothing)."""
+        n = 0
+        while self._eager_steps < 2 or (self._wants_dtn() and not self._dtn_valid):
+            self._select_dt()
+            self.advance(stats=False)
+            self._eager_steps += 1
+            n += 1
+        self._graph()
+        return n
+
+    def graph_count(self):
+        """Number of step graphs captured so far (for callers that check that
+        a timed region captured none)."""
+        return len(self._graphs)
+
     # -- HIP graphs -----------------------------------------------------------------
     graph_steps = 4  # steps per captured graph; even, so ping-pong buffers return
 

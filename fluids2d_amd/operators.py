@@ -79,6 +79,17 @@ def rk_addto(y, xs, integrator, stage, dt_dev):
          _lib.INTEGRATORS[integrator], stage, ptr(dt_dev), stream_ptr())
 
 
+def addto_terms(y, xs, coefs):
+    """y += ((c0 x0 + c1 x1) + c2 x2) over every entry, for 1 to 3 terms
+    (integrators.py:154-196 with constant coefficients)."""
+    n = len(xs)
+    assert 1 <= n <= 3 and len(coefs) == n
+    xs = list(xs) + [None] * (3 - n)
+    cs = [float(c) for c in coefs] + [0.0] * (3 - n)
+    call("f2d_addto", _dt(y), ptr(y), ptr(xs[0]), ptr(xs[1]), ptr(xs[2]), y.numel(), n,
+         cs[0], cs[1], cs[2], None, 0, stream_ptr())
+
+
 def centerstovertices(mesh, omega, vomega, addto=False):
     """operators.py:126-133"""
     call("f2d_c2v", _dt(omega), mesh.grid_ref(), ptr(omega), ptr(vomega),

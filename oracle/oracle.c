@@ -12,9 +12,16 @@
  *   - Python's left-to-right `a + b + c` is ((a + b) + c),
  *   - numba wraps negative array indices (q[i-3*s] with i-3*s < 0).
  *
- * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fPIC -shared).
+ * The three flat loops run under OpenMP over contiguous index ranges, the
+ * analogue of the reference's optional numba prange split
+ * (parallel.py:7-16) without its tail drop: every index is independent, so
+ * the results do not depend on the thread count (orc_set_threads).
+ *
+ * Build: see oracle/Makefile (gcc -O3 -fopenmp -ffp-contract=off -fPIC
+ * -shared; no -ffast-math, so no reassociation).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -187,6 +194,7 @@ static inline int64_t W(int64_t i, int64_t n) { return i < 0 ? i + n : i; }
 void orc_compflux(double* flx, const double* U, const double* q,
                   const int8_t* o, int64_t n, int64_t s, int method,
                   int64_t i0, int64_t i1) {
+#pragma omp parallel for schedule(static)
   for (int64_t i = i0; i < i1; i++) {
     if (o[i] > 4) {
       flx[i] = orc_f5(method, U[i], q[W(i - 3 * s, n)], q[W(i - 2 * s, n)],
@@ -209,6 +217,7 @@ void orc_vortexforce(double* du, const double* V, const double* q,
                      const int8_t* o, int64_t n, int64_t s, int64_t s2,
                      int sign, int method, int64_t i0, int64_t i1) {
   const double sg = (double)sign;
+#pragma omp parallel for schedule(static)
   for (int64_t i = i0; i < i1; i++) {
     if (o[i] > 0) {
       double Vm = 0.25 * (((V[i] + V[i + s]) + V[W(i - s2, n)]) +
@@ -232,6 +241,7 @@ void orc_vortexforce(double* du, const double* V, const double* q,
 void orc_innerproduct(double* ke, const double* U, const double* q,
                       const int8_t* o, int64_t n, int64_t s, int method,
                       int64_t i0, int64_t i1) {
+#pragma omp parallel for schedule(static)
   for (int64_t i = i0; i < i1; i++) {
     if (o[i] > 0) {
       double Um = 0.5 * (U[i] + U[i + s]);
@@ -343,4 +353,10 @@ int64_t orc_laplacian_coo(const int32_t* G, int64_t ny, int64_t nx, double dx,
   return c;
 }
 
-int orc_version(void) { return 1; }
+int orc_version(void) { return 2; }
+
+/* threads of the OpenMP loops above (n >= 1); returns the count in use */
+int orc_set_threads(int n) {
+  if (n >= 1) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}

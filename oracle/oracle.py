@@ -78,8 +78,16 @@ def lib():
         L.orc_laplacian_coo.argtypes = [i32p, i64, i64, d, d, d, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         i32p, i32p, dp]
+        L.orc_set_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the C flat loops (OpenMP; results do not depend on it).
+    Returns the count in use."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def _p(a, ct):
@@ -526,9 +534,14 @@ def qg_projection(mesh, u, h, pv, psi, anomaly=False):
 
 
 def qg_inversion(mesh, pv, work, psi):
-    """operators.py:186-191 (beta = 0)"""
+    """operators.py:186-191.  With beta != 0 the planetary vorticity
+    ``mesh.f`` (a vertex array the caller sets, as
+    experiments/inpreparation/gyre.py:24-29 does) is subtracted from the
+    whole work array before the solve."""
     pvback = mesh.hb * mesh.qgcoef
     centerstovertices(mesh, pv - pvback, work)
+    if mesh.param.beta != 0:
+        work -= mesh.f
     mesh.qg_helmholtz.solve(work, psi)
 
 

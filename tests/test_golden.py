@@ -18,9 +18,15 @@ def _load(name):
 
 
 def test_kat_fixture_is_the_references():
+    """tests/test_models.py asserts 37 (advection) and 25 (euler); the
+    reference's code, run as its test runs it, reaches the euler tend in 22
+    (reference_vectors.npz), the count the oracle and the device follow."""
     k = json.load(open(os.path.join(G, "kat.json")))
     assert k["advection_bodyrotation_tend10_ite"] == 37
     assert k["euler_dipole_tend10_ite"] == 25
+    v = np.load(os.path.join(G, "reference_vectors.npz"))
+    assert k["euler_dipole_tend10_ite_as_executed"] == int(v["kat_euler_ite"]) == 22
+    assert int(v["kat_advection_ite"]) == 37
 
 
 def test_oracle_reproduces_weno_points():

@@ -232,40 +232,14 @@ def euler_dipole(O, p):
     return m, omega
 
 
-def test_euler_kat_as_written_takes_one_step():
-    """tests/test_models.py:8-15 literally: no diag before run, so the first
-    CFL step (model.py:84-87) sees U = 0 and dt = min(0.9 / 1e-99, dtmax) =
-    9e98; that one step ends the run.  The current source cannot give the
-    test's 25 iterations (an older version's count)."""
-    p = P(model="euler", tend=10)
-    m = O.Model(p)
-    mesh = m.mesh
-    x, y = mesh.xy("v")
-
-    def g(x0, y0, r):
-        return np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * r ** 2))
-    omega = (g(0.55, 0.5, 0.05) - g(0.45, 0.5, 0.05)) * mesh.mskv * mesh.area
-    psi = np.zeros(mesh.shape)
-    mesh.poisson_vertices.solve(omega, psi)
-    O.perpgrad(mesh, psi, m.state.u)
-    m.state.omega[...] = omega
-    assert not m.state.U.x.any() and not m.state.U.y.any()
-    with np.errstate(all="ignore"):
-        m.run()
-    assert m.time.ite == 1 and m.time.dt == 0.9 / 1e-99
-
-
 def test_euler_dipole_iterations():
-    """The case of the reference's KAT test_euler (tests/test_models.py:8-15:
-    default Param, euler, the vortex.py dipole, tend = 10).  That test
-    expects 25 iterations; the reference's current source, restated
-    operator by operator here (equations.py:9-24, operators.py:6-119,
-    elliptic.py:114-195, integrators.py:92-107, model.py:71-87), reaches
-    tend in 22, with dt from 0.446 to 0.482.  The count 25 most likely dates
-    from an earlier grid layout (meshes.py:114-130 keeps the commented-out
-    non-periodic shape without halo); without numba the reference cannot be
-    run here, so this KAT is parity unpinned.  The test freezes the
-    restatement's count, which the device model must match."""
+    """The reference's KAT test_euler (tests/test_models.py:8-15: default
+    Param, euler, the dipole of experiments/vortex.py:8-42, which ends with a
+    diag, then run to tend = 10).  That test asserts 25 iterations, but the
+    reference's own code, run exactly as the test runs it, takes 22 (dt from
+    0.4456 to 0.4824; tests/golden/reference_vectors.npz, written by
+    tests/golden/make_reference_vectors.py, with the dt sequence compared in
+    tests/test_reference_vectors.py).  The oracle takes the same 22."""
     import oracle as O
     from helpers import P
     m, _ = euler_dipole(O, P(model="euler", tend=10))
