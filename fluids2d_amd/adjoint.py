@@ -110,6 +110,9 @@ class EulerAdjoint:
         self.work = torch.zeros((12,) + self.shape, dtype=self.dtype, device=dev)
         self.rgraphs = {}
         self._warm = False
+        # bumped by every forward: the reverse sweep reads the checkpoints
+        # and the kept segment of the latest forward only
+        self.generation = 0
         dt_ = self.dt
         # integrators.py:98-106, in Python's evaluation order
         self.c1 = (dt_,)
@@ -178,6 +181,7 @@ class EulerAdjoint:
     def forward(self, omega0):
         """omega0: padded device tensor (halo ignored); returns omega_T."""
         self._warmup()
+        self.generation += 1
         lane, seg = self.lanes[0], self.segs[0]
         s0 = seg.s(0, 0)
         s0.copy_(omega0.detach().to(self.dtype))
@@ -282,15 +286,30 @@ class EulerAdjoint:
         return out
 
 
+class StaleTrajectory(RuntimeError):
+    """The runner ran another forward after the one a backward belongs to."""
+
+
 class EulerGradient(torch.autograd.Function):
     """omega_T = EulerGradient.apply(omega0, runner): the forward run of an
-    EulerAdjoint; its backward is the hand-written reverse sweep."""
+    EulerAdjoint; its backward is the hand-written reverse sweep.
+
+    A runner holds the checkpoints of one trajectory, its latest forward.
+    The backward of an earlier apply on the same runner (another apply, or
+    a forward, came in between) raises StaleTrajectory instead of returning
+    the latest trajectory's gradient; use one runner per live graph."""
 
     @staticmethod
     def forward(ctx, omega0, runner):
+        out = runner.forward(omega0)
         ctx.runner = runner
-        return runner.forward(omega0)
+        ctx.generation = runner.generation
+        return out
 
     @staticmethod
     def backward(ctx, grad_out):
+        if ctx.runner.generation != ctx.generation:
+            raise StaleTrajectory(
+                f"the adjoint runner's checkpoints are of forward {ctx.runner.generation}; "
+                f"this backward belongs to forward {ctx.generation}")
         return ctx.runner.backward(grad_out), None

@@ -99,9 +99,9 @@ class Bulk:
 def create_file(ncfile, param, variables):
     """diagnostics.py:79-92"""
     from .ncfile import Dataset
-    from .io import get_atts_from_param
+    from .io import param_attributes
     with Dataset(ncfile, "w", format="NETCDF4") as nc:
-        nc.setncatts(get_atts_from_param(param))
+        nc.setncatts(param_attributes(param))
         nc.createDimension("t", None)
         for name in variables:
             v = nc.createVariable(name, "f", ("t",))

@@ -20,14 +20,12 @@ import json
 
 import numpy as np
 
-from .io import get_atts_from_param
 from .states import vectors
 
 
 def get_keys_from_param(param):
-    """restart.py:5-9"""
-    exceptions = ["__parameters__"]
-    return [key for key in param.__dict__ if key not in exceptions]
+    """restart.py:5-9: the names of the parameters a history file stores."""
+    return param.parameters()
 
 
 def read_param(ncfile):
@@ -164,7 +162,10 @@ def load_checkpoint(path):
         p = Param()
     finally:
         fp.QUIET = quiet
+    known = set(p.parameters())
     for k, v in atts.items():
+        if k not in known:
+            p.add_parameter(k)
         setattr(p, k, v)
     model = Model(p)
     restore_checkpoint(model, path)

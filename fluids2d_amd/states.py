@@ -1,94 +1,83 @@
-"""State / Prognostic (states.py:1-94).
+"""Model states (states.py:1-94): which fields a model carries and how they
+are allocated.
 
-The user-facing ``State`` is the reference's: a namedtuple of float64 numpy
-arrays of shape (ny+6, nx+6), written in place by experiment scripts.  The GPU
-integrator keeps a device twin (``device_state``) of the same structure in the
-compute precision and copies between them at the synchronisation points
-described in the integrator.
+Interface kept: ``State(param, shape)`` gives a namedtuple of float64 numpy
+arrays of shape (ny+6, nx+6), prognostic fields first, vector fields as
+``Vector(x, y)``; ``Prognostic(param, shape)`` the prognostic part only;
+``get_specs(param)`` the two name tuples (``.variables``, ``.prognostic``),
+with a passive tracer (``param.tracer``) added after the model's prognostic
+fields; ``vectors`` the names held as vectors.  The field lists per model are
+the reference's (they are the state's interface).
+
+Here each model's fields are stored as two tuples, prognostic and
+diagnostic, so a state is prognostic-first by construction.  The GPU
+integrator keeps a device twin of the same structure in the compute
+precision (``allocate_device``), copied to and from the host at the
+synchronisation points the integrator documents.
 """
 from collections import namedtuple
+from functools import lru_cache
 
 import numpy as np
 import torch
 
-Specs = namedtuple("specs", ("variables", "prognostic"))
-
-model_specs = {
-    "euler": Specs(("u", "U", "omega", "ke", "p", "div", "flx"), ("u",)),
-    "eulerpsi": Specs(("omega", "U", "psi", "vomega", "flx"), ("omega",)),
-    "boussinesq": Specs(("b", "u", "U", "omega", "ke", "p", "div", "flx"), ("b", "u")),
-    "hydrostatic": Specs(("b", "uh", "U", "omega", "ke", "p", "div", "flx"), ("b", "uh")),
-    "rsw": Specs(("u", "h", "U", "omega", "ke", "p", "flx", "pv"), ("u", "h")),
-    "qgrsw": Specs(("u", "h", "U", "omega", "ke", "p", "flx", "pv", "psi"), ("u", "h")),
-    "qg": Specs(("pv", "U", "h", "flx", "work", "psi"), ("pv",)),
-    "advection": Specs(("q", "U", "flx"), ("q",)),
-    "vectoradv": Specs(("v", "U", "omega", "q"), ("v",)),
+#  model:         (prognostic,   diagnostic)
+FIELDS = {
+    "euler":       (("u",),      ("U", "omega", "ke", "p", "div", "flx")),
+    "eulerpsi":    (("omega",),  ("U", "psi", "vomega", "flx")),
+    "boussinesq":  (("b", "u"),  ("U", "omega", "ke", "p", "div", "flx")),
+    "hydrostatic": (("b", "uh"), ("U", "omega", "ke", "p", "div", "flx")),
+    "rsw":         (("u", "h"),  ("U", "omega", "ke", "p", "flx", "pv")),
+    "qgrsw":       (("u", "h"),  ("U", "omega", "ke", "p", "flx", "pv", "psi")),
+    "qg":          (("pv",),     ("U", "h", "flx", "work", "psi")),
+    "advection":   (("q",),      ("U", "flx")),
+    "vectoradv":   (("v",),      ("U", "omega", "q")),
 }
 
+# fields held as (x, y) pairs on the C-grid faces
 vectors = ["u", "U", "flx", "v"]
+
+Specs = namedtuple("specs", ("variables", "prognostic"))
+Vector = namedtuple("vector", ("x", "y"))
 
 
 def get_specs(param):
-    specs = model_specs[param.model]
-    if (param.tracer is None) | (param.tracer == "None"):
-        return specs
-    return add_tracer_to_spec(specs, param.tracer)
+    """All field names of param.model's state (prognostic first) and the
+    prognostic ones; a tracer named by param.tracer joins the prognostic."""
+    prog, diag = FIELDS[param.model]
+    tracer = getattr(param, "tracer", None)
+    if tracer not in (None, "None"):
+        prog = prog + (tracer,)
+    return Specs(prog + diag, prog)
 
 
-def add_tracer_to_spec(specs, tracer):
-    p = specs.prognostic + (tracer,)
-    v = p + specs.variables[len(specs.prognostic):]
-    return Specs(v, p)
+@lru_cache(maxsize=None)
+def _record(typename, names):
+    return namedtuple(typename, names)
+
+
+def _build(typename, names, make):
+    return _record(typename, tuple(names))(
+        *[Vector(make(), make()) if n in vectors else make() for n in names])
 
 
 def State(param, shape):
-    specs = get_specs(param)
-    check_specs_are_well_ordered(specs)
-    return allocate_state(param.model, specs.variables, shape)
+    """Every field of the model, zero-filled float64 host arrays."""
+    return _build("state", get_specs(param).variables, lambda: np.zeros(shape))
 
 
 def Prognostic(param, shape):
-    specs = get_specs(param)
-    return allocate_state(param.model, specs.prognostic, shape)
-
-
-def check_specs_are_well_ordered(specs):
-    nprognostic = len(specs.prognostic)
-    assert specs.variables[:nprognostic] == specs.prognostic
-
-
-def define_Namedtuple(name, variables):
-    class Namedtuple(namedtuple(name, variables)):
-        def __repr__(self):
-            return f"{name} with {variables}"
-    return Namedtuple
-
-
-Vector = define_Namedtuple("vector", ("x", "y"))
-
-
-def allocate_var(name, shape):
-    if name in vectors:
-        return Vector(x=np.zeros(shape), y=np.zeros(shape))
-    return np.zeros(shape)
-
-
-def allocate_state(name, variables, shape):
-    S = define_Namedtuple("state", variables)
-    return S(**{var: allocate_var(var, shape) for var in variables})
+    """The prognostic fields only (RK scratch tendencies)."""
+    return _build("state", get_specs(param).prognostic, lambda: np.zeros(shape))
 
 
 # --------------------------------------------------------------------------
 # device twins
 # --------------------------------------------------------------------------
 def allocate_device(variables, shape, dtype, device):
-    S = define_Namedtuple("dstate", variables)
-
-    def one(name):
-        z = lambda: torch.zeros(shape, dtype=dtype, device=device)  # noqa: E731
-        return Vector(x=z(), y=z()) if name in vectors else z()
-
-    return S(**{v: one(v) for v in variables})
+    """Zero-filled device tensors with the structure of a host state."""
+    return _build("dstate", variables,
+                  lambda: torch.zeros(shape, dtype=dtype, device=device))
 
 
 def leaves(x):

@@ -75,31 +75,89 @@ def test_adjoint_matches_torch_autograd(dev):
 
 
 def test_adjoint_finite_differences(dev):
-    """Central differences along 8 seeded unit directions.  The map is only
-    piecewise smooth (the |beta1 - beta3| of WENO-Z and the upwind switch at
-    U = 0 have kinks), so the step must be small: eps = 3e-9 relative to the
-    field, with the loss increment formed from the difference of the two
-    final states, (w+ - w-).(w+ + w- - 2 truth) / 2, to keep FP64 rounding
-    below the comparison."""
+    """Central differences along 8 seeded unit directions, each within 1e-6
+    of the adjoint's directional derivative.
+
+    The map is only piecewise smooth: the |beta1 - beta3| of WENO-Z
+    (weno.py:188) and the upwind switch at U = 0 (weno.py:325) have kinks,
+    and along a direction whose +-eps segment crosses one, central
+    differences average two one-sided derivatives that the adjoint (the
+    exact transpose of the tangent at w0) does not.  Directions are
+    therefore drawn from a seeded stream and kept only when they stay off
+    the kinks: their forward and backward one-sided differences agree to
+    1e-7.  eps = 3e-9 relative to the field; loss increments are formed from
+    state differences, (wa - wb).(wa + wb - 2 truth) / 2, so FP64 rounding
+    stays below the comparison."""
     runner, w0, wt, dt = _setup(128, "fp64", 20, 5)
     L, g = _loss_and_grad(runner, w0, wt)
     rng = np.random.default_rng(3)
     scale = float(w0[3:-3, 3:-3].abs().max())
-    errs = []
+    eps = 3e-9 * scale * 128
     tt = wt[3:-3, 3:-3]
-    for k in range(8):
+    w00 = runner.forward(w0)[3:-3, 3:-3]
+
+    def dL(wa, wb):
+        return 0.5 * float(((wa - wb) * ((wa - tt) + (wb - tt))).sum())
+
+    errs, rejected = [], 0
+    for k in range(32):
+        if len(errs) == 8:
+            break
         d = torch.zeros_like(w0)
         d[3:-3, 3:-3] = torch.from_numpy(rng.standard_normal((128, 128))).to(w0.device)
         d /= torch.linalg.norm(d)
-        eps = 3e-9 * scale * 128
         wp = runner.forward(w0 + eps * d)[3:-3, 3:-3]
         wm = runner.forward(w0 - eps * d)[3:-3, 3:-3]
-        fd = 0.5 * float(((wp - wm) * ((wp - tt) + (wm - tt))).sum()) / (2 * eps)
+        fplus, fminus = dL(wp, w00) / eps, dL(w00, wm) / eps
+        fd = dL(wp, wm) / (2 * eps)
         ad = float((g * d).sum())
+        kinked = abs(fplus - fminus) > 1e-7 * abs(fd)
+        print(f"direction {k}: adjoint {ad:.12e} FD {fd:.12e} rel {abs(ad - fd) / abs(fd):.1e}"
+              f" one-sided gap {abs(fplus - fminus) / abs(fd):.1e}"
+              + (" (crosses a kink: skipped)" if kinked else ""))
+        if kinked:
+            rejected += 1
+            continue
         errs.append(abs(ad - fd) / abs(fd))
-        print(f"direction {k}: adjoint {ad:.12e} FD {fd:.12e} rel {errs[-1]:.1e}")
-    assert max(errs) <= 2e-6, errs
-    assert float(np.median(errs)) <= 1e-6, errs
+    print(f"{len(errs)} directions kept, {rejected} crossing a kink skipped")
+    assert len(errs) == 8
+    assert max(errs) <= 1e-6, errs
+
+
+def test_stale_trajectory_refused(dev):
+    """Two applies on one runner before a backward: the first one's backward
+    must not silently use the second trajectory's checkpoints."""
+    from fluids2d_amd.adjoint import EulerGradient, StaleTrajectory
+    runner, w0, wt, dt = _setup(64, "fp64", 4, 2)
+    xa = w0.clone().requires_grad_(True)
+    xb = (1.1 * w0).clone().requires_grad_(True)
+    wa = EulerGradient.apply(xa, runner)
+    wb = EulerGradient.apply(xb, runner)
+    with pytest.raises(StaleTrajectory):
+        (0.5 * ((wa - wt)[3:-3, 3:-3] ** 2).sum()).backward()
+    # the latest one still works and equals a fresh runner's gradient
+    (0.5 * ((wb - wt)[3:-3, 3:-3] ** 2).sum()).backward()
+    fresh, _, _, _ = _setup(64, "fp64", 4, 2, dt=dt)
+    _, gb = _loss_and_grad(fresh, 1.1 * w0, wt)
+    assert torch.equal(xb.grad.double(), gb)
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 1e-9), ("fp32", 1e-3)])
+def test_adjoint_512_against_torch_autograd(dev, precision, bar):
+    """The gradient at the adjoint workload's size (512^2 doubly periodic),
+    over 4 steps with checkpoints every 2, against the FP64 torch-autograd
+    gradient of the same discretisation."""
+    runner, w0, wt, dt = _setup(512, precision, 4, 2)
+    L, g = _loss_and_grad(runner, w0, wt)
+    h = 1.0 / 512
+    x = w0[3:-3, 3:-3].cpu().clone().requires_grad_(True)
+    wT = TE.run(x, 4, dt, h, h)
+    Lr = 0.5 * ((wT - wt[3:-3, 3:-3].cpu()) ** 2).sum()
+    Lr.backward()
+    gr = x.grad
+    e = float(torch.linalg.norm(g[3:-3, 3:-3].cpu() - gr) / torch.linalg.norm(gr))
+    print(f"\n512^2 {precision} adjoint vs FP64 torch autograd: rel L2 {e:.2e} (bar {bar})")
+    assert e <= bar, e
 
 
 def test_checkpoint_interval_does_not_change_gradient(dev):

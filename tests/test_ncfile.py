@@ -61,7 +61,7 @@ def test_read_param_splits_lists(tmp_path):
     """restart.read_param (restart.py:12-19) on a classic file: lists of
     names come back as lists, None as None, numbers at full precision."""
     import fluids2d_amd.param as fp
-    from fluids2d_amd.io import get_atts_from_param
+    from fluids2d_amd.io import param_attributes
     from fluids2d_amd.restart import read_param
     fp.QUIET = True
     p = fp.Param()
@@ -69,7 +69,7 @@ def test_read_param_splits_lists(tmp_path):
     p.var_to_store, p.noslip, p.xperiodic = ["u", "h"], ["bottom", "top"], True
     path = str(tmp_path / "h.nc")
     with ClassicDataset(path, "w") as nc:
-        nc.setncatts(get_atts_from_param(p))
+        nc.setncatts(param_attributes(p))
     q = read_param(path)
     assert (q.model, q.nx, q.ny, q.cfl, q.dtmax) == ("qgrsw", 96, 40, 0.7, 0.0123456789012345)
     assert q.var_to_store == ["u", "h"] and q.noslip == ["bottom", "top"]
