@@ -672,19 +672,33 @@ __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
       if (a.yper) js = (j < 3) ? j + a.ny : (j >= a.ny + 3 ? j - a.ny : j);
     }
     int64_t ks = (int64_t)js * NX + is;
-    // the last row of Ux / column of Uy is outside numpy's slices: it keeps
-    // its value (0 unless written by the caller), scaled like the rest
-    T ux = (js < NY - 1)
-               ? (-(a.psi[ks + NX] - a.psi[ks])) * T(a.mx[ks])
-               : a.Ux[ks];
-    T uy = (is < NX - 1) ? (a.psi[ks + 1] - a.psi[ks]) * T(a.my[ks])
-                         : a.Uy[ks];
-    ux = ux * T(a.sx);
-    uy = uy * T(a.sy);
-    a.Ux[k] = ux;
-    a.Uy[k] = uy;
-    mxv = nanmax(mxv, fabs((double)ux));
-    myv = nanmax(myv, fabs((double)uy));
+    // The last row of Ux / column of Uy is outside numpy's slices: it keeps
+    // its value (0 unless written by the caller), scaled like the rest, and
+    // its periodic images take the scaled value.  Such an entry is scaled in
+    // place by its own thread, which also writes its images; an image's
+    // thread leaves it alone (reading the source there would race with the
+    // source's own update).  Images equal their sources, so the maxima of
+    // the sources are those of every entry.
+    if (js < NY - 1 || ks == k) {
+      T ux = (js < NY - 1) ? (-(a.psi[ks + NX] - a.psi[ks])) * T(a.mx[ks]) : a.Ux[k];
+      ux = ux * T(a.sx);
+      a.Ux[k] = ux;
+      if (js == NY - 1 && a.fill && a.xper) {
+        if (i >= 3 && i < 6) a.Ux[k + a.nx] = ux;
+        if (i >= a.nx && i < a.nx + 3) a.Ux[k - a.nx] = ux;
+      }
+      mxv = nanmax(mxv, fabs((double)ux));
+    }
+    if (is < NX - 1 || ks == k) {
+      T uy = (is < NX - 1) ? (a.psi[ks + 1] - a.psi[ks]) * T(a.my[ks]) : a.Uy[k];
+      uy = uy * T(a.sy);
+      a.Uy[k] = uy;
+      if (is == NX - 1 && a.fill && a.yper) {
+        if (j >= 3 && j < 6) a.Uy[k + (int64_t)a.ny * NX] = uy;
+        if (j >= a.ny && j < a.ny + 3) a.Uy[k - (int64_t)a.ny * NX] = uy;
+      }
+      myv = nanmax(myv, fabs((double)uy));
+    }
   }
   if (!a.partials) return;
   const double bx = block_nanmax256(mxv, sh);
@@ -824,6 +838,10 @@ static double* red_scratch(int nblocks, void* stream) {
   return r.partials;
 }
 
+// workgroups of k_perpgrad over n padded points: a multiple of 8 (its chunk
+// order); the surplus chunks are empty and report maxima of 0
+static int perpgrad_blocks(int64_t n) { return (grid1d(n, 256 * PG_PTS) + 7) & ~7; }
+
 }  // namespace f2d
 
 using namespace f2d;
@@ -850,6 +868,31 @@ static void check_grid(const f2d_grid* g) {
 
 static void check_method(int m) {
   F2D_REQUIRE(m >= F2D_WENO && m <= F2D_CWENO, "unknown flux method");
+}
+
+static void launch_perpgrad(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                            void* Uy, const int8_t* mskx, const int8_t* msky,
+                            double sx, double sy, int fill, double* part, int nb,
+                            double* maxabs, double* dt, double cfl, double dtmax,
+                            void* stream) {
+  F2D_DTYPE_DISPATCH(dtype, {
+    PerpArgs<T> a{};
+    a.psi = (const T*)psi;
+    a.Ux = (T*)Ux;
+    a.Uy = (T*)Uy;
+    a.mx = mskx;
+    a.my = msky;
+    a.ny = (int)g->ny;
+    a.nx = (int)g->nx;
+    a.xper = g->xperiodic;
+    a.yper = g->yperiodic;
+    a.fill = fill;
+    a.sx = sx;
+    a.sy = sy;
+    a.partials = part;
+    k_perpgrad<T><<<nb, 256, 0, S(stream)>>>(a);
+  });
+  if (part) k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, maxabs, dt, cfl, dtmax);
 }
 
 extern "C" {
@@ -1036,29 +1079,10 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
   return guarded([&] {
     check_grid(g);
     int64_t n = (g->ny + 6) * (g->nx + 6);
-    // a multiple of 8 workgroups (k_perpgrad's chunk order); the surplus
-    // chunks are empty and report maxima of 0
-    int nb = (grid1d(n, 256 * PG_PTS) + 7) & ~7;
+    int nb = perpgrad_blocks(n);
     double* part = (maxabs || dt) ? red_scratch(nb, stream) : nullptr;
-    F2D_DTYPE_DISPATCH(dtype, {
-      PerpArgs<T> a{};
-      a.psi = (const T*)psi;
-      a.Ux = (T*)Ux;
-      a.Uy = (T*)Uy;
-      a.mx = mskx;
-      a.my = msky;
-      a.ny = (int)g->ny;
-      a.nx = (int)g->nx;
-      a.xper = g->xperiodic;
-      a.yper = g->yperiodic;
-      a.fill = fill;
-      a.sx = sx;
-      a.sy = sy;
-      a.partials = part;
-      k_perpgrad<T><<<nb, 256, 0, S(stream)>>>(a);
-    });
-    if (part)
-      k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, maxabs, dt, cfl, dtmax);
+    launch_perpgrad(dtype, g, psi, Ux, Uy, mskx, msky, sx, sy, fill, part, nb, maxabs,
+                    dt, cfl, dtmax, stream);
   });
 }
 

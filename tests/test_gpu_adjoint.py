@@ -78,48 +78,48 @@ def test_adjoint_finite_differences(dev):
     """Central differences along 8 seeded unit directions, each within 1e-6
     of the adjoint's directional derivative.
 
-    The map is only piecewise smooth: the |beta1 - beta3| of WENO-Z
-    (weno.py:188) and the upwind switch at U = 0 (weno.py:325) have kinks,
-    and along a direction whose +-eps segment crosses one, central
-    differences average two one-sided derivatives that the adjoint (the
-    exact transpose of the tangent at w0) does not.  Directions are
-    therefore drawn from a seeded stream and kept only when they stay off
-    the kinks: their forward and backward one-sided differences agree to
-    1e-7.  eps = 3e-9 relative to the field; loss increments are formed from
-    state differences, (wa - wb).(wa + wb - 2 truth) / 2, so FP64 rounding
-    stays below the comparison."""
+    The step is 3e-9 of the field's scale, where the two error sources of a
+    finite difference of this map are smallest together.  The map is only
+    piecewise smooth (the |beta1 - beta3| of WENO-Z, weno.py:188, and the
+    upwind switch at U = 0, weno.py:325, have kinks), and the central
+    differences' error from the kinks inside the +-eps segment grows in
+    proportion to eps (measured: about 5e-7 at a 10x larger step); FP64
+    rounding of the two perturbed runs gives an absolute error of about
+    1e-12 to 5e-12 here, which grows as eps shrinks.  Loss increments are
+    formed from state differences, (wa - wb).(wa + wb - 2 truth) / 2.
+
+    A relative error is meaningful only where the directional derivative is
+    not small, so of 16 seeded candidates the first 8 whose central
+    difference is at least half the RMS over the candidates are compared
+    (the choice depends on the finite differences only, not on the
+    adjoint).  The gradient as a whole is held to FP64 rounding by the
+    torch-autograd comparisons."""
     runner, w0, wt, dt = _setup(128, "fp64", 20, 5)
     L, g = _loss_and_grad(runner, w0, wt)
     rng = np.random.default_rng(3)
     scale = float(w0[3:-3, 3:-3].abs().max())
     eps = 3e-9 * scale * 128
     tt = wt[3:-3, 3:-3]
-    w00 = runner.forward(w0)[3:-3, 3:-3]
-
-    def dL(wa, wb):
-        return 0.5 * float(((wa - wb) * ((wa - tt) + (wb - tt))).sum())
-
-    errs, rejected = [], 0
-    for k in range(32):
-        if len(errs) == 8:
-            break
+    cands = []
+    for k in range(16):
         d = torch.zeros_like(w0)
         d[3:-3, 3:-3] = torch.from_numpy(rng.standard_normal((128, 128))).to(w0.device)
         d /= torch.linalg.norm(d)
         wp = runner.forward(w0 + eps * d)[3:-3, 3:-3]
         wm = runner.forward(w0 - eps * d)[3:-3, 3:-3]
-        fplus, fminus = dL(wp, w00) / eps, dL(w00, wm) / eps
-        fd = dL(wp, wm) / (2 * eps)
+        fd = 0.5 * float(((wp - wm) * ((wp - tt) + (wm - tt))).sum()) / (2 * eps)
         ad = float((g * d).sum())
-        kinked = abs(fplus - fminus) > 1e-7 * abs(fd)
-        print(f"direction {k}: adjoint {ad:.12e} FD {fd:.12e} rel {abs(ad - fd) / abs(fd):.1e}"
-              f" one-sided gap {abs(fplus - fminus) / abs(fd):.1e}"
-              + (" (crosses a kink: skipped)" if kinked else ""))
-        if kinked:
-            rejected += 1
-            continue
-        errs.append(abs(ad - fd) / abs(fd))
-    print(f"{len(errs)} directions kept, {rejected} crossing a kink skipped")
+        cands.append((k, fd, ad))
+    rms = float(np.sqrt(np.mean([fd * fd for _, fd, _ in cands])))
+    errs = []
+    for k, fd, ad in cands:
+        used = abs(fd) >= 0.5 * rms and len(errs) < 8
+        rel = abs(ad - fd) / abs(fd)
+        print(f"direction {k}: adjoint {ad:.12e} FD {fd:.12e} rel {rel:.1e}"
+              + ("" if used else " (not used)"))
+        if used:
+            errs.append(rel)
+    print(f"{len(errs)} directions compared, |FD| >= {0.5 * rms:.2e}")
     assert len(errs) == 8
     assert max(errs) <= 1e-6, errs
 
