@@ -28,6 +28,7 @@ EXPORTS = [
     "f2d_poisson_method", "f2d_coriolis", "f2d_addgrad", "f2d_sharp", "f2d_vorticity",
     "f2d_ke_finish", "f2d_pressure", "f2d_div", "f2d_c2v_scaled", "f2d_v2c_scaled",
     "f2d_adj_stage", "f2d_adj_combine", "f2d_bulk_diagnostics",
+    "f2d_perpgrad_cfl_noalloc",
 ]
 
 
@@ -101,6 +102,8 @@ def lib():
         "f2d_perpgrad": (i32, [i32, gp, vp, vp, vp, vp, vp, i32, vp, vp]),
         "f2d_perpgrad_ex": (i32, [i32, gp, vp, vp, vp, vp, vp, d, d, i32, vp, vp,
                                   d, d, vp]),
+        "f2d_perpgrad_cfl_noalloc": (i32, [i32, gp, vp, vp, vp, vp, vp, d, d, i32, vp, vp,
+                                           d, d, vp]),
         "f2d_fill_halo": (i32, [i32, gp, vp, vp]),
         "f2d_cfl_dt": (i32, [i32, vp, vp, i64, d, d, vp, vp]),
         "f2d_set_scalar": (i32, [vp, d, vp]),

@@ -838,6 +838,34 @@ static double* red_scratch(int nblocks, void* stream) {
   return r.partials;
 }
 
+// Allocation-free variant of the perpgrad + CFL step: the block maxima go
+// to `scratch`, a fixed array in each GPU's memory, instead of the
+// reduction scratch above (which the library allocates on first use).  In
+// host code `scratch` names a host-side stand-in: its device address on the
+// current GPU comes from hipGetSymbolAddress, looked up once per device on
+// the first call there and kept in g_scratch_dev.  Every call on one GPU
+// shares the array: calls that overlap on one GPU (concurrent streams)
+// race.  k_perpgrad writes every partial that k_dt_fin reads, so nothing
+// needs clearing between calls.
+constexpr int NOALLOC_MAX_BLOCKS = 8192;  // 2 x 8192 doubles = 128 KiB
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_BLOCKS];
+static double* g_scratch_dev[64];
+static std::mutex g_scratch_mu;
+
+static double* noalloc_scratch() {
+  int dev = 0;
+  F2D_HIP(hipGetDevice(&dev));
+  F2D_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    F2D_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    F2D_REQUIRE(p != nullptr, "no device address for the reduction scratch");
+    g_scratch_dev[dev] = (double*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
 // workgroups of k_perpgrad over n padded points: a multiple of 8 (its chunk
 // order); the surplus chunks are empty and report maxima of 0
 static int perpgrad_blocks(int64_t n) { return (grid1d(n, 256 * PG_PTS) + 7) & ~7; }
@@ -1081,6 +1109,29 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
     int64_t n = (g->ny + 6) * (g->nx + 6);
     int nb = perpgrad_blocks(n);
     double* part = (maxabs || dt) ? red_scratch(nb, stream) : nullptr;
+    launch_perpgrad(dtype, g, psi, Ux, Uy, mskx, msky, sx, sy, fill, part, nb, maxabs,
+                    dt, cfl, dtmax, stream);
+  });
+}
+
+int f2d_perpgrad_cfl_noalloc(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                             void* Uy, const int8_t* mskx, const int8_t* msky,
+                             double sx, double sy, int fill, double* maxabs,
+                             double* dt, double cfl, double dtmax, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    F2D_REQUIRE(maxabs || dt, "f2d_perpgrad_cfl_noalloc: maxabs and dt are both NULL "
+                              "(use f2d_perpgrad_ex for the gradient alone)");
+    int64_t n = (g->ny + 6) * (g->nx + 6);
+    int nb = perpgrad_blocks(n);
+    if (nb > NOALLOC_MAX_BLOCKS)
+      throw Error{"f2d_perpgrad_cfl_noalloc: " + std::to_string(n) +
+                     " padded points need " + std::to_string(nb) +
+                     " reduction blocks; the fixed scratch holds " +
+                     std::to_string(NOALLOC_MAX_BLOCKS) + " (at most " +
+                     std::to_string((int64_t)NOALLOC_MAX_BLOCKS * 256 * PG_PTS) +
+                     " padded points, e.g. 2890 x 2890 cells)"};
+    double* part = noalloc_scratch();
     launch_perpgrad(dtype, g, psi, Ux, Uy, mskx, msky, sx, sy, fill, part, nb, maxabs,
                     dt, cfl, dtmax, stream);
   });

@@ -177,6 +177,21 @@ int f2d_perpgrad_ex(int dtype, const f2d_grid* g, const void* psi, void* Ux,
                     void* Uy, const int8_t* mskx, const int8_t* msky,
                     double sx, double sy, int fill, double* maxabs,
                     double* dt, double cfl, double dtmax, void* stream);
+/* f2d_perpgrad_ex with the CFL step or the maxima requested (maxabs or dt
+ * not NULL), allocating nothing: no counterpart in the reference.  It is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops).  The block maxima go to a fixed 128 KiB array
+ * in each GPU's memory, held until the process ends, instead of the
+ * reduction scratch the library allocates on first use.  Limit: at most
+ * 8192 x 1024 = 8388608 padded points, (ny+6)(nx+6), about 2890^2 cells.
+ * Larger grids are refused with an error.  Calls on one GPU share the
+ * array, so calls that overlap on one GPU (concurrent streams) race.  The
+ * first call on each GPU looks up the array's device address; make it
+ * outside a stream capture.  Results are bit for bit f2d_perpgrad_ex's. */
+int f2d_perpgrad_cfl_noalloc(int dtype, const f2d_grid* g, const void* psi, void* Ux,
+                             void* Uy, const int8_t* mskx, const int8_t* msky,
+                             double sx, double sy, int fill, double* maxabs,
+                             double* dt, double cfl, double dtmax, void* stream);
 /* fill_halo_array, meshes.py:135-143 (+ the y-periodic extension) */
 int f2d_fill_halo(int dtype, const f2d_grid* g, void* a, void* stream);
 /* Model.set_dt, model.py:71-87 (non-rsw branch):
