@@ -60,7 +60,7 @@ class PoissonOpts(ctypes.Structure):
                 ("neumann", ctypes.c_int32)]
 
 
-SOLVERS = {"auto": 0, "mg": 1, "fft": 2}
+SOLVERS = {"auto": 0, "mg": 1, "fft": 2, "cap": 4}
 
 ABI_VERSION = 5  # f2d_abi_version() of the library these bindings describe
 

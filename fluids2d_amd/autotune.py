@@ -83,7 +83,8 @@ def mg_candidates(precision):
 def poisson_key(mesh, msk, maindiag):
     p = mesh.param
     wet = float(np.asarray(msk)[3:-3, 3:-3].mean())
-    return (f"mg|{_device()}|{getattr(p, 'precision', 'fp64')}|{mesh.ny}x{mesh.nx}|"
+    return (f"mg|{getattr(p, 'poisson_solver', 'auto') or 'auto'}|{_device()}|"
+            f"{getattr(p, 'precision', 'fp64')}|{mesh.ny}x{mesh.nx}|"
             f"x{int(bool(p.xperiodic))}y{int(bool(getattr(p, 'yperiodic', False)))}|"
             f"wet{wet:.1f}|md{int(maindiag > 0)}")
 
@@ -114,7 +115,8 @@ def _test_rhs(shape, msk):
 def tune_poisson(mesh, maindiag=0.0, reps=4, candidates=None, cache=True, verbose=False):
     """Best multigrid schedule (a dict of Poisson2D tuning options) for the
     vertex operator of `mesh` with mass term `maindiag`; {} when the
-    operator is solved by the FFT (nothing to tune)."""
+    operator is solved directly (FFT or capacitance solve: nothing to
+    tune)."""
     from .elliptic import Poisson2D, get_msk_from_mesh
     msk = get_msk_from_mesh(mesh, "v")
     key = poisson_key(mesh, msk, maindiag)
@@ -128,8 +130,8 @@ def tune_poisson(mesh, maindiag=0.0, reps=4, candidates=None, cache=True, verbos
     rhs = [torch.from_numpy(r).to(mesh.device, dtype) for r in _test_rhs(mesh.shape, msk)]
     for opts in cands:
         ps = Poisson2D(mesh, "v", maindiag=maindiag, tuning=opts)
-        if ps.method == "fft":
-            return {}
+        if ps.method in ("fft", "capacitance"):
+            return {}  # a direct solve: no multigrid schedule to pick
         ok = True
         for b in rhs:
             x.zero_()

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libfluids2d_amd.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["f2d_mesh.hip", "f2d_stencil.hip", "f2d_poisson.hip", "f2d_fft.hip", "f2d_ops.hip", "f2d_adjoint.hip", "f2d_neumann.hip"]
+SOURCES = ["f2d_mesh.hip", "f2d_stencil.hip", "f2d_poisson.hip", "f2d_fft.hip", "f2d_ops.hip", "f2d_adjoint.hip", "f2d_neumann.hip",
+           "f2d_capacitance.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -51,7 +52,10 @@ def build(jobs=4, verbose=False):
         if verbose and msgs.strip():
             print(msgs)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    # rocSOLVER (with rocBLAS) factorises and inverts the capacitance matrix
+    # of a masked solve once, when the solver handle is created
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

@@ -1,0 +1,544 @@
+// f2d_capacitance.hip -- masked Poisson/Helmholtz on the vertex grid as a
+// direct solve: two spectral solves of the rectangle operator around a dense
+// capacitance correction (see f2d_capacitance.hpp for the method).
+//
+// Replaces, for masks inside a rectangle the spectral solve handles, the
+// factorisation and back-substitution of Poisson2D (elliptic.py:71-87) on
+// the masked operator of get_Laplacian_sparse (elliptic.py:114-195).
+//
+// Kernels of one FP64 solve:
+//   k_cap_rhs    b masked to the unknowns W (b_W), zero elsewhere
+//   (3 spectral passes)   y = A_R^-1 b_W
+//   k_cap_gemv   s = M y_B (M = (-C)^-1), written into b_W at the points B
+//   (3 spectral passes)   y = A_R^-1 (b_W + P_B^T s)
+//   k_cap_out    x = y at the unknowns W
+// The FP32 mode wraps the same sequence, in FP32 with an FP32 M, into FP64
+// refinement (k_cap_res forms the FP64 residual, k_cap_acc accumulates).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cstring>
+#include <vector>
+
+#include "f2d_capacitance.hpp"
+#include "f2d_fft.hpp"
+
+namespace f2d {
+
+// the padded geometry every kernel of this file takes
+struct CapGeo {
+  int ny, nx, NXP, NYP;
+  int xper, yper;
+  double cx, cy, d;  // (A x)_P = cx (x_E + x_W) + cy (x_N + x_S) - d x_P
+};
+
+__device__ __forceinline__ int wrap_i(const CapGeo& g, int i) {
+  if (!g.xper) return i;
+  return i < 3 ? i + g.nx : (i >= g.nx + 3 ? i - g.nx : i);
+}
+__device__ __forceinline__ int wrap_j(const CapGeo& g, int j) {
+  if (!g.yper) return j;
+  return j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j);
+}
+
+// value of b at padded point k: given, or centerstovertices of the cell
+// field (operators.py:126-133, the reference's summation order)
+template <class TB>
+__device__ __forceinline__ TB bval(const TB* b, int64_t k, int NXP, int gmode) {
+  if (!gmode) return b[k];
+  return TB(0.25) * (((b[k - NXP - 1] + b[k - 1]) + b[k - NXP]) + b[k]);
+}
+
+// rhs = b at the unknowns, 0 elsewhere (the points B included)
+template <class TB, class T>
+__global__ __launch_bounds__(256) void k_cap_rhs(CapGeo g, const TB* __restrict__ b,
+                                                 int gmode, const int8_t* __restrict__ mw,
+                                                 T* __restrict__ rhs) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (int64_t)g.NYP * g.NXP) return;
+  rhs[k] = mw[k] ? T(bval(b, k, g.NXP, gmode)) : T(0);
+}
+
+// x at an unknown neighbour (periodic wrap), 0 at a dry one
+template <class TX>
+__device__ __forceinline__ double xw(const CapGeo& g, const TX* x,
+                                     const int8_t* mw, int j, int i) {
+  const int64_t k = (int64_t)wrap_j(g, j) * g.NXP + wrap_i(g, i);
+  return mw[k] ? (double)x[k] : 0.0;
+}
+
+// FP64 residual of the masked operator, r = b - A x at the unknowns (x read
+// at unknowns only; xin NULL: x = 0), stored as T (0 elsewhere)
+template <class TB, class TX, class T>
+__global__ __launch_bounds__(256) void k_cap_res(CapGeo g, const TB* __restrict__ b,
+                                                 int gmode, const int8_t* __restrict__ mw,
+                                                 const TX* __restrict__ xin,
+                                                 T* __restrict__ r) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (int64_t)g.NYP * g.NXP) return;
+  double v = 0.0;
+  if (mw[k]) {
+    v = (double)bval(b, k, g.NXP, gmode);
+    if (xin) {
+      const int j = (int)(k / g.NXP), i = (int)(k - (int64_t)j * g.NXP);
+      const double ax = g.cx * (xw(g, xin, mw, j, i - 1) + xw(g, xin, mw, j, i + 1)) +
+                        g.cy * (xw(g, xin, mw, j - 1, i) + xw(g, xin, mw, j + 1, i)) -
+                        g.d * (double)xin[k];
+      v -= ax;
+    }
+  }
+  r[k] = T(v);
+}
+
+// s = M y_B for the rows of this workgroup, written to rhs at the points B
+// (rhs is zero there).  y_B is gathered into LDS; each wavefront takes one
+// row at a time, two (FP64) or four (FP32) entries per lane per load, and
+// sums in FP64 in a fixed order.  Row pitch ld (a multiple of 4, padding 0).
+template <class T>
+__global__ __launch_bounds__(256) void k_cap_gemv(const T* __restrict__ M, int n, int ld,
+                                                  const int* __restrict__ bidx,
+                                                  const T* __restrict__ y,
+                                                  T* __restrict__ rhs, int rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* gy = reinterpret_cast<T*>(smem);
+  for (int j = threadIdx.x; j < ld; j += 256) gy[j] = j < n ? y[bidx[j]] : T(0);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = blockIdx.x * rows;
+  const int r1 = r0 + rows < n ? r0 + rows : n;
+  constexpr int V = sizeof(T) == 8 ? 2 : 4;  // entries per 16-byte load
+  for (int r = r0 + wave; r < r1; r += 4) {
+    const T* row = M + (int64_t)r * ld;
+    double acc = 0.0;
+    for (int j = lane * V; j < ld; j += 64 * V) {
+      if (V == 2) {
+        const double2 m = *reinterpret_cast<const double2*>(row + j);
+        acc += m.x * (double)gy[j] + m.y * (double)gy[j + 1];
+      } else {
+        const float4 m = *reinterpret_cast<const float4*>(row + j);
+        acc += ((double)m.x * gy[j] + (double)m.y * gy[j + 1]) +
+               ((double)m.z * gy[j + 2] + (double)m.w * gy[j + 3]);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) rhs[bidx[r]] = T(acc);
+  }
+}
+
+// x = y at the unknowns
+template <class T>
+__global__ __launch_bounds__(256) void k_cap_out(int64_t n, const int8_t* __restrict__ mw,
+                                                 const T* __restrict__ y, T* __restrict__ x) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < n && mw[k]) x[k] = y[k];
+}
+
+// x64 = x0 + e at the unknowns (x0 NULL: 0; x0 may be x64 itself)
+template <class TX>
+__global__ __launch_bounds__(256) void k_cap_acc(int64_t n, const int8_t* __restrict__ mw,
+                                                 const TX* x0, const float* __restrict__ e,
+                                                 double* x64) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < n && mw[k]) x64[k] = (x0 ? (double)x0[k] : 0.0) + (double)e[k];
+}
+
+__global__ __launch_bounds__(256) void k_cap_round(int64_t n, const int8_t* __restrict__ mw,
+                                                   const double* __restrict__ x64,
+                                                   float* __restrict__ x) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < n && mw[k]) x[k] = (float)x64[k];
+}
+
+__device__ __forceinline__ double bsum(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+// per-block sums of r^2 and b^2 over the unknowns, r = b - A x (FP64)
+template <class TB, class TX>
+__global__ __launch_bounds__(256) void k_cap_norms(CapGeo g, const TB* __restrict__ b,
+                                                   int gmode, const int8_t* __restrict__ mw,
+                                                   const TX* __restrict__ x,
+                                                   double* __restrict__ part) {
+  __shared__ double sh[4];
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (k < (int64_t)g.NYP * g.NXP && mw[k]) {
+    const int j = (int)(k / g.NXP), i = (int)(k - (int64_t)j * g.NXP);
+    const double bv = (double)bval(b, k, g.NXP, gmode);
+    const double ax = g.cx * (xw(g, x, mw, j, i - 1) + xw(g, x, mw, j, i + 1)) +
+                      g.cy * (xw(g, x, mw, j - 1, i) + xw(g, x, mw, j + 1, i)) -
+                      g.d * (double)x[k];
+    rr = (bv - ax) * (bv - ax);
+    bb = bv * bv;
+  }
+  rr = bsum(rr, sh);
+  bb = bsum(bb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[gridDim.x + blockIdx.x] = bb;
+  }
+}
+
+// one block: relative residual into the solve statistics
+// [iterations, relres, converged, refinements | failed, max, solves]
+// (a direct solve is accepted unless its residual is above 1e-6 or not finite)
+__global__ __launch_bounds__(256) void k_cap_stats(const double* __restrict__ part, int nb,
+                                                   int passes, double* __restrict__ st) {
+  __shared__ double sh[4];
+  double a = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    a += part[i];
+    c += part[nb + i];
+  }
+  a = bsum(a, sh);
+  c = bsum(c, sh);
+  if (threadIdx.x != 0) return;
+  const double rel = c > 0.0 ? sqrt(a / c) : 0.0;
+  const double conv = (rel <= 1e-6) ? 1.0 : 0.0;
+  st[0] = (double)passes;
+  st[1] = rel;
+  st[2] = conv;
+  st[3] = (double)(passes - 1);
+  st[4] += conv > 0.0 ? 0.0 : 1.0;
+  if ((double)(passes - 1) > st[5]) st[5] = (double)(passes - 1);
+  st[6] += 1.0;
+}
+
+// C[i, j] = (A_R^-1)(B_i, B_j) from the Green's function G of the periodic
+// doubled grid (ny2 x nx2, padded pitch nx2 + 6): sum over the images of B_j
+// in the closed directions (odd extension: the mirror image about the wall
+// carries the sign -), column-major with leading dimension n
+__global__ __launch_bounds__(256) void k_cap_build(const double* __restrict__ G, int ny2,
+                                                   int nx2, int xper, int yper,
+                                                   const int* __restrict__ bx,
+                                                   const int* __restrict__ by, int n,
+                                                   double* __restrict__ C) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int c = (int)(e / n), r = (int)(e - (int64_t)c * n);
+  const int P = nx2 + 6;
+  double v = 0.0;
+  for (int sy = 0; sy < (yper ? 1 : 2); sy++) {
+    int dy = sy ? by[r] + by[c] : by[r] - by[c];
+    dy %= ny2;
+    if (dy < 0) dy += ny2;
+    for (int sx = 0; sx < (xper ? 1 : 2); sx++) {
+      int dx = sx ? bx[r] + bx[c] : bx[r] - bx[c];
+      dx %= nx2;
+      if (dx < 0) dx += nx2;
+      const double gv = G[(int64_t)(dy + 3) * P + dx + 3];
+      v += ((sx + sy) & 1) ? -gv : gv;
+    }
+  }
+  C[e] = -v;  // -C: symmetric positive definite
+}
+
+// M = the inverse held in the lower triangle of A (column-major, n x n),
+// expanded to both triangles, row pitch ld, padding columns zero
+template <class T>
+__global__ __launch_bounds__(256) void k_cap_expand(const double* __restrict__ A, int n,
+                                                    int ld, T* __restrict__ M) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * ld) return;
+  const int i = (int)(e / ld), j = (int)(e - (int64_t)i * ld);
+  double v = 0.0;
+  if (j < n) v = i >= j ? A[i + (int64_t)j * n] : A[j + (int64_t)i * n];
+  M[e] = T(v);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct CapSolver {
+  CapGeo g{};
+  bool f64 = true;
+  int n = 0, ld = 0, rows = 4;
+  int8_t* mw = nullptr;     // unknowns (padded)
+  int* bidx = nullptr;      // padded index of each point of B
+  void* M = nullptr;        // (-C)^-1, n rows of pitch ld (FP64 or FP32)
+  void* rhs = nullptr;      // padded scratch: right-hand side (T)
+  void* y = nullptr;        // padded scratch: spectral solution (T)
+  double* x64 = nullptr;    // FP32 mode: FP64 accumulator (padded)
+  double* part = nullptr;   // residual-norm partials
+  int nbres = 0;
+  bool solved = false;      // a warm start needs an earlier solve
+  FftSolver* fft = nullptr; // the rectangle's spectral solve
+  std::vector<void*> allocs;
+};
+
+static void* calloc_dev(CapSolver* c, size_t bytes) {
+  void* p = nullptr;
+  F2D_HIP(hipMalloc(&p, bytes < 256 ? 256 : bytes));
+  F2D_HIP(hipMemset(p, 0, bytes < 256 ? 256 : bytes));
+  c->allocs.push_back(p);
+  return p;
+}
+
+static bool pw2(int64_t v) { return v >= 8 && (v & (v - 1)) == 0; }
+
+// rectangle unknown (padded coordinates): every vertex strictly inside
+static inline bool rect_unk(int64_t j, int64_t i, int64_t ny, int64_t nx, int xper,
+                            int yper) {
+  const bool oy = yper ? (j >= 3 && j < ny + 3) : (j >= 4 && j <= ny + 2);
+  const bool ox = xper ? (i >= 3 && i < nx + 3) : (i >= 4 && i <= nx + 2);
+  return oy && ox;
+}
+
+static std::vector<int64_t> boundary_points(const int8_t* unk, int64_t ny, int64_t nx,
+                                            int xper, int yper) {
+  const int64_t NXP = nx + 6, NYP = ny + 6;
+  auto wi = [&](int64_t i) { return xper ? (i < 3 ? i + nx : (i >= nx + 3 ? i - nx : i)) : i; };
+  auto wj = [&](int64_t j) { return yper ? (j < 3 ? j + ny : (j >= ny + 3 ? j - ny : j)) : j; };
+  std::vector<int64_t> out;
+  for (int64_t j = 0; j < NYP; j++)
+    for (int64_t i = 0; i < NXP; i++) {
+      if (!rect_unk(j, i, ny, nx, xper, yper) || unk[j * NXP + i]) continue;
+      const int64_t nb[4][2] = {{j, i - 1}, {j, i + 1}, {j - 1, i}, {j + 1, i}};
+      bool adj = false;
+      for (auto& q : nb) {
+        const int64_t jj = wj(q[0]), ii = wi(q[1]);
+        if (jj >= 0 && jj < NYP && ii >= 0 && ii < NXP && unk[jj * NXP + ii]) adj = true;
+      }
+      if (adj) out.push_back(j * NXP + i);
+    }
+  return out;
+}
+
+int64_t cap_boundary(const int8_t* unk, int64_t ny, int64_t nx, int xper, int yper,
+                     double maindiag, bool f64_mode) {
+  if (!pw2(nx) || !pw2(ny)) return -1;
+  if (xper && yper && maindiag == 0.0) return -1;  // singular rectangle
+  const int64_t nx2 = xper ? nx : 2 * nx, ny2 = yper ? ny : 2 * ny;
+  // the Green's function is an FP64 spectral solve on the doubled grid
+  if (!fft_fits(ny2, nx2, 1, 1, true) || nx2 > 4096 || ny2 > 4096) return -1;
+  if (!fft_fits(ny, nx, xper, yper, f64_mode)) return -1;
+  const int64_t NXP = nx + 6, NYP = ny + 6;
+  int64_t nw = 0, nr = 0;
+  for (int64_t j = 0; j < NYP; j++)
+    for (int64_t i = 0; i < NXP; i++) {
+      const bool r = rect_unk(j, i, ny, nx, xper, yper);
+      if (unk[j * NXP + i]) {
+        if (!r) return -1;
+        nw++;
+      }
+      nr += r;
+    }
+  if (nw == 0 || nw == nr) return -1;
+  return (int64_t)boundary_points(unk, ny, nx, xper, yper).size();
+}
+
+static void rocblas_check(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success)
+    throw Error{std::string(what) + ": rocBLAS/rocSOLVER status " + std::to_string((int)st)};
+}
+
+CapSolver* cap_create(const int8_t* unk, int64_t ny, int64_t nx, double dx, double dy,
+                      double maindiag, int xper, int yper, bool f64_mode) {
+  F2D_REQUIRE(cap_boundary(unk, ny, nx, xper, yper, maindiag, f64_mode) > 0,
+              "capacitance solve does not apply to this operator");
+  CapSolver* c = new CapSolver();
+  FftSolver* gfs = nullptr;
+  rocblas_handle rb = nullptr;
+  try {
+    const int64_t NXP = nx + 6, NYP = ny + 6, N = NXP * NYP;
+    CapGeo& g = c->g;
+    g.ny = (int)ny;
+    g.nx = (int)nx;
+    g.NXP = (int)NXP;
+    g.NYP = (int)NYP;
+    g.xper = xper;
+    g.yper = yper;
+    g.cx = dy / dx;
+    g.cy = dx / dy;
+    g.d = 2.0 * (g.cx + g.cy) + maindiag;
+    c->f64 = f64_mode;
+    const std::vector<int64_t> B = boundary_points(unk, ny, nx, xper, yper);
+    const int n = (int)B.size();
+    c->n = n;
+    c->ld = (n + 3) & ~3;
+    c->rows = 4 * ((n + 4 * 512 - 1) / (4 * 512));  // about 512 workgroups
+    c->mw = (int8_t*)calloc_dev(c, (size_t)N);
+    F2D_HIP(hipMemcpy(c->mw, unk, (size_t)N, hipMemcpyHostToDevice));
+    std::vector<int> bi(n), bxv(n), byv(n);
+    for (int k = 0; k < n; k++) {
+      bi[k] = (int)B[k];
+      byv[k] = (int)(B[k] / NXP) - 3;  // local coordinates: the first wall
+      bxv[k] = (int)(B[k] % NXP) - 3;  // line (closed) or column (periodic) at 0
+    }
+    c->bidx = (int*)calloc_dev(c, sizeof(int) * n);
+    F2D_HIP(hipMemcpy(c->bidx, bi.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    int* bxd = (int*)calloc_dev(c, sizeof(int) * n);
+    int* byd = (int*)calloc_dev(c, sizeof(int) * n);
+    F2D_HIP(hipMemcpy(bxd, bxv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    F2D_HIP(hipMemcpy(byd, byv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+
+    // Green's function of the periodic doubled grid: the response to a unit
+    // source at the origin (padded (3, 3)).  maindiag = 0 with a closed
+    // direction: the zero-mean solution, whose constant the odd images cancel.
+    const int64_t nx2 = xper ? nx : 2 * nx, ny2 = yper ? ny : 2 * ny;
+    const int64_t N2 = (nx2 + 6) * (ny2 + 6);
+    gfs = fft_create(ny2, nx2, dx, dy, maindiag, true, false, 1, 1);
+    double* src = nullptr;
+    double* G = nullptr;
+    F2D_HIP(hipMalloc(&src, sizeof(double) * N2));
+    F2D_HIP(hipMalloc(&G, sizeof(double) * N2));
+    F2D_HIP(hipMemset(src, 0, sizeof(double) * N2));
+    F2D_HIP(hipMemset(G, 0, sizeof(double) * N2));
+    const double one = 1.0;
+    F2D_HIP(hipMemcpy(src + 3 * (nx2 + 6) + 3, &one, sizeof(double), hipMemcpyHostToDevice));
+    fft_solve_f64(gfs, src, 0, G, 0.0, false, nullptr, 0);
+    double* A = nullptr;
+    F2D_HIP(hipMalloc(&A, sizeof(double) * (size_t)n * n));
+    const int64_t ne = (int64_t)n * n;
+    k_cap_build<<<(unsigned)((ne + 255) / 256), 256>>>(G, (int)ny2, (int)nx2, xper, yper, bxd,
+                                                      byd, n, A);
+    F2D_HIP(hipGetLastError());
+    F2D_HIP(hipDeviceSynchronize());
+    (void)hipFree(src);
+    (void)hipFree(G);
+    fft_destroy(gfs);
+    gfs = nullptr;
+
+    // M = (-C)^-1 by Cholesky factorisation and inversion (lower triangle)
+    rocblas_check(rocblas_create_handle(&rb), "rocblas_create_handle");
+    rocblas_int* info = nullptr;
+    F2D_HIP(hipMalloc(&info, sizeof(rocblas_int)));
+    rocblas_check(rocsolver_dpotrf(rb, rocblas_fill_lower, n, A, n, info), "rocsolver_dpotrf");
+    rocblas_int ih = 0;
+    F2D_HIP(hipMemcpy(&ih, info, sizeof(ih), hipMemcpyDeviceToHost));
+    if (ih != 0) {
+      (void)hipFree(info);
+      (void)hipFree(A);
+      throw Error{"capacitance matrix is not definite (potrf info " + std::to_string(ih) + ")"};
+    }
+    rocblas_check(rocsolver_dpotri(rb, rocblas_fill_lower, n, A, n, info), "rocsolver_dpotri");
+    F2D_HIP(hipMemcpy(&ih, info, sizeof(ih), hipMemcpyDeviceToHost));
+    (void)hipFree(info);
+    rocblas_destroy_handle(rb);
+    rb = nullptr;
+    if (ih != 0) {
+      (void)hipFree(A);
+      throw Error{"capacitance matrix inversion failed (potri info " + std::to_string(ih) + ")"};
+    }
+    const int64_t nm = (int64_t)n * c->ld;
+    const size_t tsz = f64_mode ? sizeof(double) : sizeof(float);
+    c->M = calloc_dev(c, tsz * (size_t)nm);
+    if (f64_mode)
+      k_cap_expand<double><<<(unsigned)((nm + 255) / 256), 256>>>(A, n, c->ld, (double*)c->M);
+    else
+      k_cap_expand<float><<<(unsigned)((nm + 255) / 256), 256>>>(A, n, c->ld, (float*)c->M);
+    F2D_HIP(hipGetLastError());
+    F2D_HIP(hipDeviceSynchronize());
+    (void)hipFree(A);
+
+    c->rhs = calloc_dev(c, tsz * (size_t)N);
+    c->y = calloc_dev(c, tsz * (size_t)N);
+    if (!f64_mode) c->x64 = (double*)calloc_dev(c, sizeof(double) * (size_t)N);
+    c->nbres = (int)((N + 255) / 256);
+    c->part = (double*)calloc_dev(c, sizeof(double) * 2 * c->nbres);
+    c->fft = fft_create(ny, nx, dx, dy, maindiag, f64_mode, !f64_mode, xper, yper);
+    // the GEMV's y_B in LDS
+    const size_t lds = tsz * (size_t)c->ld;
+    F2D_REQUIRE(lds <= 160 * 1024, "capacitance boundary too long for the LDS gather");
+    if (lds > 64 * 1024) {
+      if (f64_mode)
+        F2D_HIP(hipFuncSetAttribute((const void*)k_cap_gemv<double>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      else
+        F2D_HIP(hipFuncSetAttribute((const void*)k_cap_gemv<float>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    F2D_HIP(hipDeviceSynchronize());
+  } catch (...) {
+    if (rb) rocblas_destroy_handle(rb);
+    if (gfs) fft_destroy(gfs);
+    cap_destroy(c);
+    throw;
+  }
+  return c;
+}
+
+void cap_destroy(CapSolver* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  if (c->fft) fft_destroy(c->fft);
+  for (void* p : c->allocs) (void)hipFree(p);
+  delete c;
+}
+
+int64_t cap_size(const CapSolver* c) { return c ? c->n : 0; }
+
+static unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// the capacitance correction between the two spectral solves of rhs
+template <class T>
+static void gemv(CapSolver* c, hipStream_t s) {
+  const int nwg = (c->n + c->rows - 1) / c->rows;
+  k_cap_gemv<T><<<nwg, 256, sizeof(T) * (size_t)c->ld, s>>>(
+      (const T*)c->M, c->n, c->ld, c->bidx, (const T*)c->y, (T*)c->rhs, c->rows);
+}
+
+template <class TB, class TX>
+static void measure_res(CapSolver* c, const TB* b, int gmode, const TX* x, int passes,
+                        double* stats, hipStream_t s) {
+  k_cap_norms<TB, TX><<<c->nbres, 256, 0, s>>>(c->g, b, gmode, c->mw, x, c->part);
+  k_cap_stats<<<1, 256, 0, s>>>(c->part, c->nbres, passes, stats);
+}
+
+void cap_solve_f64(CapSolver* c, const double* b, int gmode, double* x, bool measure,
+                   double* stats, hipStream_t s) {
+  F2D_REQUIRE(c->f64, "capacitance solver built for FP32");
+  const int64_t N = (int64_t)c->g.NYP * c->g.NXP;
+  double* rhs = (double*)c->rhs;
+  double* y = (double*)c->y;
+  k_cap_rhs<double, double><<<blocks(N), 256, 0, s>>>(c->g, b, gmode, c->mw, rhs);
+  fft_solve_f64(c->fft, rhs, 0, y, 0.0, false, nullptr, s);
+  gemv<double>(c, s);
+  fft_solve_f64(c->fft, rhs, 0, y, 0.0, false, nullptr, s);
+  k_cap_out<double><<<blocks(N), 256, 0, s>>>(N, c->mw, y, x);
+  if (measure && stats) measure_res<double, double>(c, b, gmode, x, 1, stats, s);
+  c->solved = true;
+}
+
+void cap_solve_f32(CapSolver* c, const float* b, int gmode, float* x, int nref, bool warm,
+                   bool measure, double* stats, hipStream_t s) {
+  F2D_REQUIRE(!c->f64, "capacitance solver built for FP64");
+  const int64_t N = (int64_t)c->g.NYP * c->g.NXP;
+  float* rhs = (float*)c->rhs;
+  float* y = (float*)c->y;
+  const bool from_x = warm && c->solved;
+  const int passes = from_x ? (nref > 0 ? nref : 1) : (nref > 0 ? nref : 1) + 1;
+  for (int p = 0; p < passes; p++) {
+    // residual of the current iterate: the caller's x (warm, first pass),
+    // x64 (later passes) or 0 (cold, first pass)
+    if (p == 0 && from_x)
+      k_cap_res<float, float, float><<<blocks(N), 256, 0, s>>>(c->g, b, gmode, c->mw, x, rhs);
+    else if (p == 0)
+      k_cap_res<float, double, float><<<blocks(N), 256, 0, s>>>(c->g, b, gmode, c->mw,
+                                                               (const double*)nullptr, rhs);
+    else
+      k_cap_res<float, double, float><<<blocks(N), 256, 0, s>>>(c->g, b, gmode, c->mw,
+                                                               c->x64, rhs);
+    fft_solve_f32_refine(c->fft, rhs, 0, y, 0, false, 0.0, false, nullptr, s);
+    gemv<float>(c, s);
+    fft_solve_f32_refine(c->fft, rhs, 0, y, 0, false, 0.0, false, nullptr, s);
+    if (p == 0 && from_x)
+      k_cap_acc<float><<<blocks(N), 256, 0, s>>>(N, c->mw, x, y, c->x64);
+    else if (p == 0)
+      k_cap_acc<double><<<blocks(N), 256, 0, s>>>(N, c->mw, (const double*)nullptr, y, c->x64);
+    else
+      k_cap_acc<double><<<blocks(N), 256, 0, s>>>(N, c->mw, c->x64, y, c->x64);
+  }
+  k_cap_round<<<blocks(N), 256, 0, s>>>(N, c->mw, c->x64, x);
+  if (measure && stats) measure_res<float, double>(c, b, gmode, c->x64, passes, stats, s);
+  c->solved = true;
+}
+
+}  // namespace f2d

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <vector>
 
+#include "f2d_capacitance.hpp"
 #include "f2d_common.hpp"
 #include "f2d_fft.hpp"
 #include "f2d_neumann.hpp"
@@ -98,6 +99,11 @@ __device__ __forceinline__ T finite_or_zero(T v) {
 
 constexpr int BS = 256;
 constexpr int FC0 = 32;  // level-0 tile (points per side)
+// capacitance solve: the largest boundary requested explicitly (the FP64
+// gather of its values must fit a workgroup's LDS) and the largest the
+// automatic choice takes (its inverse: 8192^2 FP64 = 512 MiB per handle)
+constexpr int64_t CAP_MAX = 16384;
+constexpr int64_t CAP_AUTO_MAX = 8192;
 
 // Partial-sum regions (each REGION_SLOTS x nblocks doubles).
 enum Region { R_INIT = 0, R_RR = 1, R_RZ = 2, R_PQ = 3, R_REF = 4, R_MEAN = 5,
@@ -1510,6 +1516,7 @@ struct f2d_poisson {
   double inner_rtol = 1e-4;
   f2d::FftSolver* fft = nullptr;  // direct spectral solve, when it applies
   f2d::NeumannSolver* neu = nullptr;  // Neumann (cell) operator: Jacobi-PCG
+  f2d::CapSolver* cap = nullptr;  // masked operator: spectral + capacitance
   int64_t fft_unmeasured = 0;     // FFT solves made without statistics
   std::vector<void*> allocs;
   double* reg(int r) { return part + (int64_t)r * region; }
@@ -2323,17 +2330,45 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
                 "FFT solver requested but the operator is not fully wet with "
                 "power-of-two sizes (<= 4096, <= 2048 in a closed direction; "
                 "FP64: transformed sizes <= 2048)");
-    F2D_REQUIRE(h->opts.solver >= 0 && h->opts.solver <= 2, "opts.solver unknown");
-    if (fft_ok && h->opts.solver != F2D_SOLVER_MG)
+    F2D_REQUIRE(h->opts.solver >= 0 && h->opts.solver <= F2D_SOLVER_CAP &&
+                    h->opts.solver != F2D_SOLVER_NEUMANN_CG,
+                "opts.solver unknown");
+    if (fft_ok && h->opts.solver != F2D_SOLVER_MG && h->opts.solver != F2D_SOLVER_CAP)
       h->fft = fft_create(h->ny, h->nx, g->dx, g->dy, maindiag,
                           h->opts.mode == F2D_POISSON_F64,
                           h->opts.mode != F2D_POISSON_F64, h->xper, h->yper,
                           h->opts.neumann != 0);
+    // a masked vertex operator inside a rectangle the spectral solve
+    // handles: the direct capacitance solve (auto: with a boundary of at
+    // most CAP_AUTO_MAX points, and multigrid if its set-up fails)
+    if (!h->fft && !h->opts.neumann && h->opts.solver != F2D_SOLVER_MG &&
+        h->opts.mode != F2D_POISSON_F32) {
+      const bool f64m = h->opts.mode == F2D_POISSON_F64;
+      const int64_t nb = cap_boundary(m.data(), h->ny, h->nx, h->xper, h->yper, maindiag, f64m);
+      if (h->opts.solver == F2D_SOLVER_CAP) {
+        F2D_REQUIRE(nb > 0 && nb <= CAP_MAX,
+                    "capacitance solver requested but the operator is not a masked "
+                    "vertex operator inside a rectangle the spectral solve handles "
+                    "(power-of-two sizes, a closed direction or maindiag > 0, at most " +
+                        std::to_string(CAP_MAX) + " boundary points)");
+        h->cap = cap_create(m.data(), h->ny, h->nx, g->dx, g->dy, maindiag, h->xper,
+                            h->yper, f64m);
+      } else if (nb > 0 && nb <= CAP_AUTO_MAX) {
+        try {
+          h->cap = cap_create(m.data(), h->ny, h->nx, g->dx, g->dy, maindiag, h->xper,
+                              h->yper, f64m);
+        } catch (const Error&) {
+          h->cap = nullptr;  // multigrid solves this operator
+          (void)hipGetLastError();
+        }
+      }
+    }
     F2D_HIP(hipDeviceSynchronize());
     *out = h;
   });
   if (rc != 0 && h) {
     if (h->fft) fft_destroy(h->fft);
+    if (h->cap) cap_destroy(h->cap);
     if (h->neu) neu_destroy(h->neu);
     for (void* p : h->allocs) (void)hipFree(p);
     delete h;
@@ -2346,6 +2381,7 @@ int f2d_poisson_destroy(f2d_poisson* h) {
     if (!h) return;
     F2D_HIP(hipDeviceSynchronize());
     if (h->fft) fft_destroy(h->fft);
+    if (h->cap) cap_destroy(h->cap);
     if (h->neu) neu_destroy(h->neu);
     for (void* p : h->allocs) F2D_HIP(hipFree(p));
     delete h;
@@ -2377,7 +2413,19 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
       }
       return;
     }
-    if (h->fft) {
+    if (h->cap) {
+      Ctl* c = mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
+      if (mode == F2D_POISSON_F64) {
+        cap_solve_f64(h->cap, (const double*)b, gmode, (double*)x, sync, c->stats, s);
+        fill_x<double>(h, (double*)x, s);
+      } else {
+        const int nref = h->opts.fft_refine > 0 ? h->opts.fft_refine : 1;
+        cap_solve_f32(h->cap, (const float*)b, gmode, (float*)x, nref,
+                      h->opts.fft_warm != 0, sync, c->stats, s);
+        fill_x<float>(h, (float*)x, s);
+      }
+      if (!sync) h->fft_unmeasured++;
+    } else if (h->fft) {
       Ctl* c = mode == F2D_POISSON_F64 ? h->p64.ctl : h->p32.ctl;
       if (mode == F2D_POISSON_F64) {
         fft_solve_f64(h->fft, (const double*)b, gmode, (double*)x, rtol, sync,
@@ -2423,6 +2471,7 @@ int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
 int f2d_poisson_method(f2d_poisson* h) {
   if (!h) return -1;
   if (h->neu) return F2D_SOLVER_NEUMANN_CG;
+  if (h->cap) return F2D_SOLVER_CAP;
   return h->fft ? F2D_SOLVER_FFT : F2D_SOLVER_MG;
 }
 

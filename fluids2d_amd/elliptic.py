@@ -93,7 +93,8 @@ class Poisson2D:
         self._h = h
         # "fft": direct spectral solve (exact to rounding, no rtol test);
         # "mg": multigrid-preconditioned CG to rtol
-        self.method = {2: "fft", 3: "neumann"}.get(_lib.lib().f2d_poisson_method(h), "mg")
+        self.method = {2: "fft", 3: "neumann", 4: "capacitance"}.get(
+            _lib.lib().f2d_poisson_method(h), "mg")
         self.last_iters = 0
         self.last_relres = 0.0
         # iterations (refinements in FP32) the no-sync solves enqueue:
@@ -175,7 +176,7 @@ class Poisson2D:
         return None
 
     def _accepted(self, relres):
-        if self.method == "fft":
+        if self.method in ("fft", "capacitance"):
             # direct: exact up to rounding.  FP32 transforms refined once from
             # the previous solution: the FP64 residual of FP32 rounding at
             # the high wavenumbers (the operator amplifies them by k^2)
@@ -228,7 +229,7 @@ class Poisson2D:
     def graph_replayed(self, nsolves, captured):
         """Account for the direct solves of a graph replay (nsolves) and of
         its capture (captured, enqueued but never run)."""
-        if self.method == "fft":
+        if self.method in ("fft", "capacitance"):
             self.replayed += nsolves - captured
 
     def read_stats(self):

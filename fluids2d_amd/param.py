@@ -21,8 +21,9 @@ Fields of the MI355X path (after the reference's in the table):
 * ``poisson_maxit``  iteration cap of the Poisson solves (None: 100 in FP64,
                      8 refinements in FP32)
 * ``poisson_nu``     red-black Gauss-Seidel sweeps before/after each level
-* ``poisson_solver`` "auto" (direct FFT solve where the operator allows it,
-                     multigrid CG otherwise), "mg" or "fft"
+* ``poisson_solver`` "auto" (direct FFT solve of a fully wet rectangle, direct
+                     capacitance solve of a mask inside one, multigrid CG
+                     otherwise), "mg", "fft" or "cap"
 * ``poisson_warm``   FP32 direct solves refine from the solution held in x
                      (the previous stage's streamfunction)
 * ``poisson_opts``   dict of multigrid schedule options, as picked by
@@ -41,7 +42,7 @@ FLUX_METHODS = ("weno", "upwind", "centered", "cweno")
 INNER_METHODS = FLUX_METHODS + ("classic",)
 INTEGRATORS = ("rk3", "ef", "enrk3", "LFRA")
 PRECISIONS = ("fp64", "fp32")
-SOLVERS = ("auto", "mg", "fft")
+SOLVERS = ("auto", "mg", "fft", "cap")
 
 # models whose rhs/diag run on the GPU path
 GPU_MODELS = ("euler", "eulerpsi", "advection", "qg", "rsw", "qgrsw")

@@ -312,7 +312,8 @@ typedef struct f2d_poisson_opts {
   int32_t tail_threads;  /* workgroup size of the tail kernel (256/512/1024) */
   int32_t nu_coarse;     /* sweeps on the multi-block coarse levels (0: nu) */
   int32_t nu_tail;       /* sweeps on the tail levels (0: nu)              */
-  int32_t solver;        /* F2D_SOLVER_*: 0 auto, 1 multigrid CG, 2 FFT      */
+  int32_t solver;        /* F2D_SOLVER_*: 0 auto, 1 multigrid CG, 2 FFT,
+                            4 capacitance                                  */
   int32_t fft_refine;    /* FFT in FP32 modes: FP64 refinements (0: 1)     */
   int32_t fft_warm;      /* FFT in F32_REFINE: 1 = from the handle's second
                             solve on, the first refinement starts from x
@@ -340,6 +341,15 @@ typedef struct f2d_poisson_opts {
 #define F2D_SOLVER_MG 1
 #define F2D_SOLVER_FFT 2
 #define F2D_SOLVER_NEUMANN_CG 3  /* reported by f2d_poisson_method only */
+/* The masked vertex operator of a mask inside a rectangle the FFT solve
+ * handles (power-of-two sizes, a closed direction or maindiag > 0): the
+ * masked operator is the rectangle operator's principal submatrix on the
+ * unknowns, so x = A_R^-1 (b + s) with a correction s on the dry points
+ * next to the unknowns from the inverse of their capacitance matrix (formed
+ * once at creation with rocSOLVER).  A direct solve: two FFT solves and one
+ * dense matrix-vector product.  "auto" takes it for at most 8192 such points
+ * (FP64 and F32_REFINE modes). */
+#define F2D_SOLVER_CAP 4
 
 /* unknowns_host: int8, padded shape, 1 at unknowns (elliptic.py:102-111:
  * mskv with the periodic halos cleared).  opts may be NULL (defaults). */
@@ -356,13 +366,14 @@ int f2d_poisson_destroy(f2d_poisson* h);
  * residual; giving either synchronises `stream`. */
 int f2d_poisson_solve(f2d_poisson* h, const void* b, void* x, double rtol,
                       int maxit, int* iters, double* relres, void* stream);
-/* The method a handle solves with, F2D_SOLVER_MG or F2D_SOLVER_FFT (-1 for
- * NULL); no counterpart in the reference, whose SuperLU factor is its only
- * method (elliptic.py:78).  With FFT the solve is direct: `maxit` is unused,
- * *iters receives 1 + the FP64 refinements, *relres the measured true
- * relative residual, and the solve counts as converged when that residual
- * is below 1e-6 (`rtol` is not its acceptance test: a direct FP64 solve
- * reaches eps times the condition number, about 1e-11 at 1024^2). */
+/* The method a handle solves with, F2D_SOLVER_MG, F2D_SOLVER_FFT,
+ * F2D_SOLVER_NEUMANN_CG or F2D_SOLVER_CAP (-1 for NULL); no counterpart in
+ * the reference, whose SuperLU factor is its only method (elliptic.py:78).
+ * With FFT or CAP the solve is direct: `maxit` is unused, *iters receives
+ * 1 + the FP64 refinements, *relres the measured true relative residual,
+ * and the solve counts as converged when that residual is below 1e-6
+ * (`rtol` is not its acceptance test: a direct FP64 solve reaches eps times
+ * the condition number, about 1e-11 at 1024^2). */
 int f2d_poisson_method(f2d_poisson* h);
 /* Fused eulerpsi diagnostic solve (equations.py:80-82): b = c2v(omega)*mskv
  * computed on the fly from the cell field omega, then solve as above. */

@@ -102,12 +102,17 @@ def test_transport_tiles_bitexact(dev, precision):
 
 
 def test_autotune_poisson_converges_and_caches(dev, tmp_path, monkeypatch):
+    """Multigrid schedules (solver "mg": the basin's operator is otherwise
+    solved directly, which tune_poisson reports as {})."""
     from fluids2d_amd import autotune
     from fluids2d_amd.elliptic import Poisson2D
     monkeypatch.setenv("F2D_TUNE_CACHE", str(tmp_path / "tune.json"))
     autotune._memory.clear()
     m = bench.ensemble_member(128, 21, "fp32")
     md = m.mesh.qg_helmholtz.maindiag
+    assert m.mesh.qg_helmholtz.method == "capacitance"
+    assert autotune.tune_poisson(m.mesh, maindiag=md, cache=False) == {}
+    m.param.poisson_solver = "mg"
     opts = autotune.tune_poisson(m.mesh, maindiag=md, reps=2, verbose=True)
     assert opts and set(opts) <= set(autotune.mg_candidates("fp32")[0]) | {"refine_inner"}
     assert (tmp_path / "tune.json").exists()
