@@ -28,7 +28,8 @@ EXPORTS = [
     "f2d_poisson_method", "f2d_coriolis", "f2d_addgrad", "f2d_sharp", "f2d_vorticity",
     "f2d_ke_finish", "f2d_pressure", "f2d_div", "f2d_c2v_scaled", "f2d_v2c_scaled",
     "f2d_adj_stage", "f2d_adj_combine", "f2d_bulk_diagnostics",
-    "f2d_perpgrad_cfl_noalloc",
+    "f2d_perpgrad_cfl_noalloc", "f2d_sw_momentum", "f2d_sw_diag", "f2d_qg_pv",
+    "f2d_qg_project",
 ]
 
 
@@ -62,7 +63,34 @@ class PoissonOpts(ctypes.Structure):
 
 SOLVERS = {"auto": 0, "mg": 1, "fft": 2, "cap": 4}
 
-ABI_VERSION = 5  # f2d_abi_version() of the library these bindings describe
+
+class SwStage(ctypes.Structure):
+    """f2d_sw_stage: the RK stage terms of the fused shallow-water kernels,
+    fields 0 u.x, 1 u.y, 2 h."""
+    _fields_ = [("integrator", ctypes.c_int32), ("stage", ctypes.c_int32),
+                ("dt", ctypes.c_void_p), ("prev", (ctypes.c_void_p * 3) * 2),
+                ("out", ctypes.c_void_p * 3), ("y", ctypes.c_void_p * 3)]
+
+
+def sw_stage(integrator, stage, dt, prev=(), out=(None, None, None), y=(None, None, None)):
+    """An SwStage from tensors (None for absent arrays); integrator None:
+    tendency only.  prev: the earlier stages' (u.x, u.y, h) tendencies."""
+    def a(t):
+        return None if t is None else t.data_ptr()
+    st = SwStage()
+    st.integrator = -1 if integrator is None else INTEGRATORS[integrator]
+    st.stage = stage
+    st.dt = a(dt)
+    for k, fields in enumerate(prev):
+        for f, t in enumerate(fields):
+            st.prev[k][f] = a(t)
+    for f in range(3):
+        st.out[f] = a(out[f])
+        st.y[f] = a(y[f])
+    return st
+
+
+ABI_VERSION = 6  # f2d_abi_version() of the library these bindings describe
 
 
 _lib = None
@@ -126,6 +154,13 @@ def lib():
                                 vp, vp]),
         "f2d_adj_combine": (i32, [i32, gp, vp, vp, vp, vp, vp]),
         "f2d_bulk_diagnostics": (i32, [i32, gp, vp, vp, vp, vp, vp, vp, i32, vp, vp]),
+        "f2d_sw_momentum": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, d, vp, vp, vp, vp, i32,
+                                  ctypes.POINTER(SwStage), vp]),
+        "f2d_sw_diag": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp, d, d, vp, vp, vp, vp,
+                              vp, vp]),
+        "f2d_qg_pv": (i32, [i32, gp, vp, vp, vp, vp, vp, d, vp, vp]),
+        "f2d_qg_project": (i32, [i32, gp, vp, vp, vp, vp, vp, d, vp, vp, vp,
+                                 ctypes.POINTER(SwStage), vp]),
         "f2d_poisson_solve": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),
                                     ctypes.POINTER(d), vp]),
         "f2d_poisson_solve_c2v": (i32, [vp, vp, vp, d, i32, ctypes.POINTER(i32),

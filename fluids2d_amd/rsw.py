@@ -1,10 +1,22 @@
 """Shallow-water device models: rsw (equations.py:139-157) and qgrsw, the
-RSW tendency projected onto the QG manifold (equations.py:106-136), operator
-by operator on the device in the reference's order.
+RSW tendency projected onto the QG manifold (equations.py:106-136).
 
 The RK stage is the reference's: rhs into the stage's tendency, `addto` on
-every prognostic array (u.x, u.y, h), then diag.
+every prognostic array (u.x, u.y, h), then diag.  ``rhs`` and ``diag`` run
+operator by operator in the reference's order (the host API and forced
+stages use them); ``stage`` runs the same sequence fused (f2d_sw.hip and
+the transport stage of f2d_stencil.hip), bit for bit the same results:
+
+* rsw: the momentum kernel (vortex force, Coriolis, both gradients, fill,
+  RK update of u in place), the transport stage for h (divflux, fill, RK
+  update into the other h buffer), the diag kernel (sharp, vorticity,
+  kinetic energy, pressure, fills) -- 3 launches per stage;
+* qgrsw: the unprojected, unfilled tendencies (momentum kernel, then
+  compflux and div as the reference leaves them), the pv kernel, the
+  Helmholtz solve, the projection kernel (v2c, perpgrad, fill, RK update
+  of u and h) and the diag kernel.
 """
+import ctypes
 from collections import namedtuple
 
 from . import _lib
@@ -19,10 +31,12 @@ def _ops(name, *args):
 
 class _SW(_Base):
     prognostic = ("u", "h")
+    fused = True  # stage() through the fused kernels (False: operator by operator)
 
     def __init__(self, param, mesh):
         super().__init__(param, mesh)
         self._ds_last = None
+        self._pre = None
         self._stats = True  # host-checked solves unless a stage says otherwise
 
     def _grid(self):
@@ -58,9 +72,39 @@ class _SW(_Base):
         _ops("f2d_vorticity", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y),
              _lib.ptr(m.d_slipcoef), _lib.ptr(s.omega))
 
+    def _stage_terms(self, ds_prev, ds_out, k, dt, y):
+        """The f2d_sw_stage of RK stage k: earlier tendencies, this stage's
+        tendency (None on the last stage: not kept), fields updated."""
+        prev = [(x.u.x, x.u.y, x.h) for x in ds_prev]
+        out = (ds_out.u.x, ds_out.u.y, ds_out.h) if ds_out is not None else (None,) * 3
+        return _lib.sw_stage(self.param.integrator, k, dt, prev, out, y)
+
+    def _momentum(self, s, st, gradients, fill):
+        m, p = self.mesh, self.param
+        ptr = _lib.ptr
+        _ops("f2d_sw_momentum", self._dt(s.h), self._grid(), _lib.METHODS[p.vortexforce],
+             ptr(s.U.x), ptr(s.U.y), ptr(s.omega), ptr(m.d_ov.x), ptr(m.d_ov.y),
+             float(p.f0 * m.area * 0.25), ptr(s.ke) if gradients else None,
+             ptr(s.p) if gradients else None, ptr(m.d_mskx), ptr(m.d_msky), int(fill),
+             ctypes.byref(st))
+
+    def _diag_fused(self, s, energy):
+        m, p = self.mesh, self.param
+        ptr = _lib.ptr
+        method = _lib.METHODS.get(p.innerproduct, 0)
+        _ops("f2d_sw_diag", self._dt(s.h), self._grid(), method, ptr(s.u.x), ptr(s.u.y),
+             ptr(s.h), ptr(m.d_slipcoef), ptr(m.d_ok.x), ptr(m.d_ok.y), ptr(m.d_msk),
+             float(p.g / m.area), 0.0, ptr(s.U.x), ptr(s.U.y), ptr(s.omega),
+             ptr(s.ke) if energy else None, ptr(s.p) if energy else None)
+
     def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
         """One stage of the reference's RK (integrators.py:92-124): rhs,
         addto on u.x, u.y and h, diag."""
+        if self.fused:
+            s, alt = self._stage_fused(s, alt, ds_prev, ds_out, k, dt, stats)
+            if dt_next is not None:
+                op.cfl_dt(s.U, self.param.cfl, self.param.dtmax, dt_next)
+            return s, alt
         p = self.param
         if ds_out is None:
             if self._ds_last is None:
@@ -87,6 +131,16 @@ class _SW(_Base):
 
 class RSW(_SW):
     """Rotating shallow water (equations.py:139-157)."""
+
+    def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats):
+        m, p = self.mesh, self.param
+        st = self._stage_terms(ds_prev, ds_out, k, dt, (s.u.x, s.u.y, None))
+        self._momentum(s, st, gradients=True, fill=True)
+        op.transport_stage(m, p.compflux, s.h, alt.h, s.U, [x.h for x in ds_prev],
+                           ds_out.h if ds_out is not None else None, p.integrator, k, dt)
+        s, alt = s._replace(h=alt.h), alt._replace(h=s.h)
+        self._diag_fused(s, energy=True)
+        return s, alt
 
     def rhs(self, s, ds):
         m = self.mesh
@@ -118,6 +172,28 @@ class RSW(_SW):
 class QGRSW(_SW):
     """RSW tendency projected onto the QG manifold (equations.py:106-136,
     Thiry et al. 2024)."""
+
+    def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats):
+        m, p = self.mesh, self.param
+        ptr = _lib.ptr
+        if self._pre is None:
+            # the unprojected tendencies (the projection reads them while it
+            # writes the stage's tendency)
+            self._pre = allocate_device(self.prognostic, m.shape, s.h.dtype, s.h.device)
+        pre = self._pre
+        dt_, g = self._dt(s.h), self._grid()
+        self._momentum(s, _lib.sw_stage(None, 0, None, out=(pre.u.x, pre.u.y, None)),
+                       gradients=False, fill=False)
+        self._divflux(s, pre.h)
+        _ops("f2d_qg_pv", dt_, g, ptr(pre.u.x), ptr(pre.u.y), ptr(pre.h), ptr(m.d_slipcoef),
+             ptr(m.d_mskv), float(-p.f0 / p.H), ptr(s.pv))
+        m.qg_helmholtz.solve(s.pv, s.psi, stats=stats)
+        st = self._stage_terms(ds_prev, ds_out, k, dt, (s.u.x, s.u.y, s.h))
+        _ops("f2d_qg_project", dt_, g, ptr(s.psi), ptr(m.d_mskx), ptr(m.d_msky),
+             ptr(m.d_mskv), ptr(m.d_msk), float(p.f0 * m.area / p.g), ptr(pre.u.x),
+             ptr(pre.u.y), ptr(pre.h), ctypes.byref(st))
+        self._diag_fused(s, energy=False)
+        return s, alt
 
     def rhs(self, s, ds):
         m, p = self.mesh, self.param

@@ -255,6 +255,58 @@ int f2d_c2v_scaled(int dtype, const f2d_grid* g, const void* c, double scale,
 int f2d_v2c_scaled(int dtype, const f2d_grid* g, const void* vh, double scale,
                    const int8_t* mskv, const int8_t* msk, void* h, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Fused shallow-water stage (rsw, equations.py:139-157; qgrsw,
+ * equations.py:106-136): the operator sequence of a stage in few launches,
+ * each entry computed with the arithmetic and order of the one-operator
+ * entry points above, so FP64 results are bit for bit theirs.  Fields of
+ * a stage: 0 u.x, 1 u.y, 2 h.
+ * ------------------------------------------------------------------------ */
+typedef struct f2d_sw_stage {
+  int32_t integrator; /* F2D_RK3/F2D_EF/F2D_ENRK3; -1: tendency only       */
+  int32_t stage;      /* stage index (integrators.py:92-124)               */
+  const double* dt;   /* device float64 scalar: the step's dt              */
+  const void* prev[2][3]; /* tendencies of the earlier stages, [stage][field] */
+  void* out[3];       /* this stage's tendency (NULL: not stored)          */
+  void* y[3];         /* fields updated in place, y += the RK combination
+                         of addto, integrators.py:154-196 (NULL: none)     */
+} f2d_sw_stage;
+
+/* The momentum tendency: vortexforce of both components (weno.py:423-431,
+ * V = U.y resp. U.x, orders ovx/ovy), addcoriolis with f = f0*area*0.25
+ * (operators.py:32-39), addgrad(ke) then addgrad(p) when ke and p are
+ * given (operators.py:49-56), the halo fill when fill != 0 (meshes.py:
+ * 135-143), then the stage terms of fields 0 and 1 (st may be NULL: the
+ * tendency goes nowhere).  Reads no u: st->y may be u itself. */
+int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, const void* Uy,
+                    const void* omega, const int8_t* ovx, const int8_t* ovy, double f,
+                    const void* ke, const void* p, const int8_t* mskx, const int8_t* msky,
+                    int fill, const f2d_sw_stage* st, void* stream);
+/* The diag of rsw (equations.py:151-157) and qgrsw (equations.py:130-136):
+ * sharp (operators.py:59-64), compute_vorticity (operators.py:67-77),
+ * compute_kinetic_energy with `method` (operators.py:80-101) when ke is
+ * given, compute_pressure (operators.py:110-111) when p is given, and the
+ * fill of omega and ke. */
+int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const void* uy,
+                const void* h, const double* slipcoef, const int8_t* okx, const int8_t* oky,
+                const int8_t* msk, double g_over_area, double hb, void* Ux, void* Uy,
+                void* omega, void* ke, void* p, void* stream);
+/* qg_projection's right-hand side (operators.py:176-180, 194-201) from the
+ * unfilled, unprojected tendencies: pv = vorticity(du) + c2v(dh*scale),
+ * times mskv. */
+int f2d_qg_pv(int dtype, const f2d_grid* g, const void* dux, const void* duy, const void* dh,
+              const double* slipcoef, const int8_t* mskv, double scale, void* pv,
+              void* stream);
+/* The rest of qg_projection after the Helmholtz solve (operators.py:181-
+ * 183, 204-211) and of the stage: dh = v2c(psi*scale)*msk, du =
+ * perpgrad(psi) (covariant), the entries the slices leave keeping the
+ * unprojected tendency pre_*, fill, then the stage terms of the three
+ * fields.  st->out and st->y must not be the pre_* arrays. */
+int f2d_qg_project(int dtype, const f2d_grid* g, const void* psi, const int8_t* mskx,
+                   const int8_t* msky, const int8_t* mskv, const int8_t* msk, double scale,
+                   const void* pre_ux, const void* pre_uy, const void* pre_h,
+                   const f2d_sw_stage* st, void* stream);
+
 /* Bulk diagnostics, diagnostics.py:45-66 (Bulk.__call__): out[0..3] =
  * ke, ens, vort, angular from FP64 sums over every point of the padded
  * arrays (numpy's domsum), on the device; ke NULL (eulerpsi, whose state
