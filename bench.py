@@ -112,14 +112,17 @@ def warm(model, warmup):
     torch.cuda.synchronize()
 
 
-def timed_window(model, steps):
+def timed_window(model, steps, solver="poisson_vertices"):
     """Exactly `steps` steps between two HIP events, bracketed by device
-    synchronisations.  Raises if a step graph was captured inside."""
+    synchronisations.  Raises if a step graph was captured inside.  `solver`
+    names the mesh's elliptic solver whose statistics the window returns
+    (None: a model without one)."""
     import torch
     integ = model.integrator
-    pois = model.mesh.poisson_vertices
+    pois = getattr(model.mesh, solver) if solver else None
     integ.prepare_graph()  # the buffer assignment this window starts from
-    pois.reset_stats()
+    if pois is not None:
+        pois.reset_stats()
     ngraphs = integ.graph_count()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -131,13 +134,14 @@ def timed_window(model, steps):
     wall = time.perf_counter() - t0
     if integ.graph_count() != ngraphs:
         raise RuntimeError("a step graph was captured inside the timed window")
-    return e0.elapsed_time(e1) / steps, wall, pois.read_stats(), dts
+    return (e0.elapsed_time(e1) / steps, wall, pois.read_stats() if pois is not None else None,
+            dts)
 
 
-def timed(model, steps, warmup):
+def timed(model, steps, warmup, solver="poisson_vertices"):
     """warm(), then one timed window."""
     warm(model, warmup)
-    return timed_window(model, steps)
+    return timed_window(model, steps, solver)
 
 
 def accuracy(nx, seed, nsteps):
@@ -362,12 +366,14 @@ def _traffic_key(kernel, tile):
     return kernel
 
 
-def traffic_of(kernel, nx, prec):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 counter summary (profiles/traffic_<workload>.txt, JSON: FETCH_SIZE +
+def traffic_of(kernel, workload, prec):
+    """HBM bytes per launch of `kernel` on a workload (an eulerpsi grid size,
+    or a name such as "rsw2048x1024"), from the committed rocprofv3 counter
+    summary (profiles/traffic_<workload>_<precision>.txt, JSON: FETCH_SIZE +
     WRITE_SIZE per dispatch, scaled by the same counters on a copy kernel of
     known size), or None when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", f"traffic_eulerpsi{nx}_{prec}.txt")
+    name = f"eulerpsi{workload}" if isinstance(workload, int) else workload
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}_{prec}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -530,6 +536,114 @@ def cpu_baseline_closed(seed, steps, threads=(1,)):
             "iterations": [int(m.time.ite)] + [v["ite"] for v in runs.values()]}
 
 
+RSW_NX, RSW_NY = 2048, 1024
+
+
+def channel_islands_mask(nx, ny, seed, nisl=12):
+    """x-periodic channel with walls in y and `nisl` seeded circular
+    islands, radius U{20..60} cells, centres at least r + 3 cells from the
+    walls; the periodic halo columns equal their interior sources."""
+    m = np.ones((ny + 6, nx + 6), dtype=np.int8)
+    m[:3, :] = 0
+    m[-3:, :] = 0
+    rng = np.random.default_rng(seed)
+    jj, ii = np.mgrid[0:ny + 6, 0:nx + 6]
+    for _ in range(nisl):
+        r = int(rng.integers(20, 61))
+        cj = 3 + rng.uniform(r + 3, ny - r - 3)
+        ci = 3 + rng.uniform(0, nx)
+        d2 = (jj - cj) ** 2 + np.minimum(np.abs(ii - ci), nx - np.abs(ii - ci)) ** 2
+        m[d2 < r * r] = 0
+    m[:, :3] = m[:, nx:nx + 3]
+    m[:, nx + 3:] = m[:, 3:6]
+    return m
+
+
+def rsw_model(precision, seed, nx=RSW_NX, ny=RSW_NY):
+    """rsw on the nx x ny island channel (Lx = 2, Ly = 1, f0 = g = H = 1,
+    WENO5 everywhere, SSP-RK3, CFL 0.9), at rest, h = H (1 + 6 seeded
+    Gaussian bumps of amplitude 0.05) on the wet cells, times area."""
+    import fluids2d_amd as f2d
+    import fluids2d_amd.param as fp
+    fp.QUIET = True
+    p = f2d.Param()
+    p.model = "rsw"
+    p.nx, p.ny, p.Lx, p.Ly = nx, ny, 2.0, 1.0
+    p.xperiodic, p.yperiodic = True, False
+    p.f0 = p.g = p.H = 1.0
+    p.cfl = 0.9
+    p.compflux = p.vortexforce = p.innerproduct = "weno"
+    p.tend, p.maxite = 1e30, 10 ** 9
+    p.precision = precision
+    p.hipgraph = HIPGRAPH
+    m = f2d.Model(p)
+    m.mesh.msk[...] = channel_islands_mask(nx, ny, seed)
+    m.mesh.finalize()
+    m.integrator.model.mesh = m.mesh
+    x, y = m.mesh.xy("c")
+    rng = np.random.default_rng(seed + 1)
+    h = np.ones(m.mesh.shape)
+    for _ in range(6):
+        x0, y0 = rng.uniform(0, p.Lx), rng.uniform(0, p.Ly)
+        r = rng.uniform(0.03, 0.1) * p.Ly
+        h += 0.05 * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * r * r))
+    h[:, :3] = h[:, nx:nx + 3]
+    h[:, nx + 3:] = h[:, 3:6]
+    m.state.h[...] = h * p.H * m.mesh.area * m.mesh.msk
+    m.integrator.diag(m.state)
+    m.integrator.begin(m.state)
+    return m
+
+
+def rsw_bench(seed, steps, warmup=20):
+    """Step times of the rsw island channel in FP64 and FP32 (fused stage:
+    3 kernels per RK stage; the precisions' timed windows interleaved, as
+    for the headline workload), the live roofline of its kernels, and the
+    wet-cell mass drift of h over the timed steps (flux form, closed walls:
+    conserved to rounding)."""
+    import torch
+    from fluids2d_amd import roofline as rl
+    out = {"workload": f"rsw {RSW_NX}x{RSW_NY} x-periodic channel, walls in y, 12 seeded "
+                       f"islands (seed {seed}), f0 = g = H = 1, WENO5, SSP-RK3, CFL 0.9, "
+                       f"at rest with Gaussian bumps in h",
+           "steps": steps, "windows": WINDOWS}
+    precs = ("fp64", "fp32")
+    models, wet, m0 = {}, {}, {}
+    for prec in precs:
+        m = rsw_model(prec, seed)
+        w = (m.mesh.d_msk == 1)
+        w[:, :3] = False
+        w[:, -3:] = False
+        wet[prec] = w
+        m0[prec] = float(m.integrator.dstate.h[w].double().sum())
+        warm(m, warmup)
+        models[prec] = m
+    windows = {prec: [] for prec in precs}
+    for _ in range(WINDOWS):
+        for prec in precs:
+            windows[prec].append(timed_window(models[prec], steps, solver=None)[0])
+    for prec in precs:
+        m = models[prec]
+        ms = sum(windows[prec]) / len(windows[prec])
+        m1 = float(m.integrator.dstate.h[wet[prec]].double().sum())
+        prof = rl.sw_timing(m)
+        out[f"{prec}_ms_per_step"] = ms
+        out[f"{prec}_windows_ms_per_step"] = windows[prec]
+        out[f"{prec}_cell_steps_per_s"] = RSW_NX * RSW_NY / (ms * 1e-3)
+        out[f"{prec}_mass_drift"] = abs(m1 - m0[prec]) / m0[prec]
+        out[f"{prec}_roofline"] = {
+            "bound": "hbm", "achieved": prof["achieved_gbs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": prof["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic_of(prof["kernel"].split()[0], f"rsw{RSW_NX}x{RSW_NY}", prec),
+            "kernel": prof["kernel"], "bytes_per_launch": prof["bytes"],
+            "avg_launch_us": prof["us"],
+            "kernels": {k: {"us": round(v["us"], 2), "per_step": v["per_step"],
+                            "gbs": round(v["gbs"], 1)} for k, v in prof["all"].items()}}
+    del models, m
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -553,6 +667,10 @@ def main():
                     help="also time the batched qgrsw ensemble with this many members "
                          "(0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--rsw-steps", type=int, default=200,
+                    help="steps of the rsw 2048x1024 island channel per precision (0: skip)")
+    ap.add_argument("--only-rsw", action="store_true",
+                    help="time only the rsw island channel (for profiling its kernels)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB three times "
                          "(a kernel of known traffic, to scale rocprofv3 counters)")
@@ -574,6 +692,10 @@ def main():
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+
+    if a.only_rsw:
+        print(json.dumps({"rsw": rsw_bench(a.seed, a.rsw_steps or a.steps, a.warmup)}))
+        return
 
     from fluids2d_amd import autotune
     nx = a.nx
@@ -696,6 +818,8 @@ def main():
         if cpu is not None:
             cpu["closed256"] = cpu_baseline_closed(a.seed, a.closed_steps,
                                                    sorted({1, cpu["cores"]} | {int(t) for t in cpu["by_threads"]}))
+    if a.rsw_steps > 0:
+        out["rsw"] = rsw_bench(a.seed, a.rsw_steps)
     if a.ensemble > 0:
         out["ensemble"] = ensemble_bench(a.ensemble, a.ensemble_steps,
                                          log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -26,9 +26,11 @@
 namespace f2d {
 
 // numba's negative-index wrap (and, never reached with valid orders, the
-// high side), as the flat loops of f2d_stencil.hip
-__device__ __forceinline__ int64_t wrn(int64_t i, int64_t n) {
-  return i < 0 ? i + n : (i >= n ? i - n : i);
+// high side), as the flat loops of f2d_stencil.hip.  Flat indices are 32-bit
+// here: the entry points refuse grids of 2^31 - 4 (nx + 6) points or more.
+template <bool WRAP>
+__device__ __forceinline__ int ix(int i, int n) {
+  return WRAP ? (i < 0 ? i + n : (i >= n ? i - n : i)) : i;
 }
 
 struct SwGeo {
@@ -42,39 +44,62 @@ __device__ __forceinline__ int wsrc_j(const SwGeo& g, int j) {
   return g.yper ? (j < 3 ? j + g.ny : (j >= g.ny + 3 ? j - g.ny : j)) : j;
 }
 
+// Workgroups of 64 x 4 points (columns x rows) over the padded array, in
+// tiles numbered row-major.  The hardware places workgroup b on XCD b % 8;
+// workgroup b takes the (b / 8)-th tile of a contiguous band per XCD, so the
+// rows a tile's stencils share with the tiles above and below it are mostly
+// fetched into the same XCD's L2.
+constexpr int TW = 64, TH = 4;
+
+__device__ __forceinline__ bool tile_point(const SwGeo& g, int& i, int& j) {
+  const int gx = (g.NX + TW - 1) / TW;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b & 7, q8 = nb >> 3, rem = nb & 7;
+  const int t = x * q8 + (x < rem ? x : rem) + (b >> 3);
+  const int ty = t / gx, tx = t - ty * gx;
+  i = tx * TW + threadIdx.x;
+  j = ty * TH + threadIdx.y;
+  return i < g.NX && j < g.NY;
+}
+
+// Whether every flat index a stencil of source row js reaches (rows js - 2
+// to js + 3) lies inside the array: then no wrap is needed.
+__device__ __forceinline__ bool no_wrap(const SwGeo& g, int js) {
+  return js >= 2 && js <= g.NY - 4;
+}
+
 // vortexforce_on_interval at flat index i (weno.py:370-384): V the
 // transverse contravariant velocity, q the vertex vorticity, o the order
-template <class T>
-__device__ __forceinline__ T vforce(int method, const T* V, const T* q, int o, int64_t i,
-                                    int64_t s, int64_t s2, T sign, int64_t n) {
+template <class T, int M, bool W>
+__device__ __forceinline__ T vforce(const T* V, const T* q, int o, int i, int s, int s2,
+                                    T sign, int n) {
   if (o <= 0) return T(0);
-  const T Vm = T(0.25) * (((V[i] + V[wrn(i + s, n)]) + V[wrn(i - s2, n)]) +
-                          V[wrn(i + s - s2, n)]);
+  const T Vm = T(0.25) * (((V[i] + V[ix<W>(i + s, n)]) + V[ix<W>(i - s2, n)]) +
+                          V[ix<W>(i + s - s2, n)]);
   T f;
   if (o > 4)
-    f = f5<T>(method, Vm, q[wrn(i - 2 * s, n)], q[wrn(i - s, n)], q[i], q[wrn(i + s, n)],
-              q[wrn(i + 2 * s, n)], q[wrn(i + 3 * s, n)]);
+    f = f5<T>(M, Vm, q[ix<W>(i - 2 * s, n)], q[ix<W>(i - s, n)], q[i], q[ix<W>(i + s, n)],
+              q[ix<W>(i + 2 * s, n)], q[ix<W>(i + 3 * s, n)]);
   else if (o > 2)
-    f = f3<T>(method, Vm, q[wrn(i - s, n)], q[i], q[wrn(i + s, n)], q[wrn(i + 2 * s, n)]);
+    f = f3<T>(M, Vm, q[ix<W>(i - s, n)], q[i], q[ix<W>(i + s, n)], q[ix<W>(i + 2 * s, n)]);
   else
-    f = f1<T>(method, Vm, q[i], q[wrn(i + s, n)]);
+    f = f1<T>(M, Vm, q[i], q[ix<W>(i + s, n)]);
   return (sign * f) * Vm;
 }
 
 // innerproduct_on_interval term at flat index i (weno.py:391-403), with the
 // contravariant velocity U = u * sc formed from u as sharp stores it
-template <class T>
-__device__ __forceinline__ T iprod(int method, const T* u, T sc, int o, int64_t i,
-                                   int64_t s, int64_t n) {
-  const T Um = T(0.5) * (u[i] * sc + u[wrn(i + s, n)] * sc);
+template <class T, int M, bool W>
+__device__ __forceinline__ T iprod(const T* u, T sc, int o, int i, int s, int n) {
+  const T Um = T(0.5) * (u[i] * sc + u[ix<W>(i + s, n)] * sc);
   T f;
   if (o > 4)
-    f = f5<T>(method, Um, u[wrn(i - 2 * s, n)], u[wrn(i - s, n)], u[i], u[wrn(i + s, n)],
-              u[wrn(i + 2 * s, n)], u[wrn(i + 3 * s, n)]);
+    f = f5<T>(M, Um, u[ix<W>(i - 2 * s, n)], u[ix<W>(i - s, n)], u[i], u[ix<W>(i + s, n)],
+              u[ix<W>(i + 2 * s, n)], u[ix<W>(i + 3 * s, n)]);
   else if (o > 2)
-    f = f3<T>(method, Um, u[wrn(i - s, n)], u[i], u[wrn(i + s, n)], u[wrn(i + 2 * s, n)]);
+    f = f3<T>(M, Um, u[ix<W>(i - s, n)], u[i], u[ix<W>(i + s, n)], u[ix<W>(i + 2 * s, n)]);
   else
-    f = f1<T>(method, Um, u[i], u[wrn(i + s, n)]);
+    f = f1<T>(M, Um, u[i], u[ix<W>(i + s, n)]);
   return f * Um;
 }
 
@@ -89,8 +114,8 @@ struct Terms {
 };
 
 template <class T>
-__device__ __forceinline__ void stage_out(const Terms<T>& t, int nc, const double c[3],
-                                          int64_t k, T ds) {
+__device__ __forceinline__ void stage_out(const Terms<T>& t, int nc, const double c[3], int k,
+                                          T ds) {
   if (t.out) t.out[k] = ds;
   if (t.y && nc > 0) {
     const T x0 = (nc == 1) ? ds : t.p0[k];
@@ -103,7 +128,7 @@ __device__ __forceinline__ void stage_out(const Terms<T>& t, int nc, const doubl
 template <class T>
 struct MomArgs {
   SwGeo g;
-  int method, fill, integ, stage;
+  int fill, integ, stage;
   const T* Ux;
   const T* Uy;
   const T* w;
@@ -118,20 +143,30 @@ struct MomArgs {
   const double* dt;
 };
 
-template <class T>
+// vortex force of both components at source point q (row js, column is)
+template <class T, int M, bool W>
+__device__ __forceinline__ void vforce2(const MomArgs<T>& a, int q, int n, T& vx, T& vy) {
+  const int NX = a.g.NX;
+  // du.x: vortexforce (V = U.y, s = yshift, s2 = xshift, sign +1)
+  vx = vforce<T, M, W>(a.Uy, a.w, a.ovy[q], q, NX, 1, T(1), n);
+  // du.y: vortexforce (V = U.x, s = xshift, s2 = yshift, sign -1)
+  vy = vforce<T, M, W>(a.Ux, a.w, a.ovx[q], q, 1, NX, T(-1), n);
+}
+
+template <class T, int M>
 __global__ __launch_bounds__(256) void k_sw_momentum(MomArgs<T> a) {
   const SwGeo& g = a.g;
-  const int64_t n = (int64_t)g.NY * g.NX;
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  const int NX = g.NX, NY = g.NY;
-  const int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+  int i, j;
+  if (!tile_point(g, i, j)) return;
+  const int NX = g.NX, NY = g.NY, n = NY * NX;
+  const int k = j * NX + i;
   const int js = a.fill ? wsrc_j(g, j) : j, is = a.fill ? wsrc_i(g, i) : i;
-  const int64_t q = (int64_t)js * NX + is;
-  // du.x: vortexforce (V = U.y, s = yshift, s2 = xshift, sign +1)
-  T vx = vforce<T>(a.method, a.Uy, a.w, a.ovy[q], q, NX, 1, T(1), n);
-  // du.y: vortexforce (V = U.x, s = xshift, s2 = yshift, sign -1)
-  T vy = vforce<T>(a.method, a.Ux, a.w, a.ovx[q], q, 1, NX, T(-1), n);
+  const int q = js * NX + is;
+  T vx, vy;
+  if (no_wrap(g, js))
+    vforce2<T, M, false>(a, q, n, vx, vy);
+  else
+    vforce2<T, M, true>(a, q, n, vx, vy);
   const T ff = T(a.f);
   if (js < NY - 1 && is >= 1 && is < NX - 1)
     vx = vx + ff * (((a.Uy[q - 1] + a.Uy[q + NX - 1]) + a.Uy[q]) + a.Uy[q + NX]);
@@ -154,7 +189,6 @@ __global__ __launch_bounds__(256) void k_sw_momentum(MomArgs<T> a) {
 template <class T>
 struct DiagArgs {
   SwGeo g;
-  int method;
   const T* ux;
   const T* uy;
   const T* h;
@@ -170,14 +204,24 @@ struct DiagArgs {
   T* p;   // nullable (qgrsw)
 };
 
-template <class T>
+// compute_kinetic_energy, "weno"-family methods (operators.py:93-101):
+// ke = 0; ke += U.x-term; ke += U.y-term (then ke *= msk * 0.5)
+template <class T, int M, bool W>
+__device__ __forceinline__ T ke_sum(const DiagArgs<T>& a, T sx, T sy, int q, int n) {
+  T v = T(0);
+  const int ox = a.okx[q], oy = a.oky[q];
+  if (ox > 0) v = v + iprod<T, M, W>(a.ux, sx, ox, q, 1, n);
+  if (oy > 0) v = v + iprod<T, M, W>(a.uy, sy, oy, q, a.g.NX, n);
+  return v;
+}
+
+template <class T, int M>
 __global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a) {
   const SwGeo& g = a.g;
-  const int64_t n = (int64_t)g.NY * g.NX;
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  const int NX = g.NX;
-  const int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+  int i, j;
+  if (!tile_point(g, i, j)) return;
+  const int NX = g.NX, n = g.NY * NX;
+  const int k = j * NX + i;
   const T sx = T(a.sx), sy = T(a.sy);
   // sharp (operators.py:59-64), at every point
   a.Ux[k] = a.ux[k] * sx;
@@ -186,7 +230,7 @@ __global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a) {
   if (a.p) a.p[k] = T(a.goa) * (a.h[k] + T(a.hb));
   // vorticity and kinetic energy are filled: the source's value
   const int js = wsrc_j(g, j), is = wsrc_i(g, i);
-  const int64_t q = (int64_t)js * NX + is;
+  const int q = js * NX + is;
   // compute_vorticity (operators.py:67-77): row 0 keeps its old value; it
   // has images only through a periodic x (y closed), which its own thread
   // writes
@@ -201,12 +245,8 @@ __global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a) {
     }
   }
   if (a.ke) {
-    // compute_kinetic_energy, "weno"-family methods (operators.py:93-101):
-    // ke = 0; ke += U.x-term; ke += U.y-term; ke *= msk * 0.5
-    T v = T(0);
-    const int ox = a.okx[q], oy = a.oky[q];
-    if (ox > 0) v = v + iprod<T>(a.method, a.ux, sx, ox, q, 1, n);
-    if (oy > 0) v = v + iprod<T>(a.method, a.uy, sy, oy, q, NX, n);
+    const T v = no_wrap(g, js) ? ke_sum<T, M, false>(a, sx, sy, q, n)
+                               : ke_sum<T, M, true>(a, sx, sy, q, n);
     a.ke[k] = T((double)v * ((double)a.msk[q] * 0.5));
   }
 }
@@ -221,11 +261,10 @@ __global__ __launch_bounds__(256) void k_qg_pv(SwGeo g, const T* __restrict__ du
                                                const double* __restrict__ slip,
                                                const int8_t* __restrict__ mv, double scale,
                                                T* __restrict__ pv) {
-  const int64_t n = (int64_t)g.NY * g.NX;
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
+  int i, j;
+  if (!tile_point(g, i, j)) return;
   const int NX = g.NX;
-  const int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+  const int k = j * NX + i;
   T v = j >= 1 ? -(dux[k] - dux[k - NX]) : pv[k];
   if (i >= 1) v = v + (duy[k] - duy[k - 1]);
   T val = T((double)v * slip[k]);
@@ -264,13 +303,12 @@ struct ProjArgs {
 template <class T>
 __global__ __launch_bounds__(256) void k_qg_project(ProjArgs<T> a) {
   const SwGeo& g = a.g;
-  const int64_t n = (int64_t)g.NY * g.NX;
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
+  int i, j;
+  if (!tile_point(g, i, j)) return;
   const int NX = g.NX, NY = g.NY;
-  const int j = (int)(k / NX), i = (int)(k - (int64_t)j * NX);
+  const int k = j * NX + i;
   const int js = wsrc_j(g, j), is = wsrc_i(g, i);
-  const int64_t q = (int64_t)js * NX + is;
+  const int q = js * NX + is;
   const T* psi = a.psi;
   T dh;
   if (js < NY - 1 && is < NX - 1) {
@@ -313,6 +351,8 @@ static SwGeo sw_geo(const f2d_grid* g) {
   F2D_REQUIRE(g->halowidth == 3, "halowidth must be 3");
   F2D_REQUIRE(g->ny > 0 && g->nx > 0 && g->ny < (1 << 24) && g->nx < (1 << 24),
               "grid is empty or too large");
+  F2D_REQUIRE((g->ny + 6) * (g->nx + 6) + 4 * (g->nx + 6) < ((int64_t)1 << 31),
+              "grid too large for the fused shallow-water kernels (2^31 points)");
   SwGeo s;
   s.ny = (int)g->ny;
   s.nx = (int)g->nx;
@@ -347,9 +387,12 @@ static int stage_check(const f2d_sw_stage* st, int nf) {
   return st->integrator;
 }
 
+// launch shape of the tiled kernels: one workgroup of TW x TH threads per
+// tile (tile_point)
 static unsigned nblk(const SwGeo& g) {
-  return (unsigned)(((int64_t)g.NY * g.NX + 255) / 256);
+  return (unsigned)(((g.NX + TW - 1) / TW) * ((g.NY + TH - 1) / TH));
 }
+static const dim3 TILE_BLOCK(TW, TH);
 
 extern "C" {
 
@@ -367,7 +410,6 @@ int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, co
     F2D_SW_DISPATCH(dtype, {
       MomArgs<T> a{};
       a.g = geo;
-      a.method = method;
       a.fill = fill;
       a.integ = integ;
       a.stage = st ? st->stage : 0;
@@ -384,7 +426,14 @@ int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, co
       a.tx = terms<T>(st, 0);
       a.ty = terms<T>(st, 1);
       a.dt = st ? st->dt : nullptr;
-      k_sw_momentum<T><<<nblk(geo), 256, 0, S(stream)>>>(a);
+      switch (method) {
+        case F2D_WENO: k_sw_momentum<T, F2D_WENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+        case F2D_UPWIND: k_sw_momentum<T, F2D_UPWIND><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+        case F2D_CENTERED:
+          k_sw_momentum<T, F2D_CENTERED><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a);
+          break;
+        default: k_sw_momentum<T, F2D_CWENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+      }
     });
   });
 }
@@ -402,7 +451,6 @@ int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const 
     F2D_SW_DISPATCH(dtype, {
       DiagArgs<T> a{};
       a.g = geo;
-      a.method = method;
       a.ux = (const T*)ux;
       a.uy = (const T*)uy;
       a.h = (const T*)h;
@@ -419,7 +467,12 @@ int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const 
       a.w = (T*)omega;
       a.ke = (T*)ke;
       a.p = (T*)p;
-      k_sw_diag<T><<<nblk(geo), 256, 0, S(stream)>>>(a);
+      switch (ke ? method : F2D_WENO) {
+        case F2D_WENO: k_sw_diag<T, F2D_WENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+        case F2D_UPWIND: k_sw_diag<T, F2D_UPWIND><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+        case F2D_CENTERED: k_sw_diag<T, F2D_CENTERED><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+        default: k_sw_diag<T, F2D_CWENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
+      }
     });
   });
 }
@@ -430,7 +483,7 @@ int f2d_qg_pv(int dtype, const f2d_grid* g, const void* dux, const void* duy, co
   return guarded([&] {
     const SwGeo geo = sw_geo(g);
     F2D_REQUIRE(dux && duy && dh && slipcoef && mskv && pv, "NULL argument");
-    F2D_SW_DISPATCH(dtype, k_qg_pv<T><<<nblk(geo), 256, 0, S(stream)>>>(
+    F2D_SW_DISPATCH(dtype, k_qg_pv<T><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(
                                geo, (const T*)dux, (const T*)duy, (const T*)dh, slipcoef,
                                mskv, scale, (T*)pv));
   });
@@ -469,7 +522,7 @@ int f2d_qg_project(int dtype, const f2d_grid* g, const void* psi, const int8_t* 
       a.ty = terms<T>(st, 1);
       a.th = terms<T>(st, 2);
       a.dt = st->dt;
-      k_qg_project<T><<<nblk(geo), 256, 0, S(stream)>>>(a);
+      k_qg_project<T><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a);
     });
   });
 }

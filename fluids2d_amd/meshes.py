@@ -57,13 +57,27 @@ class Mesh:
         self.set_stencils(self.param.maxorder)
         # the Neumann "c" operator of the euler pressure projection
         self.poisson_centers = Poisson2D(self, "c") if self.param.model == "euler" else None
-        self.poisson_vertices = Poisson2D(self, "v")
+        # the vertex solvers are set up on first use (rsw uses neither):
+        # attribute access builds them from this finalize's masks
+        for name in self._LAZY:
+            self.__dict__.pop(name, None)
+        self._factories = {"poisson_vertices": lambda: Poisson2D(self, "v")}
         if self.param.model in ["qg", "qgrsw", "rsw"]:
             f0, H, g = self.param.f0, self.param.H, self.param.g
             maindiag = self.area * f0 ** 2 / (g * H)
-            self.qg_helmholtz = Poisson2D(self, "v", maindiag=maindiag)
+            self._factories["qg_helmholtz"] = lambda: Poisson2D(self, "v", maindiag=maindiag)
             self.hb = 0
             self.qgcoef = f0 / +H
+
+    _LAZY = ("poisson_vertices", "qg_helmholtz")
+
+    def __getattr__(self, name):
+        make = self.__dict__.get("_factories", {}).get(name)
+        if make is None:
+            raise AttributeError(f"{type(self).__name__!r} object has no attribute {name!r}")
+        solver = make()
+        setattr(self, name, solver)
+        return solver
 
     def _allocate(self):
         return np.zeros(self.shape, dtype="i1")

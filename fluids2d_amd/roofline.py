@@ -99,3 +99,54 @@ def profile_dominant(model, reps=20):
             "achieved_gbs": r["bytes"] / (r["us"] * 1e-6) / 1e9,
             "all": {k: dict(v, gbs=v["bytes"] / (v["us"] * 1e-6) / 1e9)
                     for k, v in rows.items()}}
+
+
+def sw_timing(model, reps=20):
+    """Mean launch time and algorithmic bytes of the fused rsw stage kernels
+    at stage 1 of RK3, each replayed back to back from a graph: the momentum
+    kernel (reads U.x, U.y, omega, ke, p, the stage-0 tendency and u, two
+    orders and two masks; writes u and the stage-1 tendency), the transport
+    stage of h (reads h, U.x, U.y, the stage-0 tendency, two orders and msk;
+    writes h and the tendency) and the diag kernel (reads u.x, u.y, h, the
+    float64 slip coefficient, two orders and msk; writes U.x, U.y, omega, ke,
+    p).  The updates go to scratch copies: the model's state is not
+    advanced (the diag rewrites the diagnostics it already holds)."""
+    integ = model.integrator
+    sw = integ.model
+    s = integ.dstate
+    m, p = model.mesh, model.param
+    d0, d1 = integ.dscratch[0], integ.dscratch[1]
+    ux, uy, h = s.u.x.clone(), s.u.y.clone(), torch.empty_like(s.h)
+    st = sw._stage_terms([d0], d1, 1, integ.d_dt, (ux, uy, None))
+    t = s.h.element_size()
+    npts = s.h.numel()
+    launches = {
+        "k_sw_momentum (vortex force + Coriolis + gradients + fill + RK update)":
+            (lambda: sw._momentum(s, st, gradients=True, fill=True), npts * (13 * t + 4)),
+        "k_transport (fused WENO5 flux + div + RK update of h)":
+            (lambda: op.transport_stage(m, p.compflux, s.h, h, s.U, [d0.h], d1.h,
+                                        p.integrator, 1, integ.d_dt), npts * (6 * t + 3)),
+        "k_sw_diag (sharp + vorticity + kinetic energy + pressure + fills)":
+            (lambda: sw._diag_fused(s, energy=True), npts * (8 * t + 11)),
+    }
+    rows = {}
+    for name, (fn, byts) in launches.items():
+        fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with _lib.capture(g):
+            for _ in range(reps):
+                fn()
+        g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        rows[name] = dict(us=us, bytes=byts, per_step=3, gbs=byts / (us * 1e-6) / 1e9)
+    name = max(rows, key=lambda k: rows[k]["us"] * rows[k]["per_step"])
+    r = rows[name]
+    return {"kernel": name, "us": r["us"], "bytes": r["bytes"], "achieved_gbs": r["gbs"],
+            "all": rows}

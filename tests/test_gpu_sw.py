@@ -160,21 +160,10 @@ def test_qgrsw_nan_cells_like_reference(dev):
     assert _rel(ds.h[ok], ds_ref.h[ok]) < 1e-11
 
 
-def _c3_mask(nx, ny, seed, nisl=12):
-    """12 seeded circular islands, radius U{20..60} cells, centres at least
-    r + 3 cells from the walls (x periodic, walls in y)."""
-    m = O.default_mask(ny, nx, 3, True, False)
-    rng = np.random.default_rng(seed)
-    jj, ii = np.mgrid[0:ny + 6, 0:nx + 6]
-    for _ in range(nisl):
-        r = int(rng.integers(20, 61))
-        cj = 3 + rng.uniform(r + 3, ny - r - 3)
-        ci = 3 + rng.uniform(0, nx)
-        d2 = (jj - cj) ** 2 + np.minimum(np.abs(ii - ci), nx - np.abs(ii - ci)) ** 2
-        m[d2 < r * r] = 0
-    m[:, :3] = m[:, nx:nx + 3]  # periodic halo columns
-    m[:, nx + 3:] = m[:, 3:6]
-    return m
+def _c3_mask(nx, ny, seed):
+    """The island channel of the rsw benchmark workload (bench.py)."""
+    from bench import channel_islands_mask
+    return channel_islands_mask(nx, ny, seed)
 
 
 def test_rsw_2048x1024_islands(dev):
