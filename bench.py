@@ -636,7 +636,7 @@ def rsw_bench(seed, steps, warmup=20):
             "unit": "GB/s", "frac": prof["achieved_gbs"] / HBM_PEAK_GBS,
             "traffic": traffic_of(prof["kernel"].split()[0], f"rsw{RSW_NX}x{RSW_NY}", prec),
             "kernel": prof["kernel"], "bytes_per_launch": prof["bytes"],
-            "avg_launch_us": prof["us"],
+            "avg_launch_us": prof["us"], "launches_timed": prof["all"][prof["kernel"]]["launches"],
             "kernels": {k: {"us": round(v["us"], 2), "per_step": v["per_step"],
                             "gbs": round(v["gbs"], 1)} for k, v in prof["all"].items()}}
     del models, m

@@ -28,8 +28,8 @@ EXPORTS = [
     "f2d_poisson_method", "f2d_coriolis", "f2d_addgrad", "f2d_sharp", "f2d_vorticity",
     "f2d_ke_finish", "f2d_pressure", "f2d_div", "f2d_c2v_scaled", "f2d_v2c_scaled",
     "f2d_adj_stage", "f2d_adj_combine", "f2d_bulk_diagnostics",
-    "f2d_perpgrad_cfl_noalloc", "f2d_sw_momentum", "f2d_sw_diag", "f2d_qg_pv",
-    "f2d_qg_project",
+    "f2d_perpgrad_cfl_noalloc", "f2d_sw_momentum", "f2d_sw_momentum_u", "f2d_sw_diag",
+    "f2d_qg_pv", "f2d_qg_project", "f2d_transport_stage_u",
 ]
 
 
@@ -69,12 +69,16 @@ class SwStage(ctypes.Structure):
     fields 0 u.x, 1 u.y, 2 h."""
     _fields_ = [("integrator", ctypes.c_int32), ("stage", ctypes.c_int32),
                 ("dt", ctypes.c_void_p), ("prev", (ctypes.c_void_p * 3) * 2),
-                ("out", ctypes.c_void_p * 3), ("y", ctypes.c_void_p * 3)]
+                ("out", ctypes.c_void_p * 3), ("y", ctypes.c_void_p * 3),
+                ("y_in", ctypes.c_void_p * 3), ("dt_log", ctypes.c_void_p)]
 
 
-def sw_stage(integrator, stage, dt, prev=(), out=(None, None, None), y=(None, None, None)):
+def sw_stage(integrator, stage, dt, prev=(), out=(None, None, None), y=(None, None, None),
+             dt_log=None, y_in=(None, None, None)):
     """An SwStage from tensors (None for absent arrays); integrator None:
-    tendency only.  prev: the earlier stages' (u.x, u.y, h) tendencies."""
+    tendency only.  prev: the earlier stages' (u.x, u.y, h) tendencies;
+    y_in: the arrays y is computed from (ping-pong; None: y in place);
+    dt_log: a one-element float64 tensor the momentum kernel copies dt to."""
     def a(t):
         return None if t is None else t.data_ptr()
     st = SwStage()
@@ -87,6 +91,8 @@ def sw_stage(integrator, stage, dt, prev=(), out=(None, None, None), y=(None, No
     for f in range(3):
         st.out[f] = a(out[f])
         st.y[f] = a(y[f])
+        st.y_in[f] = a(y_in[f])
+    st.dt_log = a(dt_log)
     return st
 
 
@@ -156,6 +162,10 @@ def lib():
         "f2d_bulk_diagnostics": (i32, [i32, gp, vp, vp, vp, vp, vp, vp, i32, vp, vp]),
         "f2d_sw_momentum": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, d, vp, vp, vp, vp, i32,
                                   ctypes.POINTER(SwStage), vp]),
+        "f2d_sw_momentum_u": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, d, vp, vp, d, d, vp, vp,
+                                    i32, ctypes.POINTER(SwStage), vp]),
+        "f2d_transport_stage_u": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
+                                        vp, vp, vp, i32, i32, vp, vp]),
         "f2d_sw_diag": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp, d, d, vp, vp, vp, vp,
                               vp, vp]),
         "f2d_qg_pv": (i32, [i32, gp, vp, vp, vp, vp, vp, d, vp, vp]),

@@ -133,6 +133,10 @@ struct TransportArgs {
   const int8_t* pmy;
   double psx, psy;
   int pfill;
+  // U = sharp(u) formed on the fly (uscale != 0): Ux/Uy hold u.x/u.y and
+  // the velocity is u.x * usx, u.y * usy, as k_sharp stores it
+  int uscale;
+  double usx, usy;
 };
 
 // U at x face (j, i): read, or perpgrad(psi) with k_perpgrad's arithmetic
@@ -142,7 +146,10 @@ struct TransportArgs {
 // unchanged.
 template <class T>
 __device__ __forceinline__ T ux_at(const TransportArgs<T>& a, int j, int i) {
-  if (!a.psi) return a.Ux[(int64_t)j * a.P + i];
+  if (!a.psi) {
+    const T u = a.Ux[(int64_t)j * a.P + i];
+    return a.uscale ? u * T(a.usx) : u;
+  }
   int js = j, is = i;
   if (a.pfill) {
     if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
@@ -155,7 +162,10 @@ __device__ __forceinline__ T ux_at(const TransportArgs<T>& a, int j, int i) {
 
 template <class T>
 __device__ __forceinline__ T uy_at(const TransportArgs<T>& a, int j, int i) {
-  if (!a.psi) return a.Uy[(int64_t)j * a.P + i];
+  if (!a.psi) {
+    const T u = a.Uy[(int64_t)j * a.P + i];
+    return a.uscale ? u * T(a.usy) : u;
+  }
   int js = j, is = i;
   if (a.pfill) {
     if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
@@ -435,6 +445,9 @@ __global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a) {
     if (a.psi) {
       ux[i] = ((-(a.psi[k + P] - a.psi[k])) * T(a.pmx[k])) * T(a.psx);
       uy[i] = ((a.psi[k + one] - a.psi[k]) * T(a.pmy[k])) * T(a.psy);
+    } else if (a.uscale) {
+      ux[i] = a.Ux[k] * T(a.usx);
+      uy[i] = a.Uy[k] * T(a.usy);
     } else {
       ux[i] = a.Ux[k];
       uy[i] = a.Uy[k];
@@ -971,7 +984,7 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
                                 const int8_t* ocx, const int8_t* ocy, const int8_t* msk,
                                 const void* ds_prev0, const void* ds_prev1, void* ds_out,
                                 int integrator, int stage, const double* dt, void* stream,
-                                double* dt_log = nullptr) {
+                                double* dt_log = nullptr, bool from_u = false) {
   return guarded([&] {
     check_grid(g);
     check_method(method);
@@ -1006,6 +1019,9 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
       a.psx = sx;
       a.psy = sy;
       a.pfill = fill;
+      a.uscale = from_u;
+      a.usx = 1.0 / (g->dx * g->dx);  // sharp's 1/dx**2, 1/dy**2
+      a.usy = 1.0 / (g->dy * g->dy);
       a.ocx = ocx;
       a.ocy = ocy;
       a.msk = msk;
@@ -1052,6 +1068,16 @@ int f2d_transport_stage_psi(int dtype, const f2d_grid* g, int method, const void
   return transport_stage_impl(dtype, g, method, q_in, q_out, nullptr, nullptr, psi, mskx,
                               msky, sx, sy, fill, ocx, ocy, msk, ds_prev0, ds_prev1, ds_out,
                               integrator, stage, dt, stream);
+}
+
+int f2d_transport_stage_u(int dtype, const f2d_grid* g, int method, const void* q_in,
+                          void* q_out, const void* ux, const void* uy, const int8_t* ocx,
+                          const int8_t* ocy, const int8_t* msk, const void* ds_prev0,
+                          const void* ds_prev1, void* ds_out, int integrator, int stage,
+                          const double* dt, void* stream) {
+  return transport_stage_impl(dtype, g, method, q_in, q_out, ux, uy, nullptr, nullptr,
+                              nullptr, 0.0, 0.0, 0, ocx, ocy, msk, ds_prev0, ds_prev1,
+                              ds_out, integrator, stage, dt, stream, nullptr, true);
 }
 
 int f2d_transport_stage_log(int dtype, const f2d_grid* g, int method, const void* q_in,

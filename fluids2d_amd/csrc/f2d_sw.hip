@@ -70,12 +70,21 @@ __device__ __forceinline__ bool no_wrap(const SwGeo& g, int js) {
 
 // vortexforce_on_interval at flat index i (weno.py:370-384): V the
 // transverse contravariant velocity, q the vertex vorticity, o the order
-template <class T, int M, bool W>
-__device__ __forceinline__ T vforce(const T* V, const T* q, int o, int i, int s, int s2,
+// V read from the array, or (SC) formed as the array times vs: the
+// contravariant velocity sharp(u) = u * (1/dx**2, 1/dy**2) as k_sharp
+// stores it
+template <class T, bool SC>
+__device__ __forceinline__ T vel(const T* V, T vs, int x) {
+  return SC ? V[x] * vs : V[x];
+}
+
+template <class T, int M, bool W, bool SC>
+__device__ __forceinline__ T vforce(const T* V, T vs, const T* q, int o, int i, int s, int s2,
                                     T sign, int n) {
   if (o <= 0) return T(0);
-  const T Vm = T(0.25) * (((V[i] + V[ix<W>(i + s, n)]) + V[ix<W>(i - s2, n)]) +
-                          V[ix<W>(i + s - s2, n)]);
+  const T Vm = T(0.25) * (((vel<T, SC>(V, vs, i) + vel<T, SC>(V, vs, ix<W>(i + s, n))) +
+                           vel<T, SC>(V, vs, ix<W>(i - s2, n))) +
+                          vel<T, SC>(V, vs, ix<W>(i + s - s2, n)));
   T f;
   if (o > 4)
     f = f5<T>(M, Vm, q[ix<W>(i - 2 * s, n)], q[ix<W>(i - s, n)], q[i], q[ix<W>(i + s, n)],
@@ -111,6 +120,7 @@ struct Terms {
   const T* p1;
   T* out;
   T* y;
+  const T* yin;  // nullable: y = yin + (...) instead of y += (...)
 };
 
 template <class T>
@@ -121,7 +131,7 @@ __device__ __forceinline__ void stage_out(const Terms<T>& t, int nc, const doubl
     const T x0 = (nc == 1) ? ds : t.p0[k];
     const T x1 = (nc == 2) ? ds : (nc > 2 ? t.p1[k] : T(0));
     const T x2 = (nc == 3) ? ds : T(0);
-    t.y[k] = t.y[k] + rk_combine<T>(nc, c, x0, x1, x2);
+    t.y[k] = (t.yin ? t.yin[k] : t.y[k]) + rk_combine<T>(nc, c, x0, x1, x2);
   }
 }
 
@@ -129,31 +139,42 @@ template <class T>
 struct MomArgs {
   SwGeo g;
   int fill, integ, stage;
-  const T* Ux;
+  const T* Ux;  // U, or with the FU kernels u (U = u * (sx, sy))
   const T* Uy;
+  double sx, sy;
   const T* w;
   const int8_t* ovx;
   const int8_t* ovy;
   double f;  // addcoriolis coefficient f0 * area * 0.25
   const T* ke;  // nullable (qgrsw: no gradient terms)
-  const T* p;
+  const T* p;   // p, or with the FU kernels h (p = goa * (h + hb))
+  double goa, hb;
   const int8_t* mx;
   const int8_t* my;
   Terms<T> tx, ty;
   const double* dt;
+  double* dt_log;  // receives *dt (one thread), nullable
 };
 
 // vortex force of both components at source point q (row js, column is)
-template <class T, int M, bool W>
+template <class T, int M, bool W, bool FU>
 __device__ __forceinline__ void vforce2(const MomArgs<T>& a, int q, int n, T& vx, T& vy) {
   const int NX = a.g.NX;
   // du.x: vortexforce (V = U.y, s = yshift, s2 = xshift, sign +1)
-  vx = vforce<T, M, W>(a.Uy, a.w, a.ovy[q], q, NX, 1, T(1), n);
+  vx = vforce<T, M, W, FU>(a.Uy, T(a.sy), a.w, a.ovy[q], q, NX, 1, T(1), n);
   // du.y: vortexforce (V = U.x, s = xshift, s2 = yshift, sign -1)
-  vy = vforce<T, M, W>(a.Ux, a.w, a.ovx[q], q, 1, NX, T(-1), n);
+  vy = vforce<T, M, W, FU>(a.Ux, T(a.sx), a.w, a.ovx[q], q, 1, NX, T(-1), n);
 }
 
-template <class T, int M>
+// pressure at x: read, or (FU) compute_pressure's (g/area) * (h + hb)
+template <class T, bool FU>
+__device__ __forceinline__ T pres(const MomArgs<T>& a, int x) {
+  return FU ? T(a.goa) * (a.p[x] + T(a.hb)) : a.p[x];
+}
+
+// FU: U = sharp(u) and p = (g/area)(h + hb) formed from u and h, with the
+// arithmetic of k_sharp and k_pressure, instead of read
+template <class T, int M, bool FU>
 __global__ __launch_bounds__(256) void k_sw_momentum(MomArgs<T> a) {
   const SwGeo& g = a.g;
   int i, j;
@@ -162,23 +183,26 @@ __global__ __launch_bounds__(256) void k_sw_momentum(MomArgs<T> a) {
   const int k = j * NX + i;
   const int js = a.fill ? wsrc_j(g, j) : j, is = a.fill ? wsrc_i(g, i) : i;
   const int q = js * NX + is;
+  if (a.dt_log && k == 0) *a.dt_log = *a.dt;
   T vx, vy;
   if (no_wrap(g, js))
-    vforce2<T, M, false>(a, q, n, vx, vy);
+    vforce2<T, M, false, FU>(a, q, n, vx, vy);
   else
-    vforce2<T, M, true>(a, q, n, vx, vy);
+    vforce2<T, M, true, FU>(a, q, n, vx, vy);
   const T ff = T(a.f);
+  const T sx = T(a.sx), sy = T(a.sy);
   if (js < NY - 1 && is >= 1 && is < NX - 1)
-    vx = vx + ff * (((a.Uy[q - 1] + a.Uy[q + NX - 1]) + a.Uy[q]) + a.Uy[q + NX]);
+    vx = vx + ff * (((vel<T, FU>(a.Uy, sy, q - 1) + vel<T, FU>(a.Uy, sy, q + NX - 1)) +
+                     vel<T, FU>(a.Uy, sy, q)) + vel<T, FU>(a.Uy, sy, q + NX));
   if (js >= 1 && js < NY - 1 && is < NX - 1)
-    vy = vy - ff * (((a.Ux[q - NX] + a.Ux[q - NX + 1]) + a.Ux[q]) + a.Ux[q + 1]);
+    vy = vy - ff * (((vel<T, FU>(a.Ux, sx, q - NX) + vel<T, FU>(a.Ux, sx, q - NX + 1)) +
+                     vel<T, FU>(a.Ux, sx, q)) + vel<T, FU>(a.Ux, sx, q + 1));
   if (a.ke) {
-    const T* phis[2] = {a.ke, a.p};
-    for (int t = 0; t < 2; t++) {
-      const T* phi = phis[t];
-      if (is >= 1) vx = vx - (phi[q] - phi[q - 1]) * T(a.mx[q]);
-      if (js >= 1) vy = vy - (phi[q] - phi[q - NX]) * T(a.my[q]);
-    }
+    // addgrad(ke), then addgrad(p)
+    if (is >= 1) vx = vx - (a.ke[q] - a.ke[q - 1]) * T(a.mx[q]);
+    if (js >= 1) vy = vy - (a.ke[q] - a.ke[q - NX]) * T(a.my[q]);
+    if (is >= 1) vx = vx - (pres<T, FU>(a, q) - pres<T, FU>(a, q - 1)) * T(a.mx[q]);
+    if (js >= 1) vy = vy - (pres<T, FU>(a, q) - pres<T, FU>(a, q - NX)) * T(a.my[q]);
   }
   double c[3] = {0.0, 0.0, 0.0};
   const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, *a.dt, c);
@@ -224,8 +248,10 @@ __global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a) {
   const int k = j * NX + i;
   const T sx = T(a.sx), sy = T(a.sy);
   // sharp (operators.py:59-64), at every point
-  a.Ux[k] = a.ux[k] * sx;
-  a.Uy[k] = a.uy[k] * sy;
+  if (a.Ux) {
+    a.Ux[k] = a.ux[k] * sx;
+    a.Uy[k] = a.uy[k] * sy;
+  }
   // compute_pressure (operators.py:110-111), at every point, unfilled
   if (a.p) a.p[k] = T(a.goa) * (a.h[k] + T(a.hb));
   // vorticity and kinetic energy are filled: the source's value
@@ -371,6 +397,7 @@ static Terms<T> terms(const f2d_sw_stage* st, int f) {
   t.p1 = (const T*)st->prev[1][f];
   t.out = (T*)st->out[f];
   t.y = (T*)st->y[f];
+  t.yin = (const T*)st->y_in[f];
   return t;
 }
 
@@ -394,19 +421,39 @@ static unsigned nblk(const SwGeo& g) {
 }
 static const dim3 TILE_BLOCK(TW, TH);
 
-extern "C" {
+template <class T, int M, bool FU>
+static void launch_momentum(const MomArgs<T>& a, const SwGeo& geo, void* stream) {
+  k_sw_momentum<T, M, FU><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a);
+}
 
-int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, const void* Uy,
-                    const void* omega, const int8_t* ovx, const int8_t* ovy, double f,
-                    const void* ke, const void* p, const int8_t* mskx, const int8_t* msky,
-                    int fill, const f2d_sw_stage* st, void* stream) {
+template <class T, bool FU>
+static void launch_momentum(const MomArgs<T>& a, const SwGeo& geo, int method, void* stream) {
+  switch (method) {
+    case F2D_WENO: launch_momentum<T, F2D_WENO, FU>(a, geo, stream); break;
+    case F2D_UPWIND: launch_momentum<T, F2D_UPWIND, FU>(a, geo, stream); break;
+    case F2D_CENTERED: launch_momentum<T, F2D_CENTERED, FU>(a, geo, stream); break;
+    default: launch_momentum<T, F2D_CWENO, FU>(a, geo, stream); break;
+  }
+}
+
+// the momentum entry points: from_u selects U = sharp(u), p from h
+static int momentum_impl(int dtype, const f2d_grid* g, int method, const void* Ux,
+                         const void* Uy, const void* omega, const int8_t* ovx,
+                         const int8_t* ovy, double f, const void* ke, const void* p,
+                         double goa, double hb, const int8_t* mskx, const int8_t* msky,
+                         int fill, const f2d_sw_stage* st, void* stream, bool from_u) {
   return guarded([&] {
     const SwGeo geo = sw_geo(g);
     F2D_REQUIRE(method >= F2D_WENO && method <= F2D_CWENO, "unknown method");
     F2D_REQUIRE(Ux && Uy && omega && ovx && ovy, "NULL argument");
-    F2D_REQUIRE((ke == nullptr) == (p == nullptr), "ke and p go together");
+    F2D_REQUIRE((ke == nullptr) == (p == nullptr), "ke and p (or h) go together");
     F2D_REQUIRE(!ke || (mskx && msky), "masks are required with the gradient terms");
     const int integ = stage_check(st, 2);
+    F2D_REQUIRE(!st || !st->dt_log || st->dt, "dt_log needs dt");
+    if (from_u && st)  // u is read around each point: it cannot be updated in place
+      for (int f = 0; f < 2; f++)
+        F2D_REQUIRE(st->y[f] != Ux && st->y[f] != Uy,
+                    "the momentum from u must update into another array (y_in = u)");
     F2D_SW_DISPATCH(dtype, {
       MomArgs<T> a{};
       a.g = geo;
@@ -415,27 +462,47 @@ int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, co
       a.stage = st ? st->stage : 0;
       a.Ux = (const T*)Ux;
       a.Uy = (const T*)Uy;
+      a.sx = 1.0 / (g->dx * g->dx);  // sharp's 1/dx**2, 1/dy**2
+      a.sy = 1.0 / (g->dy * g->dy);
       a.w = (const T*)omega;
       a.ovx = ovx;
       a.ovy = ovy;
       a.f = f;
       a.ke = (const T*)ke;
       a.p = (const T*)p;
+      a.goa = goa;
+      a.hb = hb;
       a.mx = mskx;
       a.my = msky;
       a.tx = terms<T>(st, 0);
       a.ty = terms<T>(st, 1);
       a.dt = st ? st->dt : nullptr;
-      switch (method) {
-        case F2D_WENO: k_sw_momentum<T, F2D_WENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-        case F2D_UPWIND: k_sw_momentum<T, F2D_UPWIND><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-        case F2D_CENTERED:
-          k_sw_momentum<T, F2D_CENTERED><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a);
-          break;
-        default: k_sw_momentum<T, F2D_CWENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-      }
+      a.dt_log = st ? st->dt_log : nullptr;
+      if (from_u)
+        launch_momentum<T, true>(a, geo, method, stream);
+      else
+        launch_momentum<T, false>(a, geo, method, stream);
     });
   });
+}
+
+extern "C" {
+
+int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, const void* Uy,
+                    const void* omega, const int8_t* ovx, const int8_t* ovy, double f,
+                    const void* ke, const void* p, const int8_t* mskx, const int8_t* msky,
+                    int fill, const f2d_sw_stage* st, void* stream) {
+  return momentum_impl(dtype, g, method, Ux, Uy, omega, ovx, ovy, f, ke, p, 0.0, 0.0, mskx,
+                       msky, fill, st, stream, false);
+}
+
+int f2d_sw_momentum_u(int dtype, const f2d_grid* g, int method, const void* ux,
+                      const void* uy, const void* omega, const int8_t* ovx, const int8_t* ovy,
+                      double f, const void* ke, const void* h, double g_over_area, double hb,
+                      const int8_t* mskx, const int8_t* msky, int fill,
+                      const f2d_sw_stage* st, void* stream) {
+  return momentum_impl(dtype, g, method, ux, uy, omega, ovx, ovy, f, ke, h, g_over_area, hb,
+                       mskx, msky, fill, st, stream, true);
 }
 
 int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const void* uy,
@@ -444,7 +511,8 @@ int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const 
                 void* omega, void* ke, void* p, void* stream) {
   return guarded([&] {
     const SwGeo geo = sw_geo(g);
-    F2D_REQUIRE(ux && uy && slipcoef && Ux && Uy && omega, "NULL argument");
+    F2D_REQUIRE(ux && uy && slipcoef && omega, "NULL argument");
+    F2D_REQUIRE((Ux == nullptr) == (Uy == nullptr), "Ux and Uy go together");
     F2D_REQUIRE(!ke || (method >= F2D_WENO && method <= F2D_CWENO && okx && oky && msk),
                 "kinetic energy needs a flux method (not classic), orders and msk");
     F2D_REQUIRE(!p || h, "pressure needs h");

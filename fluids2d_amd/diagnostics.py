@@ -52,6 +52,8 @@ class Bulk:
     def measure(self, out):
         """The four averages of the current device state into out[0:4]."""
         m, s = self.model.mesh, self.model.integrator.dstate
+        # the fields a fused stage does not store between steps
+        self.model.integrator.model.materialize(s, names=("U", "ke", "omega"))
         ke = getattr(s, "ke", None)
         _lib.call("f2d_bulk_diagnostics", _lib.dtype_code(s.omega.dtype), m.grid_ref(),
                   _lib.ptr(ke), _lib.ptr(s.omega), _lib.ptr(s.U.x), _lib.ptr(s.U.y),

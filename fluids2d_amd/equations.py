@@ -21,9 +21,10 @@ class _Base:
         self.param = param
         self.mesh = mesh
 
-    def materialize(self, s):
+    def materialize(self, s, names=None):
         """Fill the diagnostic fields the fused path does not store, before a
-        device -> host copy."""
+        device -> host copy or a device read (names: only these are needed;
+        None: all)."""
 
     def cfl_velocity(self, s):
         return s.U
@@ -73,8 +74,9 @@ class EulerPsi(_Base):
                         cfl=p.cfl, dtmax=p.dtmax)
         return s, alt
 
-    def materialize(self, s):
-        op.centerstovertices(self.mesh, s.omega, s.vomega)
+    def materialize(self, s, names=None):
+        if names is None or "vomega" in names:
+            op.centerstovertices(self.mesh, s.omega, s.vomega)
 
 
 class Advection(_Base):
@@ -129,7 +131,7 @@ class QG(_Base):
                 d = f.to(device=like.device, dtype=like.dtype)
             else:
                 a = np.broadcast_to(np.asarray(f, dtype=np.float64), tuple(like.shape))
-                d = torch.from_numpy(np.ascontiguousarray(a)).to(like.device).to(like.dtype)
+                d = torch.from_numpy(np.array(a)).to(like.device).to(like.dtype)  # a writable copy
             self._f_src, self._f_dev = f, d.contiguous()
         return self._f_dev
 
@@ -177,8 +179,9 @@ class QG(_Base):
                         cfl=p.cfl, dtmax=p.dtmax)
         return s, alt
 
-    def materialize(self, s):
-        self._inversion_rhs(s)
+    def materialize(self, s, names=None):
+        if names is None or "work" in names:
+            self._inversion_rhs(s)
 
 
 MODELS = {"eulerpsi": EulerPsi, "advection": Advection, "qg": QG}

@@ -62,6 +62,7 @@ class RKIntegrator:
         self.model = get_device_model(param, mesh)
         self.d_dt = torch.zeros(1, dtype=torch.float64, device=dev)
         self.d_dtn = torch.zeros(1, dtype=torch.float64, device=dev)
+        self._dt_written = None  # (value, buffer) of the last fixed dt written
         self._h_dt = torch.zeros(1, dtype=torch.float64).pin_memory()
         self._dtn_valid = False
         self.__doc__ = (self.__doc__ or "") + f"\n{rkdocs[param.integrator]}"
@@ -137,7 +138,7 @@ class RKIntegrator:
         the host state (copy in, advance, copy out).  As in the reference,
         dt is the time's current one; Model.set_dt recomputes it."""
         self.begin(state)
-        op.set_scalar(self.d_dt, time.dt)
+        self._write_dt(time.dt)
         self.device_step(time)
         self.end(state)
 
@@ -149,7 +150,7 @@ class RKIntegrator:
 
     def end(self, state, names=None):
         """Device twin -> host state."""
-        self.model.materialize(self.dstate)
+        self.model.materialize(self.dstate, names)
         download(self.dstate, state, names)
 
     def set_dt(self, time):
@@ -157,14 +158,15 @@ class RKIntegrator:
         p = self.param
         if p.dt > 0:
             time.dt = p.dt
-            op.set_scalar(self.d_dt, p.dt)
+            self._write_dt(p.dt)
             return
         if p.model == "rsw":
             c = (p.g * p.H) ** 0.5
             maxU = c / self.mesh.dx + c / self.mesh.dy
             time.dt = min(p.cfl / maxU, p.dtmax)
-            op.set_scalar(self.d_dt, time.dt)
+            self._write_dt(time.dt)
             return
+        self._dt_written = None
         if not self._dtn_valid:
             op.cfl_dt(self.model.cfl_velocity(self.dstate), p.cfl, p.dtmax, self.d_dtn)
             self._dtn_valid = True
@@ -172,6 +174,16 @@ class RKIntegrator:
         self._h_dt.copy_(self.d_dt, non_blocking=True)
         torch.cuda.current_stream().synchronize()
         time.dt = float(self._h_dt[0])
+
+    def _write_dt(self, value):
+        """A fixed dt into d_dt.  Skipped when d_dt already holds it (it was
+        the last value written to this buffer and the CFL path, which swaps
+        and overwrites the dt buffers, has not run since): a step graph of a
+        fixed-dt run then carries no set kernel."""
+        key = (float(value), self.d_dt.data_ptr())
+        if self._dt_written != key:
+            op.set_scalar(self.d_dt, value)
+            self._dt_written = key
 
     def _wants_dtn(self):
         p = self.param
@@ -217,12 +229,12 @@ class RKIntegrator:
         model.py:71-87)."""
         p = self.param
         if p.dt > 0:
-            op.set_scalar(self.d_dt, p.dt)
+            self._write_dt(p.dt)
         elif p.model == "rsw":
             c = (p.g * p.H) ** 0.5
-            op.set_scalar(self.d_dt, min(p.cfl / (c / self.mesh.dx + c / self.mesh.dy),
-                                         p.dtmax))
+            self._write_dt(min(p.cfl / (c / self.mesh.dx + c / self.mesh.dy), p.dtmax))
         else:
+            self._dt_written = None
             if not self._dtn_valid:
                 op.cfl_dt(self.model.cfl_velocity(self.dstate), p.cfl, p.dtmax, self.d_dtn)
             self.d_dt, self.d_dtn = self.d_dtn, self.d_dt

@@ -101,51 +101,55 @@ def profile_dominant(model, reps=20):
                     for k, v in rows.items()}}
 
 
-def sw_timing(model, reps=20):
+def sw_timing(model, reps=4):
     """Mean launch time and algorithmic bytes of the fused rsw stage kernels
-    at stage 1 of RK3, each replayed back to back from a graph: the momentum
-    kernel (reads U.x, U.y, omega, ke, p, the stage-0 tendency and u, two
-    orders and two masks; writes u and the stage-1 tendency), the transport
-    stage of h (reads h, U.x, U.y, the stage-0 tendency, two orders and msk;
-    writes h and the tendency) and the diag kernel (reads u.x, u.y, h, the
-    float64 slip coefficient, two orders and msk; writes U.x, U.y, omega, ke,
-    p).  The updates go to scratch copies: the model's state is not
-    advanced (the diag rewrites the diagnostics it already holds)."""
+    within steps: `reps` times, the step graph is replayed three times (12
+    steps) and two eager steps are enqueued behind it with HIP events around
+    each of their kernels, so that the device never waits for the host and
+    an interval is the kernel's duration in the step (the caches hold what
+    the step's earlier kernels left).  Bytes per launch are averaged over
+    the three RK3 stages: the momentum kernel reads u.x, u.y, omega, ke, h
+    and the earlier tendencies (0, 1, 2 pairs), two orders and two masks,
+    and writes the new u and the stage's tendency (stages 0 and 1); the
+    transport of h reads h, u.x, u.y and the earlier tendencies, two orders
+    and msk, and writes h and the tendency (stages 0, 1); the diag reads
+    u.x, u.y, the float64 slip coefficient, two orders and msk, and writes
+    omega and ke.  The model advances by the steps taken."""
     integ = model.integrator
     sw = integ.model
     s = integ.dstate
-    m, p = model.mesh, model.param
-    d0, d1 = integ.dscratch[0], integ.dscratch[1]
-    ux, uy, h = s.u.x.clone(), s.u.y.clone(), torch.empty_like(s.h)
-    st = sw._stage_terms([d0], d1, 1, integ.d_dt, (ux, uy, None))
     t = s.h.element_size()
     npts = s.h.numel()
-    launches = {
-        "k_sw_momentum (vortex force + Coriolis + gradients + fill + RK update)":
-            (lambda: sw._momentum(s, st, gradients=True, fill=True), npts * (13 * t + 4)),
-        "k_transport (fused WENO5 flux + div + RK update of h)":
-            (lambda: op.transport_stage(m, p.compflux, s.h, h, s.U, [d0.h], d1.h,
-                                        p.integrator, 1, integ.d_dt), npts * (6 * t + 3)),
-        "k_sw_diag (sharp + vorticity + kinetic energy + pressure + fills)":
-            (lambda: sw._diag_fused(s, energy=True), npts * (8 * t + 11)),
+    per_stage = {  # arrays of the element type read + written, int8 / float64 bytes
+        "k_sw_momentum": ([5 + 4, 5 + 2 + 4, 5 + 4 + 2], 4),
+        "k_transport": ([3 + 2, 4 + 2, 5 + 1], 3),
+        "k_sw_diag": ([4, 4, 4], 11),
     }
+    names = {"k_sw_momentum": "k_sw_momentum (vortex force + Coriolis + gradients + fill + "
+                              "RK update)",
+             "k_transport": "k_transport (fused WENO5 flux + div + RK update of h)",
+             "k_sw_diag": "k_sw_diag (vorticity + kinetic energy + fills)"}
+    integ.prepare_graph()
+    sw.events = []
+    try:
+        for _ in range(reps):
+            g = integ._graph()
+            for _ in range(3):
+                g.replay()
+            for _ in range(2):
+                integ._select_dt()
+                integ.advance(stats=False)
+        torch.cuda.synchronize()
+        ev = sw.events
+    finally:
+        sw.events = None
     rows = {}
-    for name, (fn, byts) in launches.items():
-        fn()
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with _lib.capture(g):
-            for _ in range(reps):
-                fn()
-        g.replay()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        g.replay()
-        e1.record()
-        torch.cuda.synchronize()
-        us = 1e3 * e0.elapsed_time(e1) / reps
-        rows[name] = dict(us=us, bytes=byts, per_step=3, gbs=byts / (us * 1e-6) / 1e9)
+    for key, (arrays, extra) in per_stage.items():
+        durs = [e0.elapsed_time(e1) * 1e3 for (n, k, e0, e1) in ev if n == key]
+        byts = npts * (t * sum(arrays) / len(arrays) + extra)
+        us = sum(durs) / len(durs)
+        rows[names[key]] = dict(us=us, bytes=byts, per_step=3, gbs=byts / (us * 1e-6) / 1e9,
+                                launches=len(durs))
     name = max(rows, key=lambda k: rows[k]["us"] * rows[k]["per_step"])
     r = rows[name]
     return {"kernel": name, "us": r["us"], "bytes": r["bytes"], "achieved_gbs": r["gbs"],

@@ -33,6 +33,12 @@ class _SW(_Base):
     prognostic = ("u", "h")
     fused = True  # stage() through the fused kernels (False: operator by operator)
 
+    @property
+    def logs_dt(self):
+        """stage(..., dt_log=) records the step's dt (the fused momentum
+        kernel copies it)."""
+        return self.fused
+
     def __init__(self, param, mesh):
         super().__init__(param, mesh)
         self._ds_last = None
@@ -72,12 +78,13 @@ class _SW(_Base):
         _ops("f2d_vorticity", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y),
              _lib.ptr(m.d_slipcoef), _lib.ptr(s.omega))
 
-    def _stage_terms(self, ds_prev, ds_out, k, dt, y):
+    def _stage_terms(self, ds_prev, ds_out, k, dt, y, dt_log=None, y_in=(None,) * 3):
         """The f2d_sw_stage of RK stage k: earlier tendencies, this stage's
-        tendency (None on the last stage: not kept), fields updated."""
+        tendency (None on the last stage: not kept), fields updated (from
+        y_in where given)."""
         prev = [(x.u.x, x.u.y, x.h) for x in ds_prev]
         out = (ds_out.u.x, ds_out.u.y, ds_out.h) if ds_out is not None else (None,) * 3
-        return _lib.sw_stage(self.param.integrator, k, dt, prev, out, y)
+        return _lib.sw_stage(self.param.integrator, k, dt, prev, out, y, dt_log, y_in)
 
     def _momentum(self, s, st, gradients, fill):
         m, p = self.mesh, self.param
@@ -88,22 +95,30 @@ class _SW(_Base):
              ptr(s.p) if gradients else None, ptr(m.d_mskx), ptr(m.d_msky), int(fill),
              ctypes.byref(st))
 
-    def _diag_fused(self, s, energy):
+    def _diag_fused(self, s, energy, store_U=True, pressure=None):
+        """sharp (unless store_U is False), vorticity, kinetic energy (with
+        energy) and pressure (default: with energy), and the fills."""
         m, p = self.mesh, self.param
         ptr = _lib.ptr
+        pressure = energy if pressure is None else pressure
         method = _lib.METHODS.get(p.innerproduct, 0)
         _ops("f2d_sw_diag", self._dt(s.h), self._grid(), method, ptr(s.u.x), ptr(s.u.y),
              ptr(s.h), ptr(m.d_slipcoef), ptr(m.d_ok.x), ptr(m.d_ok.y), ptr(m.d_msk),
-             float(p.g / m.area), 0.0, ptr(s.U.x), ptr(s.U.y), ptr(s.omega),
-             ptr(s.ke) if energy else None, ptr(s.p) if energy else None)
+             float(p.g / m.area), 0.0, ptr(s.U.x) if store_U else None,
+             ptr(s.U.y) if store_U else None, ptr(s.omega),
+             ptr(s.ke) if energy else None, ptr(s.p) if pressure else None)
 
-    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True):
+    def cfl_velocity(self, s):
+        return s.U
+
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None):
         """One stage of the reference's RK (integrators.py:92-124): rhs,
-        addto on u.x, u.y and h, diag."""
+        addto on u.x, u.y and h, diag.  dt_log: the momentum kernel copies
+        the step's dt there."""
         if self.fused:
-            s, alt = self._stage_fused(s, alt, ds_prev, ds_out, k, dt, stats)
+            s, alt = self._stage_fused(s, alt, ds_prev, ds_out, k, dt, stats, dt_log)
             if dt_next is not None:
-                op.cfl_dt(s.U, self.param.cfl, self.param.dtmax, dt_next)
+                op.cfl_dt(self.cfl_velocity(s), self.param.cfl, self.param.dtmax, dt_next)
             return s, alt
         p = self.param
         if ds_out is None:
@@ -130,17 +145,72 @@ class _SW(_Base):
 
 
 class RSW(_SW):
-    """Rotating shallow water (equations.py:139-157)."""
+    """Rotating shallow water (equations.py:139-157).
 
-    def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats):
+    The fused stage stores neither U nor p: the momentum kernel and the
+    transport of h form U = sharp(u) and p = (g/area) h from u and h (the
+    same arithmetic, so the same bits), u and h go to the other buffer of a
+    ping-pong pair, and the diag keeps omega and ke.  ``materialize`` (before
+    any read of the device state) and ``cfl_velocity`` store U and p."""
+
+    def _momentum_u(self, s, st):
+        """The momentum kernel from u and h (U and p formed on the fly)."""
         m, p = self.mesh, self.param
-        st = self._stage_terms(ds_prev, ds_out, k, dt, (s.u.x, s.u.y, None))
-        self._momentum(s, st, gradients=True, fill=True)
-        op.transport_stage(m, p.compflux, s.h, alt.h, s.U, [x.h for x in ds_prev],
-                           ds_out.h if ds_out is not None else None, p.integrator, k, dt)
-        s, alt = s._replace(h=alt.h), alt._replace(h=s.h)
-        self._diag_fused(s, energy=True)
+        ptr = _lib.ptr
+        _ops("f2d_sw_momentum_u", self._dt(s.h), self._grid(), _lib.METHODS[p.vortexforce],
+             ptr(s.u.x), ptr(s.u.y), ptr(s.omega), ptr(m.d_ov.x), ptr(m.d_ov.y),
+             float(p.f0 * m.area * 0.25), ptr(s.ke), ptr(s.h), float(p.g / m.area), 0.0,
+             ptr(m.d_mskx), ptr(m.d_msky), 1, ctypes.byref(st))
+
+    def _transport_u(self, s, h_out, dh_prev, dh_out, k, dt):
+        """The transport stage of h with U = sharp(u) formed on the fly."""
+        m, p = self.mesh, self.param
+        ptr = _lib.ptr
+        d0 = dh_prev[0] if len(dh_prev) > 0 else None
+        d1 = dh_prev[1] if len(dh_prev) > 1 else None
+        _ops("f2d_transport_stage_u", self._dt(s.h), self._grid(), _lib.METHODS[p.compflux],
+             ptr(s.h), ptr(h_out), ptr(s.u.x), ptr(s.u.y), ptr(m.d_oc.x), ptr(m.d_oc.y),
+             ptr(m.d_msk), ptr(d0), ptr(d1), ptr(dh_out), _lib.INTEGRATORS[p.integrator], k,
+             ptr(dt))
+
+    def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats, dt_log=None):
+        st = self._stage_terms(ds_prev, ds_out, k, dt, (alt.u.x, alt.u.y, None), dt_log,
+                               y_in=(s.u.x, s.u.y, None))
+        self._timed("k_sw_momentum", k, lambda: self._momentum_u(s, st))
+        self._timed("k_transport", k, lambda: self._transport_u(
+            s, alt.h, [x.h for x in ds_prev], ds_out.h if ds_out is not None else None, k, dt))
+        s, alt = s._replace(u=alt.u, h=alt.h), alt._replace(u=s.u, h=s.h)
+        self._timed("k_sw_diag", k, lambda: self._diag_fused(s, energy=True, store_U=False,
+                                                             pressure=False))
         return s, alt
+
+    events = None  # a list: eager stages record (kernel, stage, start, end) HIP events
+
+    def _timed(self, name, k, fn):
+        if self.events is None:
+            return fn()
+        import torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        self.events.append((name, k, e0, e1))
+
+    def _store_U_p(self, s):
+        m, p = self.mesh, self.param
+        dt, g = self._dt(s.h), self._grid()
+        _ops("f2d_sharp", dt, g, _lib.ptr(s.u.x), _lib.ptr(s.u.y), _lib.ptr(s.U.x),
+             _lib.ptr(s.U.y))
+        _ops("f2d_pressure", dt, g, _lib.ptr(s.h), _lib.ptr(s.p), float(p.g / m.area), 0.0)
+
+    def materialize(self, s, names=None):
+        if self.fused and (names is None or "U" in names or "p" in names):
+            self._store_U_p(s)
+
+    def cfl_velocity(self, s):
+        if self.fused:
+            self._store_U_p(s)
+        return s.U
 
     def rhs(self, s, ds):
         m = self.mesh
@@ -173,7 +243,7 @@ class QGRSW(_SW):
     """RSW tendency projected onto the QG manifold (equations.py:106-136,
     Thiry et al. 2024)."""
 
-    def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats):
+    def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats, dt_log=None):
         m, p = self.mesh, self.param
         ptr = _lib.ptr
         if self._pre is None:
@@ -182,8 +252,8 @@ class QGRSW(_SW):
             self._pre = allocate_device(self.prognostic, m.shape, s.h.dtype, s.h.device)
         pre = self._pre
         dt_, g = self._dt(s.h), self._grid()
-        self._momentum(s, _lib.sw_stage(None, 0, None, out=(pre.u.x, pre.u.y, None)),
-                       gradients=False, fill=False)
+        self._momentum(s, _lib.sw_stage(None, 0, dt, out=(pre.u.x, pre.u.y, None),
+                                        dt_log=dt_log), gradients=False, fill=False)
         self._divflux(s, pre.h)
         _ops("f2d_qg_pv", dt_, g, ptr(pre.u.x), ptr(pre.u.y), ptr(pre.h), ptr(m.d_slipcoef),
              ptr(m.d_mskv), float(-p.f0 / p.H), ptr(s.pv))
@@ -229,6 +299,7 @@ class Euler(_SW):
     host check and steps are graph-captured; the Jacobi-PCG solve always
     checks convergence on the host, so its steps are not."""
     prognostic = ("u",)
+    logs_dt = False
 
     @property
     def capturable(self):

@@ -145,6 +145,16 @@ int f2d_transport_stage_psi(int dtype, const f2d_grid* g, int method, const void
                             const void* ds_prev0, const void* ds_prev1, void* ds_out,
                             int integrator, int stage, const double* dt, void* stream);
 
+/* The same stage with the contravariant velocity formed from the covariant
+ * one: U = sharp(u), operators.py:59-64 (Ux = ux / dx**2, Uy = uy / dy**2
+ * as k_sharp stores them), bit for bit the U that f2d_sharp would store.
+ * The shallow-water stage uses it so that U need not be stored. */
+int f2d_transport_stage_u(int dtype, const f2d_grid* g, int method, const void* q_in,
+                          void* q_out, const void* ux, const void* uy, const int8_t* ocx,
+                          const int8_t* ocy, const int8_t* msk, const void* ds_prev0,
+                          const void* ds_prev1, void* ds_out, int integrator, int stage,
+                          const double* dt, void* stream);
+
 /* addto, integrators.py:154-196 (the sum of :174): over n entries
  * y += ((c0*x0 + c1*x1) + c2*x2), with the coefficients of `integrator`
  * stage `stage` evaluated from *dt as above. */
@@ -270,6 +280,10 @@ typedef struct f2d_sw_stage {
   void* out[3];       /* this stage's tendency (NULL: not stored)          */
   void* y[3];         /* fields updated in place, y += the RK combination
                          of addto, integrators.py:154-196 (NULL: none)     */
+  const void* y_in[3]; /* non-NULL: y = y_in + the combination instead
+                         (y_in left unchanged: a ping-pong pair)           */
+  double* dt_log;     /* f2d_sw_momentum copies *dt here (NULL: no copy;
+                         dt must be given, also with integrator -1)        */
 } f2d_sw_stage;
 
 /* The momentum tendency: vortexforce of both components (weno.py:423-431,
@@ -282,11 +296,22 @@ int f2d_sw_momentum(int dtype, const f2d_grid* g, int method, const void* Ux, co
                     const void* omega, const int8_t* ovx, const int8_t* ovy, double f,
                     const void* ke, const void* p, const int8_t* mskx, const int8_t* msky,
                     int fill, const f2d_sw_stage* st, void* stream);
+/* f2d_sw_momentum with U = sharp(u) (operators.py:59-64) and p =
+ * compute_pressure(h) (operators.py:110-111, p = (g/area)(h + hb)) formed
+ * from u and h on the fly, with the arithmetic of f2d_sharp and
+ * f2d_pressure: bit for bit the same tendency, without U and p stored.
+ * u is read around each point, so the update of fields 0 and 1 goes to
+ * other arrays (st->y != u, st->y_in = u). */
+int f2d_sw_momentum_u(int dtype, const f2d_grid* g, int method, const void* ux,
+                      const void* uy, const void* omega, const int8_t* ovx, const int8_t* ovy,
+                      double f, const void* ke, const void* h, double g_over_area, double hb,
+                      const int8_t* mskx, const int8_t* msky, int fill,
+                      const f2d_sw_stage* st, void* stream);
 /* The diag of rsw (equations.py:151-157) and qgrsw (equations.py:130-136):
  * sharp (operators.py:59-64), compute_vorticity (operators.py:67-77),
  * compute_kinetic_energy with `method` (operators.py:80-101) when ke is
  * given, compute_pressure (operators.py:110-111) when p is given, and the
- * fill of omega and ke. */
+ * fill of omega and ke.  Ux and Uy may be NULL (sharp not stored). */
 int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const void* uy,
                 const void* h, const double* slipcoef, const int8_t* okx, const int8_t* oky,
                 const int8_t* msk, double g_over_area, double hb, void* Ux, void* Uy,
