@@ -254,7 +254,13 @@ class QGRSW(_SW):
         dt_, g = self._dt(s.h), self._grid()
         self._momentum(s, _lib.sw_stage(None, 0, dt, out=(pre.u.x, pre.u.y, None),
                                         dt_log=dt_log), gradients=False, fill=False)
-        self._divflux(s, pre.h)
+        if p.xperiodic or getattr(p, "yperiodic", False):
+            self._divflux(s, pre.h)  # the unfilled divergence the projection reads
+        else:
+            # closed domain: the fill does nothing and the halo cells are dry
+            # (their divergence is 0, as div's product with msk leaves it),
+            # so the transport stage's divergence is div's, entry for entry
+            op.transport_rhs(m, p.compflux, s.h, s.U, pre.h)
         _ops("f2d_qg_pv", dt_, g, ptr(pre.u.x), ptr(pre.u.y), ptr(pre.h), ptr(m.d_slipcoef),
              ptr(m.d_mskv), float(-p.f0 / p.H), ptr(s.pv))
         m.qg_helmholtz.solve(s.pv, s.psi, stats=stats)
