@@ -293,7 +293,8 @@ def ensemble_sizes(count, seed):
     return [int(v) for v in rng.choice(ENSEMBLE_SIZES, size=count)]
 
 
-def ensemble_bench(count=64, steps=200, seed=99, precision="fp32", tune=True, log=None):
+def ensemble_bench(count=64, steps=200, seed=99, precision="fp32", tune=True, log=None,
+                   batch=True):
     """The batched-ensemble workload: `count` qgrsw members with sizes drawn
     by seed from ENSEMBLE_SIZES (squared), closed smoothed-noise basins,
     FP32 + refinement, `steps` steps replayed from one captured graph.
@@ -322,7 +323,7 @@ def ensemble_bench(count=64, steps=200, seed=99, precision="fp32", tune=True, lo
             log(f"ensemble: {k + 1}/{count} members set up "
                 f"({time.perf_counter() - t0:.1f} s)")
     setup = time.perf_counter() - t0
-    ens = Ensemble(members, nstreams=8)
+    ens = Ensemble(members, nstreams=8, batch=batch)
     ens.steps(2)
     if log:
         log("ensemble: 2 eager steps done")

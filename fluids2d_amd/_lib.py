@@ -30,6 +30,9 @@ EXPORTS = [
     "f2d_adj_stage", "f2d_adj_combine", "f2d_bulk_diagnostics",
     "f2d_perpgrad_cfl_noalloc", "f2d_sw_momentum", "f2d_sw_momentum_u", "f2d_sw_diag",
     "f2d_qg_pv", "f2d_qg_project", "f2d_transport_stage_u",
+    "f2d_poisson_batch_create", "f2d_poisson_batch_solve", "f2d_poisson_batch_destroy",
+    "f2d_record_begin", "f2d_record_end", "f2d_recording_size", "f2d_recording_destroy",
+    "f2d_launch_batch_create", "f2d_launch_batch_run", "f2d_launch_batch_destroy",
 ]
 
 
@@ -96,7 +99,7 @@ def sw_stage(integrator, stage, dt, prev=(), out=(None, None, None), y=(None, No
     return st
 
 
-ABI_VERSION = 6  # f2d_abi_version() of the library these bindings describe
+ABI_VERSION = 7  # f2d_abi_version() of the library these bindings describe
 
 
 _lib = None
@@ -180,6 +183,17 @@ def lib():
         "f2d_poisson_read_stats": (i32, [vp, ctypes.POINTER(d), vp]),
         "f2d_poisson_bench_kernel": (i32, [vp, i32, i32, ctypes.POINTER(d),
                                            ctypes.POINTER(d), vp]),
+        "f2d_poisson_batch_create": (i32, [ctypes.POINTER(vp), i32, ctypes.POINTER(vp),
+                                           ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "f2d_poisson_batch_solve": (i32, [vp, vp]),
+        "f2d_poisson_batch_destroy": (i32, [vp]),
+        "f2d_record_begin": (i32, []),
+        "f2d_record_end": (i32, [ctypes.POINTER(vp)]),
+        "f2d_recording_size": (i32, [vp]),
+        "f2d_recording_destroy": (i32, [vp]),
+        "f2d_launch_batch_create": (i32, [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]),
+        "f2d_launch_batch_run": (i32, [vp, vp]),
+        "f2d_launch_batch_destroy": (i32, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)

@@ -14,7 +14,7 @@ PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libfluids2d_amd.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["f2d_mesh.hip", "f2d_stencil.hip", "f2d_poisson.hip", "f2d_fft.hip", "f2d_ops.hip", "f2d_adjoint.hip", "f2d_neumann.hip",
-           "f2d_capacitance.hip", "f2d_sw.hip"]
+           "f2d_capacitance.hip", "f2d_sw.hip", "f2d_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -70,12 +70,10 @@ __device__ __forceinline__ double xw(const CapGeo& g, const TX* x,
 // FP64 residual of the masked operator, r = b - A x at the unknowns (x read
 // at unknowns only; xin NULL: x = 0), stored as T (0 elsewhere)
 template <class TB, class TX, class T>
-__global__ __launch_bounds__(256) void k_cap_res(CapGeo g, const TB* __restrict__ b,
-                                                 int gmode, const int8_t* __restrict__ mw,
-                                                 const TX* __restrict__ xin,
-                                                 T* __restrict__ r) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= (int64_t)g.NYP * g.NXP) return;
+__device__ __forceinline__ void cap_res_at(const CapGeo& g, const TB* __restrict__ b,
+                                           int gmode, const int8_t* __restrict__ mw,
+                                           const TX* __restrict__ xin, T* __restrict__ r,
+                                           int64_t k) {
   double v = 0.0;
   if (mw[k]) {
     v = (double)bval(b, k, g.NXP, gmode);
@@ -90,17 +88,25 @@ __global__ __launch_bounds__(256) void k_cap_res(CapGeo g, const TB* __restrict_
   r[k] = T(v);
 }
 
+template <class TB, class TX, class T>
+__global__ __launch_bounds__(256) void k_cap_res(CapGeo g, const TB* __restrict__ b,
+                                                 int gmode, const int8_t* __restrict__ mw,
+                                                 const TX* __restrict__ xin,
+                                                 T* __restrict__ r) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (int64_t)g.NYP * g.NXP) return;
+  cap_res_at<TB, TX, T>(g, b, gmode, mw, xin, r, k);
+}
+
 // s = M y_B for the rows of this workgroup, written to rhs at the points B
 // (rhs is zero there).  y_B is gathered into LDS; each wavefront takes one
 // row at a time, two (FP64) or four (FP32) entries per lane per load, and
 // sums in FP64 in a fixed order.  Row pitch ld (a multiple of 4, padding 0).
 template <class T>
-__global__ __launch_bounds__(256) void k_cap_gemv(const T* __restrict__ M, int n, int ld,
-                                                  const int* __restrict__ bidx,
-                                                  const T* __restrict__ y,
-                                                  T* __restrict__ rhs, int rows) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* gy = reinterpret_cast<T*>(smem);
+__device__ __forceinline__ void cap_gemv_block(const T* __restrict__ M, int n, int ld,
+                                               const int* __restrict__ bidx,
+                                               const T* __restrict__ y,
+                                               T* __restrict__ rhs, int rows, T* gy) {
   for (int j = threadIdx.x; j < ld; j += 256) gy[j] = j < n ? y[bidx[j]] : T(0);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -124,6 +130,15 @@ __global__ __launch_bounds__(256) void k_cap_gemv(const T* __restrict__ M, int n
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if (lane == 0) rhs[bidx[r]] = T(acc);
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cap_gemv(const T* __restrict__ M, int n, int ld,
+                                                  const int* __restrict__ bidx,
+                                                  const T* __restrict__ y,
+                                                  T* __restrict__ rhs, int rows) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cap_gemv_block<T>(M, n, ld, bidx, y, rhs, rows, reinterpret_cast<T*>(smem));
 }
 
 // x = y at the unknowns
@@ -209,6 +224,57 @@ __global__ __launch_bounds__(256) void k_cap_stats(const double* __restrict__ pa
   st[4] += conv > 0.0 ? 0.0 : 1.0;
   if ((double)(passes - 1) > st[5]) st[5] = (double)(passes - 1);
   st[6] += 1.0;
+}
+
+// one member of a batched FP32 solve (device table, blockIdx.y = member)
+struct CapMember {
+  const float* b;      // right-hand side at the vertices (padded)
+  float* x;            // solution: warm start in, result out
+  const int8_t* mw;    // unknowns
+  float* rhs;          // scratch: right-hand side of the spectral solves
+  float* y;            // scratch: spectral solution
+  double* x64;         // FP64 accumulator
+  const float* M;      // (-C)^-1, n rows of pitch ld
+  const int* bidx;     // padded index of each point of B
+  int n, ld, rows, pad;
+};
+
+// from: 0 x = 0 (cold first pass), 1 the caller's x, 2 the accumulator
+__global__ __launch_bounds__(256) void k_cap_res_b(CapGeo g, const CapMember* __restrict__ t,
+                                                   int from) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (int64_t)g.NYP * g.NXP) return;
+  const CapMember& m = t[blockIdx.y];
+  if (from == 1)
+    cap_res_at<float, float, float>(g, m.b, 0, m.mw, m.x, m.rhs, k);
+  else
+    cap_res_at<float, double, float>(g, m.b, 0, m.mw, from ? m.x64 : nullptr, m.rhs, k);
+}
+
+// grid x: the largest member's workgroup count (the others return early)
+__global__ __launch_bounds__(256) void k_cap_gemv_b(const CapMember* __restrict__ t) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const CapMember& m = t[blockIdx.y];
+  if ((int64_t)blockIdx.x * m.rows >= m.n) return;
+  cap_gemv_block<float>(m.M, m.n, m.ld, m.bidx, m.y, m.rhs, m.rows,
+                        reinterpret_cast<float*>(smem));
+}
+
+// x64 = x0 + y at the unknowns (from as in k_cap_res_b)
+__global__ __launch_bounds__(256) void k_cap_acc_b(int64_t n, const CapMember* __restrict__ t,
+                                                   int from) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const CapMember& m = t[blockIdx.y];
+  if (k >= n || !m.mw[k]) return;
+  const double x0 = from == 1 ? (double)m.x[k] : (from == 2 ? m.x64[k] : 0.0);
+  m.x64[k] = x0 + (double)m.y[k];
+}
+
+__global__ __launch_bounds__(256) void k_cap_round_b(int64_t n,
+                                                     const CapMember* __restrict__ t) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const CapMember& m = t[blockIdx.y];
+  if (k < n && m.mw[k]) m.x[k] = (float)m.x64[k];
 }
 
 // C[i, j] = (A_R^-1)(B_i, B_j) from the Green's function G of the periodic
@@ -539,6 +605,103 @@ void cap_solve_f32(CapSolver* c, const float* b, int gmode, float* x, int nref, 
   k_cap_round<<<blocks(N), 256, 0, s>>>(N, c->mw, c->x64, x);
   if (measure && stats) measure_res<float, double>(c, b, gmode, c->x64, passes, stats, s);
   c->solved = true;
+}
+
+// ---------------------------------------------------------------------------
+// batched FP32 solves of same-size members
+// ---------------------------------------------------------------------------
+struct CapBatch {
+  std::vector<CapSolver*> c;
+  std::vector<const float*> b;
+  std::vector<float*> x;
+  CapMember* t = nullptr;      // device member table
+  FftBatchPtrs* ft = nullptr;  // device: rhs -> y through each member's spectrum
+  int maxnwg = 0;
+  size_t lds = 0;
+};
+
+static bool same_geo(const CapGeo& a, const CapGeo& b) {
+  return a.ny == b.ny && a.nx == b.nx && a.xper == b.xper && a.yper == b.yper &&
+         a.cx == b.cx && a.cy == b.cy && a.d == b.d;
+}
+
+CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
+                           float* const* x) {
+  F2D_REQUIRE(nm >= 1 && nm <= 65535, "batch of 1 .. 65535 members");
+  CapBatch* bt = new CapBatch();
+  try {
+    std::vector<CapMember> t(nm);
+    std::vector<FftBatchPtrs> ft(nm);
+    for (int i = 0; i < nm; i++) {
+      CapSolver* m = c[i];
+      F2D_REQUIRE(m && !m->f64, "batched capacitance solves are FP32 solvers");
+      F2D_REQUIRE(b[i] && x[i], "batch member without b or x");
+      F2D_REQUIRE(same_geo(m->g, c[0]->g) && fft_same_plan(m->fft, c[0]->fft),
+                  "batch members must share the rectangle operator");
+      bt->c.push_back(m);
+      bt->b.push_back(b[i]);
+      bt->x.push_back(x[i]);
+      t[i] = CapMember{b[i], x[i], m->mw, (float*)m->rhs, (float*)m->y, m->x64,
+                       (const float*)m->M, m->bidx, m->n, m->ld, m->rows, 0};
+      ft[i] = FftBatchPtrs{m->rhs, fft_spectrum32(m->fft), m->y};
+      const int nwg = (m->n + m->rows - 1) / m->rows;
+      if (nwg > bt->maxnwg) bt->maxnwg = nwg;
+      if (sizeof(float) * (size_t)m->ld > bt->lds) bt->lds = sizeof(float) * (size_t)m->ld;
+    }
+    F2D_HIP(hipMalloc(&bt->t, sizeof(CapMember) * nm));
+    F2D_HIP(hipMalloc(&bt->ft, sizeof(FftBatchPtrs) * nm));
+    F2D_HIP(hipMemcpy(bt->t, t.data(), sizeof(CapMember) * nm, hipMemcpyHostToDevice));
+    F2D_HIP(hipMemcpy(bt->ft, ft.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
+    F2D_REQUIRE(bt->lds <= 160 * 1024, "capacitance boundary too long for the LDS gather");
+    if (bt->lds > 64 * 1024)
+      F2D_HIP(hipFuncSetAttribute((const void*)k_cap_gemv_b,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt->lds));
+  } catch (...) {
+    cap_batch_destroy(bt);
+    throw;
+  }
+  return bt;
+}
+
+void cap_batch_destroy(CapBatch* bt) {
+  if (!bt) return;
+  (void)hipDeviceSynchronize();
+  if (bt->t) (void)hipFree(bt->t);
+  if (bt->ft) (void)hipFree(bt->ft);
+  delete bt;
+}
+
+bool cap_batch_warm(const CapBatch* bt) {
+  for (const CapSolver* m : bt->c)
+    if (!m->solved) return false;
+  return true;
+}
+
+void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s) {
+  const int nm = (int)bt->c.size();
+  const CapGeo& g = bt->c[0]->g;
+  const int64_t N = (int64_t)g.NYP * g.NXP;
+  const bool from_x = warm && cap_batch_warm(bt);
+  bool none = true;
+  for (const CapSolver* m : bt->c) none = none && !m->solved;
+  if (warm && !from_x && !none) {
+    // some members warm, some cold: each member's own solve
+    for (int i = 0; i < nm; i++)
+      cap_solve_f32(bt->c[i], bt->b[i], 0, bt->x[i], nref, true, false, nullptr, s);
+    return;
+  }
+  const int passes = from_x ? (nref > 0 ? nref : 1) : (nref > 0 ? nref : 1) + 1;
+  const dim3 gp((unsigned)blocks(N), (unsigned)nm);
+  for (int p = 0; p < passes; p++) {
+    const int from = p > 0 ? 2 : (from_x ? 1 : 0);
+    k_cap_res_b<<<gp, 256, 0, s>>>(g, bt->t, from);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
+    k_cap_gemv_b<<<dim3((unsigned)bt->maxnwg, (unsigned)nm), 256, bt->lds, s>>>(bt->t);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
+    k_cap_acc_b<<<gp, 256, 0, s>>>(N, bt->t, from);
+  }
+  k_cap_round_b<<<gp, 256, 0, s>>>(N, bt->t);
+  for (CapSolver* m : bt->c) m->solved = true;
 }
 
 }  // namespace f2d

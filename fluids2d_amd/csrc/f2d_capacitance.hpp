@@ -50,4 +50,19 @@ void cap_solve_f64(CapSolver* c, const double* b, int gmode, double* x, bool mea
 void cap_solve_f32(CapSolver* c, const float* b, int gmode, float* x, int nref, bool warm,
                    bool measure, double* stats, hipStream_t s);
 
+// Batched FP32 solves of same-size members (one launch per kernel of the
+// pass for all of them; blockIdx.y = member).  Every member must be an FP32
+// capacitance solver of the same rectangle operator (grid, spacings,
+// maindiag, periodicity); masks and boundaries may differ.  b[i] and x[i]
+// are member i's fixed right-hand side and solution (gmode 0).  A solve
+// runs member i's cap_solve_f32 sequence entry for entry, so the results
+// are those of the member's own solve, bit for bit.
+struct CapBatch;
+CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
+                           float* const* x);
+void cap_batch_destroy(CapBatch* bt);
+// whether every member has solved before (a warm batch needs all of them)
+bool cap_batch_warm(const CapBatch* bt);
+void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s);
+
 }  // namespace f2d

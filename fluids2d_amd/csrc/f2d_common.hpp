@@ -8,7 +8,9 @@
 #include <stdint.h>
 
 #include <cstdio>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/fluids2d_amd.h"
 
@@ -35,10 +37,49 @@ struct Error {
     if (!(cond)) throw ::f2d::Error{std::string(msg)};          \
   } while (0)
 
+// ---------------------------------------------------------------------------
+// member batching: kernel launches recorded instead of run
+// ---------------------------------------------------------------------------
+// Between f2d_record_begin and f2d_record_end (f2d_batch.hip) the entry
+// points that support it append their kernel launches -- kernel, grid,
+// block, dynamic LDS and the by-value argument struct -- to this thread's
+// recording instead of launching them; any other entry point refuses to
+// run.  The recordings of several members, kernel for kernel alike, become
+// one launch per kernel for all of them (blockIdx.y = member), each
+// workgroup taking its member's argument struct from a device table.
+struct LaunchRec {
+  const void* fn;  // the kernel's batched instantiation
+  dim3 grid, block;
+  unsigned shmem;
+  std::vector<unsigned char> args;
+};
+struct Recording {
+  std::vector<LaunchRec> launches;
+};
+Recording*& recording();  // this thread's recording (NULL: launches run)
+bool& recordable();       // set while an entry point that can be recorded runs
+
+// Launch kernel `plain`(a, NULL), or record `batched` with a.  Both are
+// instantiations of one kernel template whose batched form replaces `a` by
+// table[blockIdx.y] (its second argument).
+template <class A>
+inline void launch_rec(void (*plain)(A, const A*), void (*batched)(A, const A*), dim3 grid,
+                       dim3 block, unsigned shmem, hipStream_t s, const A& a) {
+  if (Recording* r = recording()) {
+    LaunchRec l{(const void*)batched, grid, block, shmem, std::vector<unsigned char>(sizeof(A))};
+    std::memcpy(l.args.data(), &a, sizeof(A));
+    r->launches.push_back(std::move(l));
+    return;
+  }
+  hipLaunchKernelGGL(plain, grid, block, shmem, s, a, (const A*)nullptr);
+}
+
 // Wrap an entry point body: exceptions become a status code + message.
 template <class F>
 int guarded(F&& f) {
   try {
+    if (recording() && !recordable())
+      throw Error{"this entry point cannot be recorded into a member batch"};
     f();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -53,6 +94,15 @@ int guarded(F&& f) {
     set_error(e.what());
     return -3;
   }
+}
+
+// guarded() for an entry point whose launches all go through launch_rec
+template <class F>
+int guarded_rec(F&& f) {
+  recordable() = true;
+  const int rc = guarded(f);
+  recordable() = false;
+  return rc;
 }
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }

@@ -304,9 +304,15 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
                                                       const double* __restrict__ x64,
                                                       const double* __restrict__ bpin,
                                                       int nbp,
-                                                      const float* __restrict__ x32) {
+                                                      const float* __restrict__ x32,
+                                                      const FftBatchPtrs* __restrict__ bt =
+                                                          nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double sh[256];
+  if (bt) {  // a batch: this member's b and spectrum
+    b = reinterpret_cast<const TB*>(bt[blockIdx.y].b);
+    S = reinterpret_cast<cplx<T>*>(bt[blockIdx.y].S);
+  }
   const int nx = a.nx, P = a.P, lg = a.lgx;
   cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* b1 = b0 + P * nx;
@@ -418,8 +424,11 @@ __global__ __launch_bounds__(1024) void k_fft_cols(FftArgs a,
                                                   cplx<T>* __restrict__ S,
                                                   const cplx<T>* __restrict__ tw,
                                                   const double* __restrict__ lamx,
-                                                  const double* __restrict__ lamy) {
+                                                  const double* __restrict__ lamy,
+                                                  const FftBatchPtrs* __restrict__ bt =
+                                                      nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (bt) S = reinterpret_cast<cplx<T>*>(bt[blockIdx.y].S);
   const int ny = a.ny, lg = a.lgy;
   const int k0 = blockIdx.x * a.Q;
   const int nq = min(a.Q, a.nk - k0);
@@ -474,8 +483,14 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
                                                       double* __restrict__ x64,
                                                       const double* __restrict__ rs_src =
                                                           nullptr,
-                                                      double* __restrict__ rs_dst = nullptr) {
+                                                      double* __restrict__ rs_dst = nullptr,
+                                                      const FftBatchPtrs* __restrict__ bt =
+                                                          nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (bt) {  // a batch: this member's spectrum and solution
+    S = reinterpret_cast<const cplx<T>*>(bt[blockIdx.y].S);
+    x = reinterpret_cast<TX*>(bt[blockIdx.y].x);
+  }
   // the row sums of this call's b become the next call's (one per workgroup;
   // the forward row pass has the same grid)
   if (rs_dst && threadIdx.x == 0) rs_dst[blockIdx.x] = rs_src[blockIdx.x];
@@ -977,6 +992,35 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
     k_fft_stats<<<1, 256, 0, s>>>(f->part, f->nbres, (double)f->a.onx * f->a.ony,
                                   f->a.singular, rtol, nref, measure ? 1 : 0, stats);
 }
+
+// One FP32 spectral solve (rows forward, columns, rows inverse; x = the
+// result) for each of `nm` same-size members, `f` the plan of one of them
+// (its args, twiddles and eigenvalues), `table` (device) their arrays.
+void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s) {
+  F2D_REQUIRE(f->S32, "FFT solver built without FP32 buffers");
+  FftArgs a = f->a;
+  a.P = f->P32;
+  a.Q = f->Q32;
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  const dim3 grr(gr, nm), grc(gc, nm);
+  k_fft_rows_fwd<float, float><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+      a, nullptr, 0, nullptr, f->twx32, nullptr, nullptr, nullptr, gr, nullptr, table);
+  k_fft_cols<float><<<grc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(a, nullptr, f->twy32,
+                                                                     f->lamx, f->lamy, table);
+  k_fft_rows_inv<float, float, 0><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+      a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, table);
+}
+
+// whether two plans solve the same operator (same args and eigenvalues)
+bool fft_same_plan(const FftSolver* f, const FftSolver* g) {
+  const FftArgs &a = f->a, &b = g->a;
+  return a.nx == b.nx && a.ny == b.ny && a.ex == b.ex && a.ey == b.ey && a.onx == b.onx &&
+         a.ony == b.ony && a.cx == b.cx && a.cy == b.cy && a.d == b.d &&
+         a.singular == b.singular && f->P32 == g->P32 && f->Q32 == g->Q32 &&
+         f->S32 != nullptr && g->S32 != nullptr;
+}
+
+float2* fft_spectrum32(FftSolver* f) { return f->S32; }
 
 }  // namespace f2d
 

@@ -56,6 +56,18 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
                           int nref, bool warm, double rtol, bool measure,
                           double* stats, hipStream_t s);
 
+// Batched FP32 spectral solves of same-size members (blockIdx.y = member):
+// each member's right-hand side b (FP32, padded, gmode 0), spectrum S and
+// solution x; `f` is one member's plan (fft_same_plan holds for all).
+struct FftBatchPtrs {
+  const void* b;
+  void* S;
+  void* x;
+};
+void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s);
+bool fft_same_plan(const FftSolver* f, const FftSolver* g);
+float2* fft_spectrum32(FftSolver* f);
+
 // timing hook (see f2d_poisson_bench_kernel)
 double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
                  double* bytes);

@@ -2502,6 +2502,68 @@ int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream) {
   });
 }
 
+}  // extern "C"
+
+struct f2d_poisson_batch {
+  std::vector<f2d_poisson*> h;
+  std::vector<void*> x;
+  f2d::CapBatch* cap = nullptr;
+  int nref = 1;
+  bool warm = true;
+};
+
+extern "C" {
+
+int f2d_poisson_batch_create(f2d_poisson* const* h, int32_t n, const void* const* b,
+                             void* const* x, f2d_poisson_batch** out) {
+  return guarded([&] {
+    F2D_REQUIRE(h && b && x && out, "NULL argument");
+    F2D_REQUIRE(n >= 1, "empty batch");
+    auto* bt = new f2d_poisson_batch();
+    try {
+      std::vector<f2d::CapSolver*> c(n);
+      for (int i = 0; i < n; i++) {
+        F2D_REQUIRE(h[i] && h[i]->cap && h[i]->opts.mode != F2D_POISSON_F64,
+                    "batched solves take FP32 capacitance solvers");
+        F2D_REQUIRE(h[i]->opts.fft_refine == h[0]->opts.fft_refine &&
+                        (h[i]->opts.fft_warm != 0) == (h[0]->opts.fft_warm != 0),
+                    "batch members must share the refinement options");
+        bt->h.push_back(h[i]);
+        bt->x.push_back(x[i]);
+        c[i] = h[i]->cap;
+      }
+      bt->nref = h[0]->opts.fft_refine > 0 ? h[0]->opts.fft_refine : 1;
+      bt->warm = h[0]->opts.fft_warm != 0;
+      bt->cap = f2d::cap_batch_create(c.data(), n, reinterpret_cast<const float* const*>(b),
+                                      reinterpret_cast<float* const*>(x));
+    } catch (...) {
+      delete bt;
+      throw;
+    }
+    *out = bt;
+  });
+}
+
+int f2d_poisson_batch_solve(f2d_poisson_batch* bt, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(bt, "batch is NULL");
+    hipStream_t s = S(stream);
+    f2d::cap_batch_solve(bt->cap, bt->nref, bt->warm, s);
+    for (size_t i = 0; i < bt->h.size(); i++) {
+      f2d::fill_x<float>(bt->h[i], (float*)bt->x[i], s);
+      bt->h[i]->fft_unmeasured++;
+    }
+  });
+}
+
+int f2d_poisson_batch_destroy(f2d_poisson_batch* bt) {
+  return guarded([&] {
+    if (!bt) return;
+    f2d::cap_batch_destroy(bt->cap);
+    delete bt;
+  });
+}
+
 const double* f2d_poisson_stats(f2d_poisson* h) {
   if (!h) return nullptr;
   if (h->neu) return neu_stats(h->neu);

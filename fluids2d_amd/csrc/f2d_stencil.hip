@@ -256,8 +256,12 @@ __device__ __forceinline__ int tile_of(int b, int nt) {
   return (nt & 7) ? b : (b & 7) * (nt >> 3) + (b >> 3);
 }
 
-template <class T, int M, int TY, int TX>
-__global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
+// B: the batched instantiation (a = tab[blockIdx.y]: one member of a
+// recorded member batch, launch_rec)
+template <class T, int M, int TY, int TX, bool B = false>
+__global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a,
+                                                   const TransportArgs<T>* tab) {
+  if (B) a = tab[blockIdx.y];
   constexpr int LW = TX + 6;
   __shared__ T sq[(TY + 6) * LW];
   __shared__ T sfx[TY * (TX + 1)];
@@ -396,8 +400,10 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a) {
 // tile's last x-face column and y-face row may lie on the wrapped periodic
 // edge (ux_at / uy_at there).  Per face and per cell the arithmetic is
 // that of k_transport.
-template <class T, int M, int TY, int TX>
-__global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a) {
+template <class T, int M, int TY, int TX, bool B = false>
+__global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a,
+                                                        const TransportArgs<T>* tab) {
+  if (B) a = tab[blockIdx.y];
   constexpr int LW = TX + 6;
   constexpr int RP = 256 / TX;             // rows per pass of the workgroup
   constexpr int NQR = TY + 6;              // q rows of the tile
@@ -546,9 +552,11 @@ void launch_transport_tile(const TransportArgs<T>& a0, hipStream_t s) {
   const bool whole = a.ny % TY == 0 && a.nx % TX == 0 &&
                      (int64_t)(a.ny + 6) * (int64_t)a.P < ((int64_t)1 << 31);
   if (whole)
-    k_transport_full<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
+    launch_rec(k_transport_full<T, M, TY, TX, false>, k_transport_full<T, M, TY, TX, true>,
+               dim3(nb), dim3(256), 0, s, a);
   else
-    k_transport<T, M, TY, TX><<<nb, 256, 0, s>>>(a);
+    launch_rec(k_transport<T, M, TY, TX, false>, k_transport<T, M, TY, TX, true>, dim3(nb),
+               dim3(256), 0, s, a);
 }
 
 // tile shape by code (f2d_grid.tile): the arithmetic per cell is the same
@@ -985,7 +993,7 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
                                 const void* ds_prev0, const void* ds_prev1, void* ds_out,
                                 int integrator, int stage, const double* dt, void* stream,
                                 double* dt_log = nullptr, bool from_u = false) {
-  return guarded([&] {
+  return guarded_rec([&] {
     check_grid(g);
     check_method(method);
     F2D_REQUIRE(!dt_log || dt, "dt_log needs dt");

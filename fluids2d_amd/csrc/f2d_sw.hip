@@ -174,8 +174,11 @@ __device__ __forceinline__ T pres(const MomArgs<T>& a, int x) {
 
 // FU: U = sharp(u) and p = (g/area)(h + hb) formed from u and h, with the
 // arithmetic of k_sharp and k_pressure, instead of read
-template <class T, int M, bool FU>
-__global__ __launch_bounds__(256) void k_sw_momentum(MomArgs<T> a) {
+// B: the batched instantiation (a = tab[blockIdx.y]: one member of a
+// recorded member batch, launch_rec)
+template <class T, int M, bool FU, bool B = false>
+__global__ __launch_bounds__(256) void k_sw_momentum(MomArgs<T> a, const MomArgs<T>* tab) {
+  if (B) a = tab[blockIdx.y];
   const SwGeo& g = a.g;
   int i, j;
   if (!tile_point(g, i, j)) return;
@@ -239,8 +242,9 @@ __device__ __forceinline__ T ke_sum(const DiagArgs<T>& a, T sx, T sy, int q, int
   return v;
 }
 
-template <class T, int M>
-__global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a) {
+template <class T, int M, bool B = false>
+__global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a, const DiagArgs<T>* tab) {
+  if (B) a = tab[blockIdx.y];
   const SwGeo& g = a.g;
   int i, j;
   if (!tile_point(g, i, j)) return;
@@ -281,12 +285,28 @@ __global__ __launch_bounds__(256) void k_sw_diag(DiagArgs<T> a) {
 // vorticity(du) * slip (row 0 keeping its old pv), then pv += c2v(dh *
 // scale) on [1:, 1:], pv *= mskv.  Every point from its own index.
 template <class T>
-__global__ __launch_bounds__(256) void k_qg_pv(SwGeo g, const T* __restrict__ dux,
-                                               const T* __restrict__ duy,
-                                               const T* __restrict__ dh,
-                                               const double* __restrict__ slip,
-                                               const int8_t* __restrict__ mv, double scale,
-                                               T* __restrict__ pv) {
+struct PvArgs {
+  SwGeo g;
+  const T* dux;
+  const T* duy;
+  const T* dh;
+  const double* slip;
+  const int8_t* mv;
+  double scale;
+  T* pv;
+};
+
+template <class T, bool B = false>
+__global__ __launch_bounds__(256) void k_qg_pv(PvArgs<T> a, const PvArgs<T>* tab) {
+  if (B) a = tab[blockIdx.y];
+  const SwGeo& g = a.g;
+  const T* __restrict__ dux = a.dux;
+  const T* __restrict__ duy = a.duy;
+  const T* __restrict__ dh = a.dh;
+  const double* __restrict__ slip = a.slip;
+  const int8_t* __restrict__ mv = a.mv;
+  const double scale = a.scale;
+  T* __restrict__ pv = a.pv;
   int i, j;
   if (!tile_point(g, i, j)) return;
   const int NX = g.NX;
@@ -326,8 +346,9 @@ struct ProjArgs {
 // fill(du, dh), then the RK addto of u and h.  Every point takes its
 // source's values (the fill); the rows and columns the slices skip keep the
 // pre-projection tendency.
-template <class T>
-__global__ __launch_bounds__(256) void k_qg_project(ProjArgs<T> a) {
+template <class T, bool B = false>
+__global__ __launch_bounds__(256) void k_qg_project(ProjArgs<T> a, const ProjArgs<T>* tab) {
+  if (B) a = tab[blockIdx.y];
   const SwGeo& g = a.g;
   int i, j;
   if (!tile_point(g, i, j)) return;
@@ -423,7 +444,8 @@ static const dim3 TILE_BLOCK(TW, TH);
 
 template <class T, int M, bool FU>
 static void launch_momentum(const MomArgs<T>& a, const SwGeo& geo, void* stream) {
-  k_sw_momentum<T, M, FU><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a);
+  launch_rec(k_sw_momentum<T, M, FU, false>, k_sw_momentum<T, M, FU, true>, nblk(geo),
+             TILE_BLOCK, 0, S(stream), a);
 }
 
 template <class T, bool FU>
@@ -436,13 +458,29 @@ static void launch_momentum(const MomArgs<T>& a, const SwGeo& geo, int method, v
   }
 }
 
+template <class T, int M>
+static void launch_diag(const DiagArgs<T>& a, const SwGeo& geo, void* stream) {
+  launch_rec(k_sw_diag<T, M, false>, k_sw_diag<T, M, true>, nblk(geo), TILE_BLOCK, 0,
+             S(stream), a);
+}
+
+template <class T>
+static void launch_diag(const DiagArgs<T>& a, int method, const SwGeo& geo, void* stream) {
+  switch (method) {
+    case F2D_WENO: launch_diag<T, F2D_WENO>(a, geo, stream); break;
+    case F2D_UPWIND: launch_diag<T, F2D_UPWIND>(a, geo, stream); break;
+    case F2D_CENTERED: launch_diag<T, F2D_CENTERED>(a, geo, stream); break;
+    default: launch_diag<T, F2D_CWENO>(a, geo, stream); break;
+  }
+}
+
 // the momentum entry points: from_u selects U = sharp(u), p from h
 static int momentum_impl(int dtype, const f2d_grid* g, int method, const void* Ux,
                          const void* Uy, const void* omega, const int8_t* ovx,
                          const int8_t* ovy, double f, const void* ke, const void* p,
                          double goa, double hb, const int8_t* mskx, const int8_t* msky,
                          int fill, const f2d_sw_stage* st, void* stream, bool from_u) {
-  return guarded([&] {
+  return guarded_rec([&] {
     const SwGeo geo = sw_geo(g);
     F2D_REQUIRE(method >= F2D_WENO && method <= F2D_CWENO, "unknown method");
     F2D_REQUIRE(Ux && Uy && omega && ovx && ovy, "NULL argument");
@@ -509,7 +547,7 @@ int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const 
                 const void* h, const double* slipcoef, const int8_t* okx, const int8_t* oky,
                 const int8_t* msk, double g_over_area, double hb, void* Ux, void* Uy,
                 void* omega, void* ke, void* p, void* stream) {
-  return guarded([&] {
+  return guarded_rec([&] {
     const SwGeo geo = sw_geo(g);
     F2D_REQUIRE(ux && uy && slipcoef && omega, "NULL argument");
     F2D_REQUIRE((Ux == nullptr) == (Uy == nullptr), "Ux and Uy go together");
@@ -535,12 +573,7 @@ int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const 
       a.w = (T*)omega;
       a.ke = (T*)ke;
       a.p = (T*)p;
-      switch (ke ? method : F2D_WENO) {
-        case F2D_WENO: k_sw_diag<T, F2D_WENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-        case F2D_UPWIND: k_sw_diag<T, F2D_UPWIND><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-        case F2D_CENTERED: k_sw_diag<T, F2D_CENTERED><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-        default: k_sw_diag<T, F2D_CWENO><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a); break;
-      }
+      launch_diag<T>(a, ke ? method : F2D_WENO, geo, stream);
     });
   });
 }
@@ -548,12 +581,14 @@ int f2d_sw_diag(int dtype, const f2d_grid* g, int method, const void* ux, const 
 int f2d_qg_pv(int dtype, const f2d_grid* g, const void* dux, const void* duy, const void* dh,
               const double* slipcoef, const int8_t* mskv, double scale, void* pv,
               void* stream) {
-  return guarded([&] {
+  return guarded_rec([&] {
     const SwGeo geo = sw_geo(g);
     F2D_REQUIRE(dux && duy && dh && slipcoef && mskv && pv, "NULL argument");
-    F2D_SW_DISPATCH(dtype, k_qg_pv<T><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(
-                               geo, (const T*)dux, (const T*)duy, (const T*)dh, slipcoef,
-                               mskv, scale, (T*)pv));
+    F2D_SW_DISPATCH(dtype, {
+      const PvArgs<T> a{geo, (const T*)dux, (const T*)duy, (const T*)dh, slipcoef, mskv,
+                        scale, (T*)pv};
+      launch_rec(k_qg_pv<T, false>, k_qg_pv<T, true>, nblk(geo), TILE_BLOCK, 0, S(stream), a);
+    });
   });
 }
 
@@ -561,7 +596,7 @@ int f2d_qg_project(int dtype, const f2d_grid* g, const void* psi, const int8_t* 
                    const int8_t* msky, const int8_t* mskv, const int8_t* msk, double scale,
                    const void* pre_ux, const void* pre_uy, const void* pre_h,
                    const f2d_sw_stage* st, void* stream) {
-  return guarded([&] {
+  return guarded_rec([&] {
     const SwGeo geo = sw_geo(g);
     F2D_REQUIRE(psi && mskx && msky && mskv && msk && pre_ux && pre_uy && pre_h,
                 "NULL argument");
@@ -590,7 +625,8 @@ int f2d_qg_project(int dtype, const f2d_grid* g, const void* psi, const int8_t* 
       a.ty = terms<T>(st, 1);
       a.th = terms<T>(st, 2);
       a.dt = st->dt;
-      k_qg_project<T><<<nblk(geo), TILE_BLOCK, 0, S(stream)>>>(a);
+      launch_rec(k_qg_project<T, false>, k_qg_project<T, true>, nblk(geo), TILE_BLOCK, 0,
+                 S(stream), a);
     });
   });
 }

@@ -19,21 +19,38 @@ DEFAULT_RTOL = {"fp64": 1e-13, "fp32": 1e-9}
 DEFAULT_MAXIT = {"fp64": 100, "fp32": 8}
 
 
-def _destroy(h):
+def _destroy(h, name="f2d_poisson_destroy"):
     try:
-        _lib.lib().f2d_poisson_destroy(h)
+        getattr(_lib.lib(), name)(h)
     except Exception:
         pass
 
 
-# handles released during a graph capture, destroyed at the next safe point
+# (handle, destructor) pairs released during a graph capture, destroyed at
+# the next safe point in release order
 _PENDING = []
 
 
 def _flush_pending():
     """Destroy handles whose owners went away during a graph capture."""
     while _PENDING and not _lib.capturing():
-        _destroy(_PENDING.pop())
+        _destroy(*_PENDING.pop(0))
+
+
+def _release(h, name):
+    """Destroy a library handle now, or after the capture in progress
+    (freeing device memory would invalidate it)."""
+    if h is None or not h.value:
+        return
+    try:
+        if _lib.capturing():
+            _PENDING.append((h, name))
+            return
+        _destroy(h, name)
+    except TypeError:
+        # at interpreter shutdown the module's globals may already be gone;
+        # the process releases the device memory itself
+        pass
 
 
 class PoissonNotConverged(RuntimeError):
@@ -118,19 +135,8 @@ class Poisson2D:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is None or not h.value:
-            return
         self._h = None
-        try:
-            if _lib.capturing():
-                # freeing device memory would invalidate the capture: later
-                _PENDING.append(h)
-                return
-            _destroy(h)
-        except TypeError:
-            # at interpreter shutdown the module's globals may already be
-            # gone; the process releases the device memory itself
-            pass
+        _release(h, "f2d_poisson_destroy")
 
 
     def handle(self):
@@ -248,6 +254,56 @@ class Poisson2D:
         _lib.call("f2d_poisson_bench_kernel", self._h, kind, reps, ctypes.byref(us),
                   ctypes.byref(by), _lib.stream_ptr())
         return us.value, by.value
+
+
+class PoissonBatch:
+    """The solves of many same-size members with one launch per kernel
+    (f2d_poisson_batch_*): solvers[i].solve(bs[i], xs[i], stats=False) for
+    every i, bit for bit.  Each solver must use the capacitance method in an
+    FP32 precision and the solvers must share the rectangle operator (grid,
+    spacings, maindiag, periodicity); their masks may differ.  bs and xs
+    are fixed device arrays: the batch is bound to them.  The ensembles of
+    fluids2d_amd.ensemble use it (the reference solves one model at a time,
+    elliptic.py:80-87)."""
+
+    def __init__(self, solvers, bs, xs):
+        self.solvers = list(solvers)
+        self.arrays = (list(bs), list(xs))  # keeps the bound arrays alive
+        n = len(self.solvers)
+        arr = ctypes.c_void_p * n
+        h = ctypes.c_void_p()
+        _flush_pending()
+        _lib.call("f2d_poisson_batch_create", arr(*(s.handle() for s in self.solvers)), n,
+                  arr(*(_lib.ptr(b) for b in bs)), arr(*(_lib.ptr(x) for x in xs)),
+                  ctypes.byref(h))
+        self._h = h
+
+    @staticmethod
+    def eligible(solver):
+        return solver.method == "capacitance" and solver.precision != "fp64"
+
+    @staticmethod
+    def key(solver):
+        """Solvers with equal keys may share a batch."""
+        m = solver.mesh
+        p = m.param
+        return (m.shape, m.dx, m.dy, solver.maindiag, bool(p.xperiodic),
+                bool(getattr(p, "yperiodic", False)), solver.precision,
+                tuple(sorted(solver.tuning.items())), bool(getattr(p, "poisson_warm", True)))
+
+    def bound_to(self, bs, xs):
+        return all(a.data_ptr() == b.data_ptr() for a, b in zip(self.arrays[0], bs)) and \
+            all(a.data_ptr() == b.data_ptr() for a, b in zip(self.arrays[1], xs))
+
+    def solve(self):
+        for s in self.solvers:
+            s.calls += 1
+        _lib.call("f2d_poisson_batch_solve", self._h, _lib.stream_ptr())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        self._h = None
+        _release(h, "f2d_poisson_batch_destroy")
 
 
 def get_msk_from_mesh(mesh, location):

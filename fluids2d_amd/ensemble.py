@@ -16,16 +16,67 @@ Results do not depend on S, on the packing or on the graph: each member's
 kernels and their order are those of its own ``device_steps``.  Reductions
 of the CFL step use per-stream scratch (``f2d_reserve_stream_scratch``),
 so concurrent members share no buffer.
+
+Member batching (``batch=True``, the default): members whose stages pause
+at a Helmholtz solve of the capacitance method in FP32 (qgrsw) are grouped
+by operator (same grid, spacings, f0/g/H, periodicity, methods) and each
+group is stepped stage by stage as one: per stage, one launch per kernel
+for the whole group before the solve (``LaunchBatch``: the members' entry
+point calls recorded and merged, blockIdx.y = member), the group's solve
+as one batch (``elliptic.PoissonBatch``), one launch per kernel after it;
+then each member's CFL reduction.  Groups run side by side on the streams,
+members outside any group take whole steps on their lanes.  Every member's
+kernels keep their arithmetic and order, so the results are still those of
+its solo run, bit for bit.
 """
+import ctypes
 import time
 
 import torch
 
 from . import _lib
+from .elliptic import PoissonBatch, _flush_pending, _release
+
+
+class LaunchBatch:
+    """The kernel launches of ``parts`` (one callable per member, each making
+    the same entry point calls on its own arrays), recorded and merged into
+    one launch per kernel for all members (f2d_record_begin / end,
+    f2d_launch_batch_create / run)."""
+
+    def __init__(self, parts):
+        recs = []
+        self._h = None
+        try:
+            for fn in parts:
+                _lib.call("f2d_record_begin")
+                h = ctypes.c_void_p()
+                try:
+                    fn()
+                finally:
+                    _lib.call("f2d_record_end", ctypes.byref(h))
+                    recs.append(h)
+            self.launches = _lib.lib().f2d_recording_size(recs[0])
+            arr = ctypes.c_void_p * len(recs)
+            h = ctypes.c_void_p()
+            _flush_pending()
+            _lib.call("f2d_launch_batch_create", arr(*recs), len(recs), ctypes.byref(h))
+            self._h = h
+        finally:
+            for r in recs:
+                _lib.lib().f2d_recording_destroy(r)
+
+    def run(self):
+        _lib.call("f2d_launch_batch_run", self._h, _lib.stream_ptr())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        self._h = None
+        _release(h, "f2d_launch_batch_destroy")
 
 
 class Ensemble:
-    def __init__(self, models, nstreams=8, graph_steps=2, hipgraph=True):
+    def __init__(self, models, nstreams=8, graph_steps=2, hipgraph=True, batch=True):
         if graph_steps % 2:
             raise ValueError("graph_steps must be even (ping-pong buffers)")
         self.models = list(models)
@@ -33,8 +84,14 @@ class Ensemble:
             raise ValueError("empty ensemble")
         self.G = int(graph_steps)
         self.hipgraph = hipgraph
+        self.batch = batch
+        self._groups = None
+        self._groups_key = None
+        self._inb = set()
+        self._lb = {}
         dev = self.models[0].mesh.device
         self.dtlog = torch.zeros((len(self.models), self.G), dtype=torch.float64, device=dev)
+        self._slot = torch.zeros(len(self.models), dtype=torch.float64, device=dev)
         self._eager = 0
         self._set_streams(nstreams)
 
@@ -58,11 +115,123 @@ class Ensemble:
         self.graph = None
         self._key = None
 
+    # -- member batching ---------------------------------------------------------------
+    def _batchable(self, i):
+        m = self.models[i]
+        integ = m.integrator
+        model = integ.model
+        return (self.batch and m.param.integrator in _lib.NSTAGES and hasattr(integ, "staged")
+                and not integ.forcings and hasattr(model, "stage_split")
+                and getattr(model, "fused", False)
+                and PoissonBatch.eligible(m.mesh.qg_helmholtz))
+
+    def _group_key(self, i):
+        m = self.models[i]
+        p = m.param
+        return (type(m.integrator.model).__name__, PoissonBatch.key(m.mesh.qg_helmholtz),
+                p.integrator, p.compflux, p.vortexforce, p.innerproduct, p.f0, p.g, p.H,
+                m.mesh.grid.tile)
+
+    def groups(self):
+        """The member groups: a list of (member indices, PoissonBatch), one
+        per operator, the solves bound to the members' current pv and psi
+        (rebuilt, with the groups' launch batches, when a member's solver
+        or arrays change)."""
+        idx = [i for i in range(len(self.models)) if self._batchable(i)]
+        key = tuple((i, id(self.models[i].mesh.qg_helmholtz), self._group_key(i),
+                     self.models[i].integrator.dstate.pv.data_ptr(),
+                     self.models[i].integrator.dstate.psi.data_ptr()) for i in idx)
+        if self._groups is None or self._groups_key != key:
+            by = {}
+            for i in idx:
+                by.setdefault(self._group_key(i), []).append(i)
+            groups = []
+            for members in by.values():
+                solvers = [self.models[i].mesh.qg_helmholtz for i in members]
+                st = [self.models[i].integrator.dstate for i in members]
+                groups.append((members, PoissonBatch(solvers, [x.pv for x in st],
+                                                     [x.psi for x in st])))
+            self._groups, self._groups_key = groups, key
+            self._inb = {i for members, _ in groups for i in members}
+            self._lb = {}
+        return self._groups
+
+    def _batched_step(self, col):
+        """One step of every member, the groups stage by stage with one
+        launch per kernel; member i's dt into col[i]."""
+        groups = self.groups()
+        bodies = [[] for _ in self.streams]
+        for g in range(len(groups)):
+            bodies[g % len(bodies)].append(lambda g=g: self._group_step(g))
+        for j, lane in enumerate(self.lanes):
+            for i in lane:
+                if i not in self._inb:
+                    bodies[j].append(lambda i=i: self._member_step(i, self._slot[i:i + 1]))
+        main = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(main)
+        for body, s in zip(bodies, self.streams):
+            with torch.cuda.stream(s):
+                for b in body:
+                    b()
+        for s in self.streams:
+            main.wait_stream(s)
+        col.copy_(self._slot)
+
+    def _group_step(self, g):
+        members, solve = self._groups[g]
+        plans = []
+        for i in members:
+            integ = self.models[i].integrator
+            integ._select_dt()
+            plans.append(integ.staged(dt_log=self._slot[i:i + 1]))
+        nst = len(plans[0])
+        dts = tuple(self.models[i].integrator.d_dt.data_ptr() for i in members)
+
+        def part(plan, k):
+            if k > 0:
+                plan[k - 1][1]()  # after stage k-1's solve
+            if k < nst:
+                plan[k][0]()      # before stage k's solve
+
+        for k in range(nst + 1):
+            self._launch((g, k, dts), [lambda p=p, k=k: part(p, k) for p in plans])
+            if k < nst:
+                solve.solve()
+        for p in plans:
+            if p[-1][2] is not None:
+                p[-1][2]()  # the member's CFL step
+
+    def _launch(self, key, parts):
+        """The members' parts as one launch per kernel: the batch recorded
+        for this key (recorded now, outside a capture), or the parts one by
+        one where they cannot be recorded."""
+        if key not in self._lb:
+            if _lib.capturing():
+                raise RuntimeError("ensemble: a member batch was not recorded before the capture")
+            try:
+                self._lb[key] = LaunchBatch(parts)
+            except _lib.F2DError:
+                self._lb[key] = None  # an entry point that cannot be recorded
+        lb = self._lb[key]
+        if lb is None:
+            for fn in parts:
+                fn()
+        else:
+            lb.run()
+
     # -- stepping -------------------------------------------------------------------
     def _member_step(self, i, dst):
         integ = self.models[i].integrator
         integ._select_dt()
         integ.advance(stats=False, dt_log=dst)
+
+    def _step_all(self, col):
+        """One step of every member; member i's dt into col[i]."""
+        if self.batch and self.groups():
+            self._batched_step(col)
+        else:
+            self._fork(lambda i: self._member_step(i, col[i:i + 1]))
 
     def _fork(self, body):
         main = torch.cuda.current_stream()
@@ -80,13 +249,19 @@ class Ensemble:
 
     def _capture(self):
         key = self._graph_key()
+        if self.batch:
+            self.groups()  # set up (device tables) before the capture
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with _lib.capture(g):
-            def body(i):
+            if self.batch and self.groups():
                 for n in range(self.G):
-                    self._member_step(i, self.dtlog[i, n:n + 1])
-            self._fork(body)
+                    self._step_all(self.dtlog[:, n])
+            else:
+                def body(i):
+                    for n in range(self.G):
+                        self._member_step(i, self.dtlog[i, n:n + 1])
+                self._fork(body)
         if self._graph_key() != key:
             raise RuntimeError("ensemble capture did not return the buffers to their roles")
         self.graph, self._key = g, key
@@ -113,8 +288,7 @@ class Ensemble:
                 dts[:, n:n + self.G].copy_(self.dtlog)
                 n += self.G
                 continue
-            col = n
-            self._fork(lambda i: self._member_step(i, dts[i, col:col + 1]))
+            self._step_all(dts[:, n])
             self._eager += 1
             n += 1
         return dts

@@ -220,6 +220,30 @@ class RKIntegrator:
         self.dstate, self.dalt = s, alt
         self._dtn_valid = dtn is not None
 
+    def staged(self, dt_log=None):
+        """advance(stats=False, dt_log=dt_log) cut at each stage's elliptic
+        solve, for a caller that batches the solves of many models (the
+        model must offer ``stage_split``; no forcing): per stage, the
+        (pre, post, cfl) callables of the model (cfl None but on the last
+        stage).  Running pre, the solve, post, cfl for k = 0, 1, ... is the
+        step.  The dt is the one in d_dt now (call after _select_dt)."""
+        assert not self.forcings and hasattr(self.model, "stage_split")
+        if dt_log is not None and not getattr(self.model, "logs_dt", False):
+            self._log_dt(dt_log)
+            dt_log = None
+        s = self.dstate
+        dtn = self.d_dtn if self._wants_dtn() else None
+        plan = []
+        for k in range(self.nstages):
+            ds_out = _one(self.dscratch[k]) if k < self.nstages - 1 else None
+            prev = [_one(self.dscratch[j]) for j in range(k)]
+            last = k == self.nstages - 1
+            plan.append(self.model.stage_split(s, prev, ds_out, k, self.d_dt,
+                                               dtn if last else None,
+                                               dt_log if k == 0 else None))
+        self._dtn_valid = dtn is not None
+        return plan
+
     def device_step(self, time):
         self.advance()
         time.pushforward()

@@ -244,6 +244,26 @@ class QGRSW(_SW):
     Thiry et al. 2024)."""
 
     def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats, dt_log=None):
+        self._stage_pre(s, k, dt, dt_log)
+        self.mesh.qg_helmholtz.solve(s.pv, s.psi, stats=stats)
+        self._stage_post(s, ds_prev, ds_out, k, dt)
+        return s, alt
+
+    def stage_split(self, s, ds_prev, ds_out, k, dt, dt_next, dt_log=None):
+        """stage() cut at its Helmholtz solve, for callers that batch the
+        solves of many models: (pre, post, cfl) callables (cfl None unless
+        dt_next is given); pre, the solve of s.pv into s.psi by
+        mesh.qg_helmholtz, post, then cfl run the stage.  pre and post
+        launch only kernels that f2d_record_begin can record."""
+        cfl = None
+        if dt_next is not None:
+            def cfl():
+                op.cfl_dt(self.cfl_velocity(s), self.param.cfl, self.param.dtmax, dt_next)
+        return ((lambda: self._stage_pre(s, k, dt, dt_log)),
+                (lambda: self._stage_post(s, ds_prev, ds_out, k, dt)), cfl)
+
+    def _stage_pre(self, s, k, dt, dt_log):
+        """The unprojected tendencies and the pv of the projection."""
         m, p = self.mesh, self.param
         ptr = _lib.ptr
         if self._pre is None:
@@ -263,13 +283,18 @@ class QGRSW(_SW):
             op.transport_rhs(m, p.compflux, s.h, s.U, pre.h)
         _ops("f2d_qg_pv", dt_, g, ptr(pre.u.x), ptr(pre.u.y), ptr(pre.h), ptr(m.d_slipcoef),
              ptr(m.d_mskv), float(-p.f0 / p.H), ptr(s.pv))
-        m.qg_helmholtz.solve(s.pv, s.psi, stats=stats)
+
+    def _stage_post(self, s, ds_prev, ds_out, k, dt):
+        """The projection (RK update of u and h) and the diag, from s.psi."""
+        m, p = self.mesh, self.param
+        ptr = _lib.ptr
+        pre = self._pre
+        dt_, g = self._dt(s.h), self._grid()
         st = self._stage_terms(ds_prev, ds_out, k, dt, (s.u.x, s.u.y, s.h))
         _ops("f2d_qg_project", dt_, g, ptr(s.psi), ptr(m.d_mskx), ptr(m.d_msky),
              ptr(m.d_mskv), ptr(m.d_msk), float(p.f0 * m.area / p.g), ptr(pre.u.x),
              ptr(pre.u.y), ptr(pre.h), ctypes.byref(st))
         self._diag_fused(s, energy=False)
-        return s, alt
 
     def rhs(self, s, ds):
         m, p = self.mesh, self.param

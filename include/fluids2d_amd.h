@@ -488,6 +488,59 @@ int f2d_poisson_read_stats(f2d_poisson* h, double* stats_host, void* stream);
 int f2d_poisson_bench_kernel(f2d_poisson* h, int kind, int reps, double* us,
                              double* bytes, void* stream);
 
+/* Member-batched solves for ensembles of independent models (no counterpart
+ * in the reference, which runs one Poisson2D.solve per model and call,
+ * elliptic.py:80-87): n handles of the same rectangle operator (grid,
+ * spacings, maindiag, periodicity; masks may differ), each solving with
+ * F2D_SOLVER_CAP in an FP32 mode with the same refinement options, bound to
+ * fixed device arrays b[i] (right-hand side at the vertices) and x[i]
+ * (solution, in/out).  f2d_poisson_batch_solve then performs
+ * f2d_poisson_solve(h[i], b[i], x[i], ..., NULL, NULL, stream) for every
+ * member -- the same kernels' arithmetic, so the same bits -- with one
+ * launch per kernel of the solve for all members.  Nothing synchronises;
+ * the solves count in each handle's statistics as solves without
+ * iters/relres.  The batch does not own the handles; destroy it first. */
+typedef struct f2d_poisson_batch f2d_poisson_batch;
+int f2d_poisson_batch_create(f2d_poisson* const* h, int32_t n, const void* const* b,
+                             void* const* x, f2d_poisson_batch** out);
+/* The solves of every member on `stream` (no counterpart in the reference:
+ * Poisson2D.solve per model, elliptic.py:80-87). */
+int f2d_poisson_batch_solve(f2d_poisson_batch* bt, void* stream);
+/* Frees the batch's device tables (no counterpart in the reference). */
+int f2d_poisson_batch_destroy(f2d_poisson_batch* bt);
+
+/* Member batching of kernel launches (no counterpart in the reference, which
+ * advances one model per process, model.py:29-69).  Between
+ * f2d_record_begin and f2d_record_end the calling thread's calls of
+ * f2d_sw_momentum, f2d_sw_momentum_u, f2d_sw_diag, f2d_qg_pv,
+ * f2d_qg_project and the f2d_transport_stage family check their arguments
+ * and record their kernel launches (kernel, grid and argument values)
+ * instead of enqueueing them; every other entry point fails while a
+ * recording is open.  f2d_launch_batch_create merges the recordings of n
+ * members, which must have recorded the same kernels with the same grids in
+ * the same order, into device tables; f2d_launch_batch_run then enqueues
+ * one launch per recorded kernel for all members (member i's workgroups
+ * run member i's recorded arguments: its launches' results bit for bit,
+ * provided no two members write the same arrays).  Recordings and batches
+ * are independent of each other once created. */
+typedef struct f2d_recording f2d_recording;
+typedef struct f2d_launch_batch f2d_launch_batch;
+int f2d_record_begin(void);
+/* Closes the thread's recording into *out (no counterpart in the
+ * reference). */
+int f2d_record_end(f2d_recording** out);
+/* Number of launches recorded, -1 for NULL (no counterpart in the
+ * reference). */
+int f2d_recording_size(const f2d_recording* r);
+/* Frees a recording (no counterpart in the reference). */
+int f2d_recording_destroy(f2d_recording* r);
+/* Merges n recordings and synchronises (no counterpart in the reference). */
+int f2d_launch_batch_create(f2d_recording* const* recs, int32_t n, f2d_launch_batch** out);
+/* Enqueues the batch on `stream` (no counterpart in the reference). */
+int f2d_launch_batch_run(f2d_launch_batch* bt, void* stream);
+/* Frees the batch's tables, synchronising (no counterpart in the reference). */
+int f2d_launch_batch_destroy(f2d_launch_batch* bt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,3 +157,93 @@ def test_capacitance_choice(dev, L):
     h = _handle(L, unk, pp, 0.0, L.POISSON_F64, "auto")
     assert L.lib().f2d_poisson_method(h) == L.SOLVERS["mg"]
     L.call("f2d_poisson_destroy", h)
+
+
+def test_capacitance_batch(dev, L):
+    """f2d_poisson_batch_solve against each member's own f2d_poisson_solve,
+    bit for bit (solution and halo images): two groups -- closed 64^2
+    basins and x-periodic 64x128 channels with islands -- of different
+    masks, cold first solve, then warm solves of changed right-hand sides
+    written into the bound arrays; a batch with some members warm and some
+    cold; refusals of members that cannot share a batch."""
+    import torch
+    from fluids2d_amd._lib import F2DError
+    mode = L.POISSON_F32_REFINE
+    rng = np.random.default_rng(21)
+
+    def group(ny, nx, xper, kind, seeds, maindiag):
+        p = P(model="rsw", nx=nx, ny=ny, xperiodic=xper, Lx=nx / 64.0, Ly=ny / 64.0)
+        out = []
+        for sd in seeds:
+            mesh = O.Mesh(p, msk=_mask(kind, ny, nx, xper, False, sd))
+            unk = O.unknown_mask(p, mesh.mskv)
+            out.append(dict(p=p, mskv=mesh.mskv, hb=_handle(L, unk, p, maindiag, mode),
+                            hs=_handle(L, unk, p, maindiag, mode),
+                            b=torch.zeros(unk.shape, dtype=torch.float32, device=dev),
+                            xb=torch.zeros(unk.shape, dtype=torch.float32, device=dev),
+                            xs=torch.zeros(unk.shape, dtype=torch.float32, device=dev)))
+        return out
+
+    groups = [group(64, 64, False, "basin", (1, 2, 3), 0.0),
+              group(64, 128, True, "islands", (4, 5), 0.4)]
+
+    def batch(ms):
+        arr = ctypes.c_void_p * len(ms)
+        h = ctypes.c_void_p()
+        L.call("f2d_poisson_batch_create", arr(*(m["hb"] for m in ms)), len(ms),
+               arr(*(L.ptr(m["b"]) for m in ms)), arr(*(L.ptr(m["xb"]) for m in ms)),
+               ctypes.byref(h))
+        return h
+
+    batches = [batch(ms) for ms in groups]
+    for rnd in range(3):
+        for ms, bt in zip(groups, batches):
+            for m in ms:
+                m["b"].copy_(torch.from_numpy(rng.standard_normal(m["mskv"].shape) *
+                                              m["mskv"]).to(dev, torch.float32))
+                L.call("f2d_poisson_solve", m["hs"], L.ptr(m["b"]), L.ptr(m["xs"]), 1e-9, 10,
+                       None, None, L.stream_ptr())
+            L.call("f2d_poisson_batch_solve", bt, L.stream_ptr())
+        torch.cuda.synchronize()
+        for g, ms in enumerate(groups):
+            for i, m in enumerate(ms):
+                assert torch.equal(m["xb"], m["xs"]), (rnd, g, i)
+                assert float(m["xb"].abs().max()) > 0
+    print(f"\nbatched capacitance: {[len(ms) for ms in groups]} members, 3 solves bit-equal")
+
+    # members warm and cold together: each member's own solve
+    ms = groups[0]
+    fresh = group(64, 64, False, "basin", (1, 2, 3), 0.0)
+    for m, f in zip(ms, fresh):
+        f["b"].copy_(m["b"])
+    L.call("f2d_poisson_solve", fresh[1]["hb"], L.ptr(fresh[1]["b"]), L.ptr(fresh[1]["xb"]),
+           1e-9, 10, None, None, L.stream_ptr())
+    L.call("f2d_poisson_solve", fresh[1]["hs"], L.ptr(fresh[1]["b"]), L.ptr(fresh[1]["xs"]),
+           1e-9, 10, None, None, L.stream_ptr())
+    bt = batch(fresh)
+    L.call("f2d_poisson_batch_solve", bt, L.stream_ptr())
+    for f in fresh:
+        L.call("f2d_poisson_solve", f["hs"], L.ptr(f["b"]), L.ptr(f["xs"]), 1e-9, 10,
+               None, None, L.stream_ptr())
+    torch.cuda.synchronize()
+    for i, f in enumerate(fresh):
+        assert torch.equal(f["xb"], f["xs"]), i
+    L.call("f2d_poisson_batch_destroy", bt)
+
+    # members that cannot share a batch
+    with pytest.raises(F2DError, match="share the rectangle"):
+        batch([groups[0][0], groups[1][0]])
+    p = groups[0][0]["p"]
+    unk = O.unknown_mask(p, groups[0][0]["mskv"])
+    h64 = _handle(L, unk, p, 0.0, L.POISSON_F64)
+    hmg = _handle(L, unk, p, 0.0, mode, "mg")
+    for h in (h64, hmg):
+        with pytest.raises(F2DError, match="FP32 capacitance"):
+            batch([dict(groups[0][0], hb=h)])
+        L.call("f2d_poisson_destroy", h)
+    for bt in batches:
+        L.call("f2d_poisson_batch_destroy", bt)
+    for ms in groups + [fresh]:
+        for m in ms:
+            L.call("f2d_poisson_destroy", m["hb"])
+            L.call("f2d_poisson_destroy", m["hs"])

@@ -64,6 +64,49 @@ def test_ensemble_matches_solo_runs(dev, precision):
             assert np.isfinite(a[name][wet]).all(), (k, name)
 
 
+BATCH_SPECS = [(64, "qgrsw", 11), (128, "qgrsw", 12), (64, "qgrsw", 13), (32, "qgrsw", 14),
+               (64, "qgrsw", 15), (40, "qg", 16), (128, "qgrsw", 17), (48, "qgrsw", 18)]
+
+
+def test_ensemble_batched_solves(dev):
+    """Same-size qgrsw members stepped stage by stage as groups -- one
+    launch per kernel before and after the solves, the Helmholtz solves
+    batched (FP32 capacitance solves: groups 64^2 x3, 128^2 x2, 32^2 x1;
+    the 48^2 qgrsw member (multigrid) and the qg member take whole steps)
+    -- give the unbatched ensemble's and each member's solo run, bit for
+    bit, eager and replayed."""
+    from fluids2d_amd.ensemble import Ensemble
+    print(f"\nbatched ensemble: {BATCH_SPECS}", flush=True)
+    runs = [_members(BATCH_SPECS, "fp32") for _ in range(3)]
+    for ms in runs:
+        for m in ms:
+            m.integrator.begin(m.state)
+    eb = Ensemble(runs[0], nstreams=4, graph_steps=2, batch=True)
+    eu = Ensemble(runs[1], nstreams=4, graph_steps=2, batch=False)
+    sizes = sorted(len(members) for members, _ in eb.groups())
+    assert sizes == [1, 2, 3], sizes
+    d = []
+    for ens in (eb, eu):
+        d.append(np.concatenate([ens.steps(5).cpu().numpy(), ens.steps(4).cpu().numpy()],
+                                axis=1))
+    assert eb.graph is not None
+    # every stage part of every group ran as recorded launch batches
+    assert eb._lb and all(v is not None and v.launches > 0 for v in eb._lb.values())
+    assert np.array_equal(d[0], d[1])
+    for k, (mb, mu, ms) in enumerate(zip(*runs)):
+        dts = np.concatenate([ms.integrator.device_steps(5).cpu().numpy(),
+                              ms.integrator.device_steps(4).cpu().numpy()])
+        assert np.array_equal(dts, d[0][k]), k
+        names = NAMES[ms.param.model]
+        a, b, c = _state(mb, names), _state(mu, names), _state(ms, names)
+        wet = ms.mesh.msk == 1
+        for name in a:
+            for other in (b, c):
+                assert np.array_equal(np.nan_to_num(a[name]), np.nan_to_num(other[name])), \
+                    (k, name)
+            assert np.isfinite(a[name][wet]).all(), (k, name)
+
+
 def test_ensemble_stream_tuning(dev):
     from fluids2d_amd.ensemble import Ensemble
     ms = _members(SPECS[:4], "fp32")
