@@ -9,7 +9,9 @@
 // Kernels of one FP64 solve:
 //   k_cap_rhs    b masked to the unknowns W (b_W), zero elsewhere
 //   (3 spectral passes)   y = A_R^-1 b_W
-//   k_cap_gemv   s = M y_B (M = (-C)^-1), written into b_W at the points B
+//   k_cap_symv   partial products of the symmetric M = (-C)^-1 with y_B,
+//                one 64 x 64 tile of its lower triangle per workgroup
+//   k_cap_symv_sum  s = M y_B from the partials, written into b_W at B
 //   (3 spectral passes)   y = A_R^-1 (b_W + P_B^T s)
 //   k_cap_out    x = y at the unknowns W
 // The FP32 mode wraps the same sequence, in FP32 with an FP32 M, into FP64
@@ -98,47 +100,123 @@ __global__ __launch_bounds__(256) void k_cap_res(CapGeo g, const TB* __restrict_
   cap_res_at<TB, TX, T>(g, b, gmode, mw, xin, r, k);
 }
 
-// s = M y_B for the rows of this workgroup, written to rhs at the points B
-// (rhs is zero there).  y_B is gathered into LDS; each wavefront takes one
-// row at a time, two (FP64) or four (FP32) entries per lane per load, and
-// sums in FP64 in a fixed order.  Row pitch ld (a multiple of 4, padding 0).
-template <class T>
-__device__ __forceinline__ void cap_gemv_block(const T* __restrict__ M, int n, int ld,
-                                               const int* __restrict__ bidx,
-                                               const T* __restrict__ y,
-                                               T* __restrict__ rhs, int rows, T* gy) {
-  for (int j = threadIdx.x; j < ld; j += 256) gy[j] = j < n ? y[bidx[j]] : T(0);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = blockIdx.x * rows;
-  const int r1 = r0 + rows < n ? r0 + rows : n;
-  constexpr int V = sizeof(T) == 8 ? 2 : 4;  // entries per 16-byte load
-  for (int r = r0 + wave; r < r1; r += 4) {
-    const T* row = M + (int64_t)r * ld;
-    double acc = 0.0;
-    for (int j = lane * V; j < ld; j += 64 * V) {
-      if (V == 2) {
-        const double2 m = *reinterpret_cast<const double2*>(row + j);
-        acc += m.x * (double)gy[j] + m.y * (double)gy[j + 1];
-      } else {
-        const float4 m = *reinterpret_cast<const float4*>(row + j);
-        acc += ((double)m.x * gy[j] + (double)m.y * gy[j + 1]) +
-               ((double)m.z * gy[j + 2] + (double)m.w * gy[j + 3]);
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) rhs[bidx[r]] = T(acc);
-  }
+// M = (-C)^-1 is symmetric: only the tiles (I, J), I >= J, of its lower
+// triangle are stored, 64 x 64 entries each, row-major, in the order
+// t = I (I + 1) / 2 + J, zero beyond n (diagonal tiles hold both
+// triangles).  s = M y_B in two passes: each tile's workgroup forms
+// M_IJ y_J (row partials) and, off the diagonal, M_IJ^T y_I (column
+// partials); then each row of s sums its nt partials in a fixed order (FP64,
+// deterministic).  Half the bytes of the full matrix are read.
+constexpr int CT = 64;  // tile size
+
+__device__ __forceinline__ void tile_ij(int t, int& I, int& J) {
+  int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > t) i--;
+  while ((i + 1) * (i + 2) / 2 <= t) i++;
+  I = i;
+  J = t - i * (i + 1) / 2;
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_cap_gemv(const T* __restrict__ M, int n, int ld,
+__device__ __forceinline__ void load4(const T* p, double m[4]) {
+  if (sizeof(T) == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    m[0] = v.x;
+    m[1] = v.y;
+    m[2] = v.z;
+    m[3] = v.w;
+  } else {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 b = *reinterpret_cast<const double2*>(p + 2);
+    m[0] = a.x;
+    m[1] = a.y;
+    m[2] = b.x;
+    m[3] = b.y;
+  }
+}
+
+// partials of tile t (256 threads): part[t][0][r] = sum_c M_IJ[r][c] y_J[c],
+// part[t][1][c] = sum_r M_IJ[r][c] y_I[r] (I > J).  Thread k holds columns
+// 4 (k % 16) .. + 3 of rows k / 16 + 16 q, q = 0 .. 3.
+template <class T>
+__device__ __forceinline__ void cap_symv_tile(const T* __restrict__ Mt, int n,
+                                              const int* __restrict__ bidx,
+                                              const T* __restrict__ y,
+                                              double* __restrict__ part, int t) {
+  __shared__ double yi[CT], yj[CT], cs[4][CT];
+  int I, J;
+  tile_ij(t, I, J);
+  const int tid = threadIdx.x;
+  if (tid < CT) {
+    const int gi = I * CT + tid;
+    yi[tid] = gi < n ? (double)y[bidx[gi]] : 0.0;
+  } else if (tid < 2 * CT) {
+    const int gj = J * CT + tid - CT;
+    yj[tid - CT] = gj < n ? (double)y[bidx[gj]] : 0.0;
+  }
+  __syncthreads();
+  const int c4 = (tid & 15) * 4, r0 = tid >> 4;
+  const T* tile = Mt + (int64_t)t * CT * CT;
+  double m[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) load4<T>(tile + (r0 + 16 * q) * CT + c4, m[q]);
+  double* pr = part + (int64_t)t * 2 * CT;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    double v = (m[q][0] * yj[c4] + m[q][1] * yj[c4 + 1]) +
+               (m[q][2] * yj[c4 + 2] + m[q][3] * yj[c4 + 3]);
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 15) == 0) pr[r0 + 16 * q] = v;
+  }
+  if (I == J) return;  // uniform per workgroup
+  double cv[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    double v = (m[0][c] * yi[r0] + m[1][c] * yi[r0 + 16]) +
+               (m[2][c] * yi[r0 + 32] + m[3][c] * yi[r0 + 48]);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    cv[c] = v;
+  }
+  const int wave = tid >> 6;
+  if ((tid & 63) < 16) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) cs[wave][c4 + c] = cv[c];
+  }
+  __syncthreads();
+  if (tid < CT) pr[CT + tid] = (cs[0][tid] + cs[1][tid]) + (cs[2][tid] + cs[3][tid]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cap_symv(const T* __restrict__ Mt, int n,
                                                   const int* __restrict__ bidx,
                                                   const T* __restrict__ y,
-                                                  T* __restrict__ rhs, int rows) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  cap_gemv_block<T>(M, n, ld, bidx, y, rhs, rows, reinterpret_cast<T*>(smem));
+                                                  double* __restrict__ part) {
+  cap_symv_tile<T>(Mt, n, bidx, y, part, (int)blockIdx.x);
+}
+
+// s_i = sum_{J <= I} part[(I, J)][0][r] + sum_{K > I} part[(K, I)][1][r],
+// i = 64 I + r, into rhs at the point B_i
+template <class T>
+__device__ __forceinline__ void cap_symv_sum_at(int n, int nt, const double* __restrict__ part,
+                                                const int* __restrict__ bidx,
+                                                T* __restrict__ rhs, int i) {
+  const int I = i / CT, r = i - I * CT;
+  double acc = 0.0;
+  for (int J = 0; J <= I; J++) acc += part[((int64_t)I * (I + 1) / 2 + J) * 2 * CT + r];
+  for (int K = I + 1; K < nt; K++)
+    acc += part[((int64_t)K * (K + 1) / 2 + I) * 2 * CT + CT + r];
+  rhs[bidx[i]] = T(acc);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cap_symv_sum(int n, int nt,
+                                                      const double* __restrict__ part,
+                                                      const int* __restrict__ bidx,
+                                                      T* __restrict__ rhs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) cap_symv_sum_at<T>(n, nt, part, bidx, rhs, i);
 }
 
 // x = y at the unknowns
@@ -234,9 +312,10 @@ struct CapMember {
   float* rhs;          // scratch: right-hand side of the spectral solves
   float* y;            // scratch: spectral solution
   double* x64;         // FP64 accumulator
-  const float* M;      // (-C)^-1, n rows of pitch ld
+  const float* M;      // (-C)^-1, lower-triangle tiles
   const int* bidx;     // padded index of each point of B
-  int n, ld, rows, pad;
+  double* part;        // tile partials
+  int n, nt, ntl, pad;
 };
 
 // from: 0 x = 0 (cold first pass), 1 the caller's x, 2 the accumulator
@@ -251,13 +330,17 @@ __global__ __launch_bounds__(256) void k_cap_res_b(CapGeo g, const CapMember* __
     cap_res_at<float, double, float>(g, m.b, 0, m.mw, from ? m.x64 : nullptr, m.rhs, k);
 }
 
-// grid x: the largest member's workgroup count (the others return early)
-__global__ __launch_bounds__(256) void k_cap_gemv_b(const CapMember* __restrict__ t) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// grid x: the largest member's tile count (the others return early)
+__global__ __launch_bounds__(256) void k_cap_symv_b(const CapMember* __restrict__ t) {
   const CapMember& m = t[blockIdx.y];
-  if ((int64_t)blockIdx.x * m.rows >= m.n) return;
-  cap_gemv_block<float>(m.M, m.n, m.ld, m.bidx, m.y, m.rhs, m.rows,
-                        reinterpret_cast<float*>(smem));
+  if ((int)blockIdx.x >= m.ntl) return;
+  cap_symv_tile<float>(m.M, m.n, m.bidx, m.y, m.part, (int)blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_cap_symv_sum_b(const CapMember* __restrict__ t) {
+  const CapMember& m = t[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < m.n) cap_symv_sum_at<float>(m.n, m.nt, m.part, m.bidx, m.rhs, i);
 }
 
 // x64 = x0 + y at the unknowns (from as in k_cap_res_b)
@@ -306,17 +389,20 @@ __global__ __launch_bounds__(256) void k_cap_build(const double* __restrict__ G,
   C[e] = -v;  // -C: symmetric positive definite
 }
 
-// M = the inverse held in the lower triangle of A (column-major, n x n),
-// expanded to both triangles, row pitch ld, padding columns zero
+// the lower-triangle tiles of M from the inverse held in the lower
+// triangle of A (column-major, n x n); zero beyond n
 template <class T>
-__global__ __launch_bounds__(256) void k_cap_expand(const double* __restrict__ A, int n,
-                                                    int ld, T* __restrict__ M) {
+__global__ __launch_bounds__(256) void k_cap_tiles(const double* __restrict__ A, int n,
+                                                   int64_t ntl, T* __restrict__ Mt) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (int64_t)n * ld) return;
-  const int i = (int)(e / ld), j = (int)(e - (int64_t)i * ld);
+  if (e >= ntl * CT * CT) return;
+  const int t = (int)(e / (CT * CT)), w = (int)(e - (int64_t)t * CT * CT);
+  int I, J;
+  tile_ij(t, I, J);
+  const int gi = I * CT + w / CT, gj = J * CT + w % CT;
   double v = 0.0;
-  if (j < n) v = i >= j ? A[i + (int64_t)j * n] : A[j + (int64_t)i * n];
-  M[e] = T(v);
+  if (gi < n && gj < n) v = gi >= gj ? A[gi + (int64_t)gj * n] : A[gj + (int64_t)gi * n];
+  Mt[e] = T(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -325,10 +411,11 @@ __global__ __launch_bounds__(256) void k_cap_expand(const double* __restrict__ A
 struct CapSolver {
   CapGeo g{};
   bool f64 = true;
-  int n = 0, ld = 0, rows = 4;
+  int n = 0, nt = 0, ntl = 0;  // boundary points, tile rows, stored tiles
   int8_t* mw = nullptr;     // unknowns (padded)
   int* bidx = nullptr;      // padded index of each point of B
-  void* M = nullptr;        // (-C)^-1, n rows of pitch ld (FP64 or FP32)
+  void* M = nullptr;        // (-C)^-1, lower-triangle tiles (FP64 or FP32)
+  double* tpart = nullptr;  // tile partials of the product, 2 x 64 per tile
   void* rhs = nullptr;      // padded scratch: right-hand side (T)
   void* y = nullptr;        // padded scratch: spectral solution (T)
   double* x64 = nullptr;    // FP32 mode: FP64 accumulator (padded)
@@ -428,8 +515,8 @@ CapSolver* cap_create(const int8_t* unk, int64_t ny, int64_t nx, double dx, doub
     const std::vector<int64_t> B = boundary_points(unk, ny, nx, xper, yper);
     const int n = (int)B.size();
     c->n = n;
-    c->ld = (n + 3) & ~3;
-    c->rows = 4 * ((n + 4 * 512 - 1) / (4 * 512));  // about 512 workgroups
+    c->nt = (n + CT - 1) / CT;
+    c->ntl = c->nt * (c->nt + 1) / 2;
     c->mw = (int8_t*)calloc_dev(c, (size_t)N);
     F2D_HIP(hipMemcpy(c->mw, unk, (size_t)N, hipMemcpyHostToDevice));
     std::vector<int> bi(n), bxv(n), byv(n);
@@ -493,13 +580,14 @@ CapSolver* cap_create(const int8_t* unk, int64_t ny, int64_t nx, double dx, doub
       (void)hipFree(A);
       throw Error{"capacitance matrix inversion failed (potri info " + std::to_string(ih) + ")"};
     }
-    const int64_t nm = (int64_t)n * c->ld;
+    const int64_t nm = (int64_t)c->ntl * CT * CT;
     const size_t tsz = f64_mode ? sizeof(double) : sizeof(float);
     c->M = calloc_dev(c, tsz * (size_t)nm);
+    c->tpart = (double*)calloc_dev(c, sizeof(double) * 2 * CT * (size_t)c->ntl);
     if (f64_mode)
-      k_cap_expand<double><<<(unsigned)((nm + 255) / 256), 256>>>(A, n, c->ld, (double*)c->M);
+      k_cap_tiles<double><<<(unsigned)((nm + 255) / 256), 256>>>(A, n, c->ntl, (double*)c->M);
     else
-      k_cap_expand<float><<<(unsigned)((nm + 255) / 256), 256>>>(A, n, c->ld, (float*)c->M);
+      k_cap_tiles<float><<<(unsigned)((nm + 255) / 256), 256>>>(A, n, c->ntl, (float*)c->M);
     F2D_HIP(hipGetLastError());
     F2D_HIP(hipDeviceSynchronize());
     (void)hipFree(A);
@@ -510,17 +598,6 @@ CapSolver* cap_create(const int8_t* unk, int64_t ny, int64_t nx, double dx, doub
     c->nbres = (int)((N + 255) / 256);
     c->part = (double*)calloc_dev(c, sizeof(double) * 2 * c->nbres);
     c->fft = fft_create(ny, nx, dx, dy, maindiag, f64_mode, !f64_mode, xper, yper);
-    // the GEMV's y_B in LDS
-    const size_t lds = tsz * (size_t)c->ld;
-    F2D_REQUIRE(lds <= 160 * 1024, "capacitance boundary too long for the LDS gather");
-    if (lds > 64 * 1024) {
-      if (f64_mode)
-        F2D_HIP(hipFuncSetAttribute((const void*)k_cap_gemv<double>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      else
-        F2D_HIP(hipFuncSetAttribute((const void*)k_cap_gemv<float>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
     F2D_HIP(hipDeviceSynchronize());
   } catch (...) {
     if (rb) rocblas_destroy_handle(rb);
@@ -546,9 +623,10 @@ static unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 // the capacitance correction between the two spectral solves of rhs
 template <class T>
 static void gemv(CapSolver* c, hipStream_t s) {
-  const int nwg = (c->n + c->rows - 1) / c->rows;
-  k_cap_gemv<T><<<nwg, 256, sizeof(T) * (size_t)c->ld, s>>>(
-      (const T*)c->M, c->n, c->ld, c->bidx, (const T*)c->y, (T*)c->rhs, c->rows);
+  k_cap_symv<T><<<c->ntl, 256, 0, s>>>((const T*)c->M, c->n, c->bidx, (const T*)c->y,
+                                       c->tpart);
+  k_cap_symv_sum<T><<<blocks(c->n), 256, 0, s>>>(c->n, c->nt, c->tpart, c->bidx,
+                                                  (T*)c->rhs);
 }
 
 template <class TB, class TX>
@@ -616,8 +694,7 @@ struct CapBatch {
   std::vector<float*> x;
   CapMember* t = nullptr;      // device member table
   FftBatchPtrs* ft = nullptr;  // device: rhs -> y through each member's spectrum
-  int maxnwg = 0;
-  size_t lds = 0;
+  int maxntl = 0, maxn = 0;
 };
 
 static bool same_geo(const CapGeo& a, const CapGeo& b) {
@@ -642,20 +719,15 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
       bt->b.push_back(b[i]);
       bt->x.push_back(x[i]);
       t[i] = CapMember{b[i], x[i], m->mw, (float*)m->rhs, (float*)m->y, m->x64,
-                       (const float*)m->M, m->bidx, m->n, m->ld, m->rows, 0};
+                       (const float*)m->M, m->bidx, m->tpart, m->n, m->nt, m->ntl, 0};
       ft[i] = FftBatchPtrs{m->rhs, fft_spectrum32(m->fft), m->y};
-      const int nwg = (m->n + m->rows - 1) / m->rows;
-      if (nwg > bt->maxnwg) bt->maxnwg = nwg;
-      if (sizeof(float) * (size_t)m->ld > bt->lds) bt->lds = sizeof(float) * (size_t)m->ld;
+      if (m->ntl > bt->maxntl) bt->maxntl = m->ntl;
+      if (m->n > bt->maxn) bt->maxn = m->n;
     }
     F2D_HIP(hipMalloc(&bt->t, sizeof(CapMember) * nm));
     F2D_HIP(hipMalloc(&bt->ft, sizeof(FftBatchPtrs) * nm));
     F2D_HIP(hipMemcpy(bt->t, t.data(), sizeof(CapMember) * nm, hipMemcpyHostToDevice));
     F2D_HIP(hipMemcpy(bt->ft, ft.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
-    F2D_REQUIRE(bt->lds <= 160 * 1024, "capacitance boundary too long for the LDS gather");
-    if (bt->lds > 64 * 1024)
-      F2D_HIP(hipFuncSetAttribute((const void*)k_cap_gemv_b,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt->lds));
   } catch (...) {
     cap_batch_destroy(bt);
     throw;
@@ -696,7 +768,8 @@ void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s) {
     const int from = p > 0 ? 2 : (from_x ? 1 : 0);
     k_cap_res_b<<<gp, 256, 0, s>>>(g, bt->t, from);
     fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
-    k_cap_gemv_b<<<dim3((unsigned)bt->maxnwg, (unsigned)nm), 256, bt->lds, s>>>(bt->t);
+    k_cap_symv_b<<<dim3((unsigned)bt->maxntl, (unsigned)nm), 256, 0, s>>>(bt->t);
+    k_cap_symv_sum_b<<<dim3(blocks(bt->maxn), (unsigned)nm), 256, 0, s>>>(bt->t);
     fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
     k_cap_acc_b<<<gp, 256, 0, s>>>(N, bt->t, from);
   }
