@@ -672,6 +672,11 @@ def main():
                     help="steps of the rsw 2048x1024 island channel per precision (0: skip)")
     ap.add_argument("--only-rsw", action="store_true",
                     help="time only the rsw island channel (for profiling its kernels)")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="time only the qgrsw ensemble (for profiling its kernels)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="ensemble: default stream count and solver schedules (no tuner "
+                         "runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain elementwise copy of MIB MiB three times "
                          "(a kernel of known traffic, to scale rocprofv3 counters)")
@@ -696,6 +701,11 @@ def main():
 
     if a.only_rsw:
         print(json.dumps({"rsw": rsw_bench(a.seed, a.rsw_steps or a.steps, a.warmup)}))
+        return
+    if a.only_ensemble:
+        print(json.dumps({"ensemble": ensemble_bench(
+            a.ensemble or 64, a.ensemble_steps, tune=not a.no_tune,
+            log=lambda m: print(m, file=sys.stderr, flush=True))}))
         return
 
     from fluids2d_amd import autotune

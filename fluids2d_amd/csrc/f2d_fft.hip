@@ -404,18 +404,9 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
       const cplx<T> u = csub<T>(C, D);
       X = mk<T>(T(0.5) * u.y, T(-0.5) * u.x);
     }
-    const int y = y0 + 2 * p + ab;
-    S[(int64_t)k * a.ny + y] = X;
-    if (a.ey == 1) {
-      // closed y: the workgroups cover rows 0 .. ony-1; row 2 ony - y is the
-      // reflection of row y with the opposite sign, row ony a boundary line
-      if (y > 0)
-        S[(int64_t)k * a.ny + 2 * a.ony - y] = mk<T>(-X.x, -X.y);
-      else
-        S[(int64_t)k * a.ny + a.ony] = mk<T>(T(0), T(0));
-    } else if (a.ey == 2) {
-      S[(int64_t)k * a.ny + 2 * a.ony - 1 - y] = X;  // the even reflection
-    }
+    // (closed y: the workgroups cover rows 0 .. ony-1; the column pass
+    // forms the reflected rows itself)
+    S[(int64_t)k * a.ny + y0 + 2 * p + ab] = X;
   }
 }
 
@@ -436,7 +427,27 @@ __global__ __launch_bounds__(1024) void k_fft_cols(FftArgs a,
   cplx<T>* b1 = b0 + a.Q * ny;
   cplx<T>* tl = b1 + a.Q * ny;
   cplx<T>* col = S + (int64_t)k0 * ny;
-  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) b0[w] = col[w];
+  if (a.ey) {
+    // closed y: rows 0 .. ony-1 are stored; row 2 ony - y is row y with the
+    // opposite sign and row ony a boundary line (odd), or row 2 ony - 1 - y
+    // is row y (even)
+    const int on = a.ony;
+    for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+      const int q = w >> lg, y = w & (ny - 1);
+      cplx<T> v;
+      if (y < on) {
+        v = col[w];
+      } else if (a.ey == 1) {
+        const cplx<T> u = y > on ? col[q * ny + 2 * on - y] : mk<T>(T(0), T(0));
+        v = mk<T>(-u.x, -u.y);
+      } else {
+        v = col[q * ny + 2 * on - 1 - y];
+      }
+      b0[w] = v;
+    }
+  } else {
+    for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) b0[w] = col[w];
+  }
   for (int w = threadIdx.x; w < ny; w += blockDim.x) tl[w] = tw[w];
   __syncthreads();
   cplx<T>* r = fft_lds<T>(b0, b1, ny, lg, nq, tl);
@@ -451,7 +462,9 @@ __global__ __launch_bounds__(1024) void k_fft_cols(FftArgs a,
   }
   __syncthreads();
   const cplx<T>* z = fft_lds<T>(r, o, ny, lg, nq, tl);
+  // (closed y: the row pass reads the problem's rows only)
   for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+    if (a.ey && (w & (ny - 1)) >= a.ony) continue;
     const cplx<T> v = z[w];
     col[w] = mk<T>(v.x, -v.y);
   }
