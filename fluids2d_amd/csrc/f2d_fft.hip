@@ -56,6 +56,8 @@ __device__ __forceinline__ int row_block(int b, int G) {
 struct FftArgs {
   int nx, ny, lgx, lgy;
   int nk;        // nx / 2 + 1 spectral columns
+  int ncu;       // column-pass units: nk complex columns, or (odd x) the
+                 // pairs of real columns, (nk + 1) / 2
   int NXP;       // padded row stride
   int P;         // row pairs per workgroup (row passes)
   int Q;         // spectral columns per workgroup (column pass)
@@ -406,7 +408,78 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
     }
     // (closed y: the workgroups cover rows 0 .. ony-1; the column pass
     // forms the reflected rows itself)
-    S[(int64_t)k * a.ny + y0 + 2 * p + ab] = X;
+    const int64_t at = (int64_t)k * a.ny + y0 + 2 * p + ab;
+    if (a.ex == 1)  // odd x: the spectrum of an odd real row is imaginary
+      reinterpret_cast<T*>(S)[at] = X.y;
+    else
+      S[at] = X;
+  }
+}
+
+// Column pass of an odd (Dirichlet) x: the spectrum of an odd real row is
+// imaginary (x[nx - n] = -x[n] gives X = -conj(X)), so its imaginary part
+// is stored and two columns k1 = 2u, k2 = 2u + 1 travel as one complex
+// column z = s1 + i s2.  (An even extension's spectrum carries a phase
+// e^(i pi k / nx): those columns stay complex.)
+// With Z = F(z) and Zc(ky) = conj(Z(ny - ky)), F(s1) = (Z + Zc) / 2 and
+// F(s2) = (Z - Zc) / (2i); after the division by the eigenvalues (even in
+// ky, so both stay spectra of real columns) the inverse of
+// F(s1) c1 + i F(s2) c2 = Z (c1 + c2) / 2 + Zc (c1 - c2) / 2 returns the two
+// solved columns as its real and imaginary parts.
+template <class T>
+__device__ void cols_real_pairs(const FftArgs& a, T* __restrict__ Sr,
+                                const cplx<T>* __restrict__ tw,
+                                const double* __restrict__ lamx,
+                                const double* __restrict__ lamy) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ny = a.ny, lg = a.lgy, on = a.ony;
+  const int u0 = blockIdx.x * a.Q;
+  const int nq = min(a.Q, a.ncu - u0);
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* b1 = b0 + a.Q * ny;
+  cplx<T>* tl = b1 + a.Q * ny;
+  // value of stored column k at doubled row y (closed y: reflected rows)
+  auto sval = [&](int k, int y) -> T {
+    const T* c = Sr + (int64_t)k * ny;
+    if (!a.ey || y < on) return c[y];
+    if (a.ey == 1) return y > on ? -c[2 * on - y] : T(0);
+    return c[2 * on - 1 - y];
+  };
+  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+    const int q = w >> lg, y = w & (ny - 1);
+    const int k1 = 2 * (u0 + q), k2 = k1 + 1;
+    b0[w] = mk<T>(sval(k1, y), k2 < a.nk ? sval(k2, y) : T(0));
+  }
+  for (int w = threadIdx.x; w < ny; w += blockDim.x) tl[w] = tw[w];
+  __syncthreads();
+  cplx<T>* r = fft_lds<T>(b0, b1, ny, lg, nq, tl);
+  cplx<T>* o = r == b0 ? b1 : b0;
+  const double inv = 1.0 / ((double)a.nx * (double)a.ny);
+  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+    const int q = w >> lg, ky = w & (ny - 1);
+    const int k1 = 2 * (u0 + q), k2 = k1 + 1;
+    const double l1 = lamx[k1] + lamy[ky];
+    const T c1 = l1 != 0.0 ? T(inv / l1) : T(0);
+    T c2 = T(0);
+    if (k2 < a.nk) {
+      const double l2 = lamx[k2] + lamy[ky];
+      c2 = l2 != 0.0 ? T(inv / l2) : T(0);
+    }
+    const cplx<T> Z = r[w], Zm = r[q * ny + ((ny - ky) & (ny - 1))];
+    const T hp = T(0.5) * (c1 + c2), hm = T(0.5) * (c1 - c2);
+    const T zx = Z.x * hp + Zm.x * hm, zy = Z.y * hp - Zm.y * hm;
+    o[w] = mk<T>(zx, -zy);  // conjugate: inverse by forward
+  }
+  __syncthreads();
+  const cplx<T>* z = fft_lds<T>(o, r, ny, lg, nq, tl);
+  // (closed y: the row pass reads the problem's rows only)
+  for (int w = threadIdx.x; w < nq * ny; w += blockDim.x) {
+    const int q = w >> lg, y = w & (ny - 1);
+    if (a.ey && y >= on) continue;
+    const int k1 = 2 * (u0 + q), k2 = k1 + 1;
+    const cplx<T> v = z[w];
+    Sr[(int64_t)k1 * ny + y] = v.x;
+    if (k2 < a.nk) Sr[(int64_t)k2 * ny + y] = -v.y;
   }
 }
 
@@ -420,6 +493,10 @@ __global__ __launch_bounds__(1024) void k_fft_cols(FftArgs a,
                                                       nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (bt) S = reinterpret_cast<cplx<T>*>(bt[blockIdx.y].S);
+  if (a.ex == 1) {
+    cols_real_pairs<T>(a, reinterpret_cast<T*>(S), tw, lamx, lamy);
+    return;
+  }
   const int ny = a.ny, lg = a.lgy;
   const int k0 = blockIdx.x * a.Q;
   const int nq = min(a.Q, a.nk - k0);
@@ -517,7 +594,13 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
   const int n2 = a.nk * 2 * P;
   for (int w = threadIdx.x; w < n2; w += blockDim.x) {
     const int ab = w & 1, p = (w >> 1) % P, k = (w >> 1) / P;
-    const cplx<T> X = S[(int64_t)k * a.ny + y0 + 2 * p + ab];
+    const int64_t at = (int64_t)k * a.ny + y0 + 2 * p + ab;
+    cplx<T> X;
+    if (a.ex == 1) {  // odd x: the imaginary part stored
+      X = mk<T>(T(0), reinterpret_cast<const T*>(S)[at]);
+    } else {
+      X = S[at];
+    }
     (ab == 0 ? b1 : b0)[p * nx + k] = X;
   }
   __syncthreads();
@@ -849,6 +932,7 @@ FftSolver* fft_create(int64_t ny, int64_t nx, double dx, double dy,
     a.lgx = ilog2(nx);
     a.lgy = ilog2(ny);
     a.nk = (int)(nx / 2 + 1);
+    a.ncu = a.ex == 1 ? (a.nk + 1) / 2 : a.nk;
     a.NXP = (int)(onx + 6);
     a.cx = dy / dx;
     a.cy = dx / dy;
@@ -926,7 +1010,7 @@ static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
   const cplx<T>* twx = d ? (const cplx<T>*)f->twx64 : (const cplx<T>*)f->twx32;
   const cplx<T>* twy = d ? (const cplx<T>*)f->twy64 : (const cplx<T>*)f->twy32;
   // closed y: rows 0 .. ony-1 only (the rest mirror them, see k_fft_rows_fwd)
-  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
   const double* bpin = ((xres || xres32) && a.singular) ? rs_in : nullptr;
   k_fft_rows_fwd<T, TB><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
                                             xres32);
@@ -1014,7 +1098,7 @@ void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStr
   FftArgs a = f->a;
   a.P = f->P32;
   a.Q = f->Q32;
-  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
   const dim3 grr(gr, nm), grc(gc, nm);
   k_fft_rows_fwd<float, float><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
       a, nullptr, 0, nullptr, f->twx32, nullptr, nullptr, nullptr, gr, nullptr, table);
@@ -1058,7 +1142,7 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
   a.Q = f64 ? f->Q64 : f->Q32;
   const size_t lr = f64 ? f->lds_r64 : f->lds_r32, lc = f64 ? f->lds_c64 : f->lds_c32;
   // closed y: rows 0 .. ony-1 only (the rest mirror them, see k_fft_rows_fwd)
-  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.nk + a.Q - 1) / a.Q;
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
   const double N = (double)a.nx * a.ny, NS = (double)a.nk * a.ny;
   const double w = f64 ? 8.0 : 4.0;  // real word
   hipEvent_t e0, e1;
