@@ -889,6 +889,8 @@ static void plan_blocks(const FftArgs& a, int& P, int& Q, size_t& lr, size_t& lc
                    (2 * P + 1) * a.nx * c > LDS_CAP))
     P >>= 1;
   Q = a.ny >= 1024 ? 1 : 1024 / a.ny;
+  // (odd x: a unit is a pair of columns; as many workgroups as unpaired)
+  if (a.ex == 1 && Q > 1) Q >>= 1;
   while (Q > 1 && (2 * Q + 1) * a.ny * c > LDS_CAP) Q >>= 1;
   lr = (2 * (size_t)P + 1) * a.nx * c;
   lc = (2 * (size_t)Q + 1) * a.ny * c;
