@@ -1518,6 +1518,7 @@ struct f2d_poisson {
   f2d::NeumannSolver* neu = nullptr;  // Neumann (cell) operator: Jacobi-PCG
   f2d::CapSolver* cap = nullptr;  // masked operator: spectral + capacitance
   int64_t fft_unmeasured = 0;     // FFT solves made without statistics
+  int refine_next = -1;           // FP32 FFT solves: refinements (0: none), -1 options
   std::vector<void*> allocs;
   double* reg(int r) { return part + (int64_t)r * region; }
 };
@@ -2433,7 +2434,9 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
       } else {
         const int nref = mode == F2D_POISSON_F32
                              ? 0
-                             : (h->opts.fft_refine > 0 ? h->opts.fft_refine : 1);
+                             : (h->refine_next >= 0
+                                    ? h->refine_next
+                                    : (h->opts.fft_refine > 0 ? h->opts.fft_refine : 1));
         fft_solve_f32_refine(h->fft, (const float*)b, gmode, (float*)x, nref,
                              h->opts.fft_warm != 0, rtol,
                              sync, c->stats, s);
@@ -2466,6 +2469,14 @@ int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
                           void* stream) {
   return solve_common(h, omega, x, G_FROM_C2V, rtol, maxit, iters, relres,
                       stream);
+}
+
+int f2d_poisson_set_refine(f2d_poisson* h, int nref) {
+  return guarded([&] {
+    F2D_REQUIRE(h, "handle is NULL");
+    F2D_REQUIRE(nref >= -1 && nref <= 16, "refinements must be in -1 .. 16");
+    h->refine_next = nref;
+  });
 }
 
 int f2d_poisson_method(f2d_poisson* h) {

@@ -223,6 +223,15 @@ class Poisson2D:
             _lib.call("f2d_poisson_solve_c2v", self._h, _lib.ptr(omega), _lib.ptr(x),
                       self.rtol, n, None, None, _lib.stream_ptr())
 
+    def set_refine(self, nref):
+        """FP64 refinements of the following direct FFT solves in FP32 (0:
+        plain FP32 solves, None: the handle's option); host state only, so
+        a captured graph keeps the choice each of its solves was made with."""
+        v = -1 if nref is None else int(nref)
+        if getattr(self, "_refine", -1) != v:
+            _lib.call("f2d_poisson_set_refine", self._h, v)
+            self._refine = v
+
     def _stats_host(self):
         buf = (ctypes.c_double * 8)()
         _lib.call("f2d_poisson_read_stats", self._h, buf, _lib.stream_ptr())

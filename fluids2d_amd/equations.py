@@ -68,8 +68,15 @@ class EulerPsi(_Base):
             op.transport_stage(m, p.compflux, s.omega, alt.omega, s.U, ds_prev, ds_out,
                                p.integrator, k, dt, dt_log)
         s, alt = s._replace(omega=alt.omega), alt._replace(omega=s.omega)
+        last = k == _lib.NSTAGES[p.integrator] - 1
+        if p.precision != "fp64" and getattr(p, "poisson_refine", "every") == "last":
+            # plain FP32 solves inside the step; the last one refines in FP64
+            # from the previous stage's solution.  (A host-checked solve
+            # refines too: the FP64 residual of a plain FP32 solution is
+            # that of x rounded to FP32, about 1e-3 at 1024^2.)
+            m.poisson_vertices.set_refine(None if (last or stats) else 0)
         m.poisson_vertices.solve_c2v(s.omega, s.psi, stats=stats)
-        if k == _lib.NSTAGES[p.integrator] - 1:
+        if last:
             op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
                         cfl=p.cfl, dtmax=p.dtmax)
         return s, alt

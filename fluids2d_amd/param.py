@@ -26,6 +26,11 @@ Fields of the MI355X path (after the reference's in the table):
                      otherwise), "mg", "fft" or "cap"
 * ``poisson_warm``   FP32 direct solves refine from the solution held in x
                      (the previous stage's streamfunction)
+* ``poisson_refine`` "last" (eulerpsi in FP32): only the last RK stage's
+                     direct FFT solve of each step refines in FP64, the
+                     others are plain FP32 solves (1.7e-6 from FP64 after
+                     1000 steps at 1024^2); "every": every stage refines
+                     (5.1e-7)
 * ``poisson_opts``   dict of multigrid schedule options, as picked by
                      ``fluids2d_amd.autotune``
 * ``tile``           LDS tile code of the fused transport stage (0 16x64,
@@ -92,6 +97,8 @@ SCHEMA = (
     ("poisson_nu",     2,            None,          "multigrid smoothing sweeps"),
     ("poisson_solver", "auto",       SOLVERS,       "Poisson method"),
     ("poisson_warm",   True,         None,          "FP32 direct solves from x"),
+    ("poisson_refine", "last",       ("every", "last"),
+                                                    "FP64 refinement of FP32 solves"),
     ("poisson_opts",   None,         None,          "multigrid schedule options"),
     ("tile",           0,            None,          "transport LDS tile code"),
     ("hipgraph",       True,         None,          "replay captured step graphs"),

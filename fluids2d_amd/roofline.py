@@ -62,7 +62,8 @@ def profile_dominant(model, reps=20):
     run on) and pick the dominant one: the largest mean launch time x
     launches per step.  Launch counts come from the solver's schedule (FFT:
     one solve per stage; in FP32 one FP64-residual refinement, from the
-    previous solution or after a cold FP32 solve; MG: the iterations of the
+    previous solution or after a cold FP32 solve, at every stage or only at
+    the last with plain FP32 solves before it; MG: the iterations of the
     last solves)."""
     pois = model.mesh.poisson_vertices
     prec = model.param.precision
@@ -70,6 +71,11 @@ def profile_dominant(model, reps=20):
     if pois.method == "fft":
         if prec == "fp64":
             per = {3: 3, 5: 3, 6: 3}
+        elif getattr(model.param, "poisson_warm", True) and \
+                getattr(model.param, "poisson_refine", "every") == "last":
+            # plain FP32 solves in stages 0 and 1; the last stage's refines
+            # once in FP64 from the previous stage's solution
+            per = {3: 2, 8: 1, 5: 3, 6: 2, 9: 1}
         elif getattr(model.param, "poisson_warm", True):
             # one FP64-residual refinement from the previous solution
             per = {8: 3, 5: 3, 9: 3}

@@ -452,6 +452,12 @@ int f2d_poisson_solve(f2d_poisson* h, const void* b, void* x, double rtol,
  * (`rtol` is not its acceptance test: a direct FP64 solve reaches eps times
  * the condition number, about 1e-11 at 1024^2). */
 int f2d_poisson_method(f2d_poisson* h);
+/* FP64 refinement passes of the handle's following direct FFT solves in
+ * F2D_POISSON_F32_REFINE mode: 0 makes them plain FP32 solves, -1 returns
+ * to f2d_poisson_opts.fft_refine (no counterpart in the reference, whose
+ * solves are FP64; the capacitance and multigrid solves ignore it).  A
+ * stepper may refine only the last RK stage's solve of each step. */
+int f2d_poisson_set_refine(f2d_poisson* h, int nref);
 /* Fused eulerpsi diagnostic solve (equations.py:80-82): b = c2v(omega)*mskv
  * computed on the fly from the cell field omega, then solve as above. */
 int f2d_poisson_solve_c2v(f2d_poisson* h, const void* omega, void* x,
