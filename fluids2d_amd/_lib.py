@@ -31,7 +31,7 @@ EXPORTS = [
     "f2d_perpgrad_cfl_noalloc", "f2d_sw_momentum", "f2d_sw_momentum_u", "f2d_sw_diag",
     "f2d_qg_pv", "f2d_qg_project", "f2d_transport_stage_u",
     "f2d_poisson_batch_create", "f2d_poisson_batch_solve", "f2d_poisson_batch_destroy",
-    "f2d_poisson_set_refine",
+    "f2d_poisson_set_refine", "f2d_nonfinite",
     "f2d_record_begin", "f2d_record_end", "f2d_recording_size", "f2d_recording_destroy",
     "f2d_launch_batch_create", "f2d_launch_batch_run", "f2d_launch_batch_destroy",
 ]
@@ -189,6 +189,7 @@ def lib():
         "f2d_poisson_batch_solve": (i32, [vp, vp]),
         "f2d_poisson_batch_destroy": (i32, [vp]),
         "f2d_poisson_set_refine": (i32, [vp, i32]),
+        "f2d_nonfinite": (i32, [i32, gp, vp, vp, vp, vp]),
         "f2d_record_begin": (i32, []),
         "f2d_record_end": (i32, [ctypes.POINTER(vp)]),
         "f2d_recording_size": (i32, [vp]),

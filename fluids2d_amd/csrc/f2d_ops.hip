@@ -84,6 +84,16 @@ __global__ void k_ke_finish(T* __restrict__ ke, const int8_t* __restrict__ msk,
   ke[k] = T((double)ke[k] * ((double)msk[k] * 0.5));
 }
 
+// flag[0] = 1 where a wet entry (msk NULL: any entry) is not finite; the
+// flag is only ever set here (the caller clears it)
+template <class T>
+__global__ void k_nonfinite(const T* __restrict__ a, const int8_t* __restrict__ msk,
+                            int64_t n, int32_t* __restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if ((msk == nullptr || msk[k]) && !isfinite((double)a[k])) flag[0] = 1;
+}
+
 // compute_pressure, operators.py:110-111: p = (g/area) * (h + hb)
 template <class T>
 __global__ void k_pressure(const T* __restrict__ h, T* __restrict__ p, double goa,
@@ -306,6 +316,17 @@ int f2d_ke_finish(int dtype, const f2d_grid* g, void* ke, const int8_t* msk,
     const int64_t n = (g->ny + 6) * (g->nx + 6);
     F2D_OPS_DISPATCH(dtype, k_ke_finish<T><<<grid1d(n, 256), 256, 0, S(stream)>>>(
                                 (T*)ke, msk, n));
+  });
+}
+
+int f2d_nonfinite(int dtype, const f2d_grid* g, const void* a, const int8_t* msk,
+                  int32_t* flag, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    F2D_REQUIRE(a && flag, "NULL argument");
+    const int64_t n = (g->ny + 6) * (g->nx + 6);
+    F2D_OPS_DISPATCH(dtype, k_nonfinite<T><<<grid1d(n, 256), 256, 0, S(stream)>>>(
+                                (const T*)a, msk, n, flag));
   });
 }
 

@@ -189,6 +189,19 @@ class RKIntegrator:
         p = self.param
         return not (p.dt > 0 or p.model == "rsw")
 
+    def nonfinite(self):
+        """Names of the prognostic fields of the device state with a
+        non-finite value on a wet cell (one flag per field, read on the host:
+        a synchronisation point)."""
+        names = [n for n in self.specs.prognostic]
+        flags = torch.zeros(len(names), dtype=torch.int32, device=self.d_dt.device)
+        m = self.mesh
+        for k, n in enumerate(names):
+            for a in leaves(getattr(self.dstate, n)):
+                _lib.call("f2d_nonfinite", _lib.dtype_code(a.dtype), m.grid_ref(), _lib.ptr(a),
+                          _lib.ptr(m.d_msk), _lib.ptr(flags[k:k + 1]), _lib.stream_ptr())
+        return [n for n, f in zip(names, flags.cpu().tolist()) if f]
+
     def _log_dt(self, dt_log):
         """The step's dt into dt_log by a kernel of its own (models whose
         first stage does not log it)."""

@@ -48,11 +48,14 @@ class Model:
             self.save_to_file()
             integ = self.integrator
             integ.begin(self.state)
+            self.nonfinite = []
             while (not self.time.finished) and (not self.stop):
                 integ.set_dt(self.time)
                 integ.device_step(self.time)
                 self.progress()
                 self._host_hooks()
+            if not self.nonfinite:
+                self._check_finite()
             integ.end(self.state)
             self.progress()
             toc = wallclock()
@@ -71,10 +74,25 @@ class Model:
                 d()
         if not (host or self.time.save_to_file):
             return
+        if self._check_finite():
+            return
         self.integrator.end(self.state)
         for d in host:
             d()
         self.save_to_file()
+
+    def _check_finite(self):
+        """At the loop's synchronisation points (history, host diagnostics,
+        the end): stop cleanly, as on ctrl-C, when a prognostic field holds
+        a non-finite value on a wet cell (no counterpart in the reference).
+        Returns True when it stops the run."""
+        bad = self.integrator.nonfinite()
+        if not bad:
+            return False
+        print(f"\n non-finite {', '.join(bad)} at ite={self.time.ite}, stopping", end="")
+        self.nonfinite = bad
+        self.stop = True
+        return True
 
     def finalize(self):
         for diag in self.diags:

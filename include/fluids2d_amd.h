@@ -244,6 +244,12 @@ int f2d_sharp(int dtype, const f2d_grid* g, const void* ux, const void* uy,
 /* compute_vorticity, operators.py:67-77 (slipcoef: float64 array) */
 int f2d_vorticity(int dtype, const f2d_grid* g, const void* ux, const void* uy,
                   const double* slipcoef, void* omega, void* stream);
+/* Failure detection (no counterpart in the reference, which has no NaN
+ * check; its loop stops only on SIGINT, model.py:33-45): sets *flag
+ * (int32, device) to 1 if an entry of the padded array `a` where msk is
+ * wet (msk NULL: any entry) is not finite.  Never clears the flag. */
+int f2d_nonfinite(int dtype, const f2d_grid* g, const void* a, const int8_t* msk,
+                  int32_t* flag, void* stream);
 /* the last statement of compute_kinetic_energy, operators.py:99-101:
  * ke *= msk * 0.5 (the inner products are f2d_innerproduct) */
 int f2d_ke_finish(int dtype, const f2d_grid* g, void* ke, const int8_t* msk,
