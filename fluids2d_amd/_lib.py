@@ -186,7 +186,7 @@ def lib():
                                            ctypes.POINTER(d), vp]),
         "f2d_poisson_batch_create": (i32, [ctypes.POINTER(vp), i32, ctypes.POINTER(vp),
                                            ctypes.POINTER(vp), ctypes.POINTER(vp)]),
-        "f2d_poisson_batch_solve": (i32, [vp, vp]),
+        "f2d_poisson_batch_solve": (i32, [vp, i32, vp]),
         "f2d_poisson_batch_destroy": (i32, [vp]),
         "f2d_poisson_set_refine": (i32, [vp, i32]),
         "f2d_nonfinite": (i32, [i32, gp, vp, vp, vp, vp]),

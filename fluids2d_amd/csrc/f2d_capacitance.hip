@@ -343,6 +343,20 @@ __global__ __launch_bounds__(256) void k_cap_symv_sum_b(const CapMember* __restr
   if (i < m.n) cap_symv_sum_at<float>(m.n, m.nt, m.part, m.bidx, m.rhs, i);
 }
 
+// plain FP32 solves: rhs = b at the unknowns; x = y at the unknowns
+__global__ __launch_bounds__(256) void k_cap_rhs_b(CapGeo g, const CapMember* __restrict__ t) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (int64_t)g.NYP * g.NXP) return;
+  const CapMember& m = t[blockIdx.y];
+  m.rhs[k] = m.mw[k] ? m.b[k] : 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_cap_out_b(int64_t n, const CapMember* __restrict__ t) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const CapMember& m = t[blockIdx.y];
+  if (k < n && m.mw[k]) m.x[k] = m.y[k];
+}
+
 // x64 = x0 + y at the unknowns (from as in k_cap_res_b)
 __global__ __launch_bounds__(256) void k_cap_acc_b(int64_t n, const CapMember* __restrict__ t,
                                                    int from) {
@@ -657,6 +671,16 @@ void cap_solve_f32(CapSolver* c, const float* b, int gmode, float* x, int nref, 
   const int64_t N = (int64_t)c->g.NYP * c->g.NXP;
   float* rhs = (float*)c->rhs;
   float* y = (float*)c->y;
+  if (nref == 0) {  // a plain FP32 solve, no FP64 refinement
+    k_cap_rhs<float, float><<<blocks(N), 256, 0, s>>>(c->g, b, gmode, c->mw, rhs);
+    fft_solve_f32_refine(c->fft, rhs, 0, y, 0, false, 0.0, false, nullptr, s);
+    gemv<float>(c, s);
+    fft_solve_f32_refine(c->fft, rhs, 0, y, 0, false, 0.0, false, nullptr, s);
+    k_cap_out<float><<<blocks(N), 256, 0, s>>>(N, c->mw, y, x);
+    if (measure && stats) measure_res<float, float>(c, b, gmode, x, 1, stats, s);
+    c->solved = true;
+    return;
+  }
   const bool from_x = warm && c->solved;
   const int passes = from_x ? (nref > 0 ? nref : 1) : (nref > 0 ? nref : 1) + 1;
   for (int p = 0; p < passes; p++) {
@@ -753,6 +777,17 @@ void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s) {
   const int nm = (int)bt->c.size();
   const CapGeo& g = bt->c[0]->g;
   const int64_t N = (int64_t)g.NYP * g.NXP;
+  if (nref == 0) {  // plain FP32 solves (cap_solve_f32 with nref 0)
+    const dim3 gp((unsigned)blocks(N), (unsigned)nm);
+    k_cap_rhs_b<<<gp, 256, 0, s>>>(g, bt->t);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
+    k_cap_symv_b<<<dim3((unsigned)bt->maxntl, (unsigned)nm), 256, 0, s>>>(bt->t);
+    k_cap_symv_sum_b<<<dim3(blocks(bt->maxn), (unsigned)nm), 256, 0, s>>>(bt->t);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
+    k_cap_out_b<<<gp, 256, 0, s>>>(N, bt->t);
+    for (CapSolver* m : bt->c) m->solved = true;
+    return;
+  }
   const bool from_x = warm && cap_batch_warm(bt);
   bool none = true;
   for (const CapSolver* m : bt->c) none = none && !m->solved;

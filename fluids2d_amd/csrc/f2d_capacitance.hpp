@@ -63,6 +63,7 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
 void cap_batch_destroy(CapBatch* bt);
 // whether every member has solved before (a warm batch needs all of them)
 bool cap_batch_warm(const CapBatch* bt);
+// nref 0: plain FP32 solves (no FP64 refinement)
 void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s);
 
 }  // namespace f2d

@@ -1518,7 +1518,7 @@ struct f2d_poisson {
   f2d::NeumannSolver* neu = nullptr;  // Neumann (cell) operator: Jacobi-PCG
   f2d::CapSolver* cap = nullptr;  // masked operator: spectral + capacitance
   int64_t fft_unmeasured = 0;     // FFT solves made without statistics
-  int refine_next = -1;           // FP32 FFT solves: refinements (0: none), -1 options
+  int refine_next = -1;  // FP32 direct solves: refinements (0: none), -1 options
   std::vector<void*> allocs;
   double* reg(int r) { return part + (int64_t)r * region; }
 };
@@ -2420,7 +2420,9 @@ static int solve_common(f2d_poisson* h, const void* b, void* x, int gmode,
         cap_solve_f64(h->cap, (const double*)b, gmode, (double*)x, sync, c->stats, s);
         fill_x<double>(h, (double*)x, s);
       } else {
-        const int nref = h->opts.fft_refine > 0 ? h->opts.fft_refine : 1;
+        const int nref = h->refine_next >= 0
+                             ? h->refine_next
+                             : (h->opts.fft_refine > 0 ? h->opts.fft_refine : 1);
         cap_solve_f32(h->cap, (const float*)b, gmode, (float*)x, nref,
                       h->opts.fft_warm != 0, sync, c->stats, s);
         fill_x<float>(h, (float*)x, s);
@@ -2555,11 +2557,12 @@ int f2d_poisson_batch_create(f2d_poisson* const* h, int32_t n, const void* const
   });
 }
 
-int f2d_poisson_batch_solve(f2d_poisson_batch* bt, void* stream) {
+int f2d_poisson_batch_solve(f2d_poisson_batch* bt, int nref, void* stream) {
   return guarded([&] {
     F2D_REQUIRE(bt, "batch is NULL");
+    F2D_REQUIRE(nref >= -1 && nref <= 16, "refinements must be in -1 .. 16");
     hipStream_t s = S(stream);
-    f2d::cap_batch_solve(bt->cap, bt->nref, bt->warm, s);
+    f2d::cap_batch_solve(bt->cap, nref >= 0 ? nref : bt->nref, bt->warm, s);
     for (size_t i = 0; i < bt->h.size(); i++) {
       f2d::fill_x<float>(bt->h[i], (float*)bt->x[i], s);
       bt->h[i]->fft_unmeasured++;

@@ -224,9 +224,10 @@ class Poisson2D:
                       self.rtol, n, None, None, _lib.stream_ptr())
 
     def set_refine(self, nref):
-        """FP64 refinements of the following direct FFT solves in FP32 (0:
-        plain FP32 solves, None: the handle's option); host state only, so
-        a captured graph keeps the choice each of its solves was made with."""
+        """FP64 refinements of the following direct (FFT or capacitance)
+        solves in FP32 (0: plain FP32 solves, None: the handle's option);
+        host state only, so a captured graph keeps the choice each of its
+        solves was made with."""
         v = -1 if nref is None else int(nref)
         if getattr(self, "_refine", -1) != v:
             _lib.call("f2d_poisson_set_refine", self._h, v)
@@ -304,10 +305,13 @@ class PoissonBatch:
         return all(a.data_ptr() == b.data_ptr() for a, b in zip(self.arrays[0], bs)) and \
             all(a.data_ptr() == b.data_ptr() for a, b in zip(self.arrays[1], xs))
 
-    def solve(self):
+    def solve(self, nref=None):
+        """nref: FP64 refinements (0: plain FP32 solves, None: the solvers'
+        option), as Poisson2D.set_refine."""
         for s in self.solvers:
             s.calls += 1
-        _lib.call("f2d_poisson_batch_solve", self._h, _lib.stream_ptr())
+        _lib.call("f2d_poisson_batch_solve", self._h, -1 if nref is None else int(nref),
+                  _lib.stream_ptr())
 
     def __del__(self):
         h = getattr(self, "_h", None)

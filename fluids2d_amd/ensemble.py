@@ -130,7 +130,7 @@ class Ensemble:
         p = m.param
         return (type(m.integrator.model).__name__, PoissonBatch.key(m.mesh.qg_helmholtz),
                 p.integrator, p.compflux, p.vortexforce, p.innerproduct, p.f0, p.g, p.H,
-                m.mesh.grid.tile)
+                m.mesh.grid.tile, getattr(p, "poisson_refine", "every"))
 
     def groups(self):
         """The member groups: a list of (member indices, PoissonBatch), one
@@ -194,10 +194,11 @@ class Ensemble:
             if k < nst:
                 plan[k][0]()      # before stage k's solve
 
+        model = self.models[members[0]].integrator.model
         for k in range(nst + 1):
             self._launch((g, k, dts), [lambda p=p, k=k: part(p, k) for p in plans])
             if k < nst:
-                solve.solve()
+                solve.solve(nref=model.refine_for(k))
         for p in plans:
             if p[-1][2] is not None:
                 p[-1][2]()  # the member's CFL step

@@ -129,6 +129,8 @@ class _SW(_Base):
         else:
             ds = ds_out
         self._stats = stats
+        if hasattr(self, "refine_for"):  # the fused stage's solve schedule
+            self.mesh.qg_helmholtz.set_refine(self.refine_for(k, stats))
         try:
             self.rhs(s, ds)
         finally:
@@ -245,16 +247,29 @@ class QGRSW(_SW):
 
     def _stage_fused(self, s, alt, ds_prev, ds_out, k, dt, stats, dt_log=None):
         self._stage_pre(s, k, dt, dt_log)
-        self.mesh.qg_helmholtz.solve(s.pv, s.psi, stats=stats)
+        solver = self.mesh.qg_helmholtz
+        solver.set_refine(self.refine_for(k, stats))
+        solver.solve(s.pv, s.psi, stats=stats)
         self._stage_post(s, ds_prev, ds_out, k, dt)
         return s, alt
+
+    def refine_for(self, k, stats=False):
+        """FP64 refinements of stage k's Helmholtz solve (Poisson2D.set_refine):
+        in FP32 with ``poisson_refine`` "last", plain FP32 solves before the
+        last stage (a host-checked solve always refines); None: the
+        solver's option."""
+        p = self.param
+        if p.precision == "fp64" or getattr(p, "poisson_refine", "every") != "last":
+            return None
+        return None if (stats or k == _lib.NSTAGES[p.integrator] - 1) else 0
 
     def stage_split(self, s, ds_prev, ds_out, k, dt, dt_next, dt_log=None):
         """stage() cut at its Helmholtz solve, for callers that batch the
         solves of many models: (pre, post, cfl) callables (cfl None unless
         dt_next is given); pre, the solve of s.pv into s.psi by
-        mesh.qg_helmholtz, post, then cfl run the stage.  pre and post
-        launch only kernels that f2d_record_begin can record."""
+        mesh.qg_helmholtz (with ``refine_for(k)`` refinements), post, then
+        cfl run the stage.  pre and post launch only kernels that
+        f2d_record_begin can record."""
         cfl = None
         if dt_next is not None:
             def cfl():

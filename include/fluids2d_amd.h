@@ -458,11 +458,11 @@ int f2d_poisson_solve(f2d_poisson* h, const void* b, void* x, double rtol,
  * (`rtol` is not its acceptance test: a direct FP64 solve reaches eps times
  * the condition number, about 1e-11 at 1024^2). */
 int f2d_poisson_method(f2d_poisson* h);
-/* FP64 refinement passes of the handle's following direct FFT solves in
- * F2D_POISSON_F32_REFINE mode: 0 makes them plain FP32 solves, -1 returns
- * to f2d_poisson_opts.fft_refine (no counterpart in the reference, whose
- * solves are FP64; the capacitance and multigrid solves ignore it).  A
- * stepper may refine only the last RK stage's solve of each step. */
+/* FP64 refinement passes of the handle's following direct (FFT or
+ * capacitance) solves in the FP32 modes: 0 makes them plain FP32 solves, -1
+ * returns to f2d_poisson_opts.fft_refine (no counterpart in the reference,
+ * whose solves are FP64; multigrid ignores it).  A stepper may refine only
+ * the last RK stage's solve of each step. */
 int f2d_poisson_set_refine(f2d_poisson* h, int nref);
 /* Fused eulerpsi diagnostic solve (equations.py:80-82): b = c2v(omega)*mskv
  * computed on the fly from the cell field omega, then solve as above. */
@@ -516,8 +516,9 @@ typedef struct f2d_poisson_batch f2d_poisson_batch;
 int f2d_poisson_batch_create(f2d_poisson* const* h, int32_t n, const void* const* b,
                              void* const* x, f2d_poisson_batch** out);
 /* The solves of every member on `stream` (no counterpart in the reference:
- * Poisson2D.solve per model, elliptic.py:80-87). */
-int f2d_poisson_batch_solve(f2d_poisson_batch* bt, void* stream);
+ * Poisson2D.solve per model, elliptic.py:80-87), with `nref` FP64
+ * refinements (0: plain FP32 solves; -1: the handles' option). */
+int f2d_poisson_batch_solve(f2d_poisson_batch* bt, int nref, void* stream);
 /* Frees the batch's device tables (no counterpart in the reference). */
 int f2d_poisson_batch_destroy(f2d_poisson_batch* bt);
 

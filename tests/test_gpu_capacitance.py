@@ -127,8 +127,17 @@ def test_capacitance_warm_fp32(dev, L):
         errs.append(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
         print(f"\nsolve {k}: passes={it} relres={rr:.2e} err={errs[-1]:.2e}")
         assert it == (2 if k == 0 else 1)
-    L.call("f2d_poisson_destroy", h)
     assert max(errs) <= 1e-7
+    # a plain FP32 solve (no refinement): FP32 accuracy of the solution
+    L.call("f2d_poisson_set_refine", h, 0)
+    b = b0 + 0.1 * rng.standard_normal(m.shape) * mesh.mskv
+    x_ref = np.zeros(m.shape)
+    ref.solve(b, x_ref)
+    it, rr = _solve(L, h, torch.from_numpy(b).to(dev, torch.float32), dx, False)
+    err = np.linalg.norm(dx.double().cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+    print(f"\nplain FP32 solve: passes={it} relres={rr:.2e} err={err:.2e}")
+    assert it == 1 and err <= 1e-5
+    L.call("f2d_poisson_destroy", h)
 
 
 def test_capacitance_choice(dev, L):
@@ -196,20 +205,25 @@ def test_capacitance_batch(dev, L):
         return h
 
     batches = [batch(ms) for ms in groups]
-    for rnd in range(3):
+    # solves 0-2 with the handles' refinement (cold, then warm), 3-4 plain FP32
+    # (nref 0: the batch argument, the solo handles' f2d_poisson_set_refine),
+    # 5 warm again from a plain solution
+    for rnd in range(6):
+        nref = 0 if rnd in (3, 4) else -1
         for ms, bt in zip(groups, batches):
             for m in ms:
                 m["b"].copy_(torch.from_numpy(rng.standard_normal(m["mskv"].shape) *
                                               m["mskv"]).to(dev, torch.float32))
+                L.call("f2d_poisson_set_refine", m["hs"], nref)
                 L.call("f2d_poisson_solve", m["hs"], L.ptr(m["b"]), L.ptr(m["xs"]), 1e-9, 10,
                        None, None, L.stream_ptr())
-            L.call("f2d_poisson_batch_solve", bt, L.stream_ptr())
+            L.call("f2d_poisson_batch_solve", bt, nref, L.stream_ptr())
         torch.cuda.synchronize()
         for g, ms in enumerate(groups):
             for i, m in enumerate(ms):
                 assert torch.equal(m["xb"], m["xs"]), (rnd, g, i)
                 assert float(m["xb"].abs().max()) > 0
-    print(f"\nbatched capacitance: {[len(ms) for ms in groups]} members, 3 solves bit-equal")
+    print(f"\nbatched capacitance: {[len(ms) for ms in groups]} members, 6 solves bit-equal")
 
     # members warm and cold together: each member's own solve
     ms = groups[0]
@@ -221,7 +235,7 @@ def test_capacitance_batch(dev, L):
     L.call("f2d_poisson_solve", fresh[1]["hs"], L.ptr(fresh[1]["b"]), L.ptr(fresh[1]["xs"]),
            1e-9, 10, None, None, L.stream_ptr())
     bt = batch(fresh)
-    L.call("f2d_poisson_batch_solve", bt, L.stream_ptr())
+    L.call("f2d_poisson_batch_solve", bt, -1, L.stream_ptr())
     for f in fresh:
         L.call("f2d_poisson_solve", f["hs"], L.ptr(f["b"]), L.ptr(f["xs"]), 1e-9, 10,
                None, None, L.stream_ptr())

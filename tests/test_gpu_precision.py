@@ -10,8 +10,11 @@ the two workloads that run in that precision, each with its error bar:
   geos_adj.py:37-49) at 64^2 and 96x128 for 200 steps: relative L2 of u and
   h <= 1e-5.
 
-The oracle runs the reference's algorithm in FP64 (SuperLU solves); the
-device runs FP32 arithmetic with FP64 refinement of every solve.
+The oracle runs the reference's algorithm in FP64 (SuperLU solves).  The
+device runs FP32 arithmetic, stepped two ways: ``Model.run`` (host-checked
+solves, each refined in FP64) and ``device_steps`` (graph-replayed steps
+whose solves are plain FP32 in the first RK stages and refined once in FP64
+at the last, ``Param.poisson_refine = "last"``).
 """
 import os
 import sys
@@ -46,7 +49,23 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_euler_periodic_fp32_refined_1000_steps(dev):
+def _steps(m, nsteps, stepping):
+    """nsteps steps by Model.run or by device_steps (then the host state and
+    the time pushed forward by the steps' dt, as Model.run leaves them)."""
+    if stepping == "run":
+        m.run()
+        return
+    integ = m.integrator
+    integ.begin(m.state)
+    dts = integ.device_steps(nsteps).cpu().numpy()
+    integ.end(m.state)
+    for v in dts:
+        m.time.dt = float(v)
+        m.time.pushforward()
+
+
+@pytest.mark.parametrize("stepping", ["run", "device_steps"])
+def test_euler_periodic_fp32_refined_1000_steps(dev, stepping):
     import fluids2d_amd as f2d
     from bench import initial_omega
     n, nsteps = 128, 1000
@@ -64,8 +83,9 @@ def test_euler_periodic_fp32_refined_1000_steps(dev):
     m = f2d.Model(p)
     m.state.omega[...] = w0
     m.integrator.diag(m.state)
-    print(f"\n128^2 eulerpsi, dt {dt:.4e}: device FP32 + refinement, oracle FP64", flush=True)
-    m.run()
+    print(f"\n128^2 eulerpsi, dt {dt:.4e}: device FP32 + refinement ({stepping}), "
+          "oracle FP64", flush=True)
+    _steps(m, nsteps, stepping)
     ref.run(nsteps)
     assert m.time.ite == ref.time.ite == nsteps
     e = _rel(m.state.omega[3:-3, 3:-3], ref.state.omega[3:-3, 3:-3])
@@ -73,8 +93,9 @@ def test_euler_periodic_fp32_refined_1000_steps(dev):
     assert e <= 1e-5
 
 
+@pytest.mark.parametrize("stepping", ["run", "device_steps"])
 @pytest.mark.parametrize("n,ny,seed", [(64, 64, 11), (96, 128, 12)])
-def test_qgrsw_basin_fp32_refined(dev, n, ny, seed):
+def test_qgrsw_basin_fp32_refined(dev, n, ny, seed, stepping):
     import fluids2d_amd as f2d
     from bench import basin_mask
     nsteps = 200
@@ -106,8 +127,9 @@ def test_qgrsw_basin_fp32_refined(dev, n, ny, seed):
     s.u.y[...] = rs.u.y
     s.h[...] = rs.h
     m.integrator.diag(s)
-    print(f"\nqgrsw {n}x{ny} basin: device FP32 + refinement, oracle FP64", flush=True)
-    m.run()
+    print(f"\nqgrsw {n}x{ny} basin: device FP32 + refinement ({stepping}), oracle FP64",
+          flush=True)
+    _steps(m, nsteps, stepping)
     with np.errstate(divide="ignore", invalid="ignore"):
         ref.run(nsteps)
     assert m.time.ite == ref.time.ite == nsteps
