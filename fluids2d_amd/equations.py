@@ -69,11 +69,15 @@ class EulerPsi(_Base):
                                p.integrator, k, dt, dt_log)
         s, alt = s._replace(omega=alt.omega), alt._replace(omega=s.omega)
         last = k == _lib.NSTAGES[p.integrator] - 1
-        if p.precision != "fp64" and getattr(p, "poisson_refine", "every") == "last":
+        if p.precision != "fp64" and getattr(p, "poisson_refine", "every") == "last" and \
+                m.poisson_vertices.method == "fft":
             # plain FP32 solves inside the step; the last one refines in FP64
             # from the previous stage's solution.  (A host-checked solve
             # refines too: the FP64 residual of a plain FP32 solution is
-            # that of x rounded to FP32, about 1e-3 at 1024^2.)
+            # that of x rounded to FP32, about 1e-3 at 1024^2.)  The masked
+            # (capacitance) solve refines at every stage: its plain FP32
+            # solutions cost the vorticity 5e-6 over 500 steps at 128^2,
+            # against 6e-7.
             m.poisson_vertices.set_refine(None if (last or stats) else 0)
         m.poisson_vertices.solve_c2v(s.omega, s.psi, stats=stats)
         if last:

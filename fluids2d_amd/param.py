@@ -26,11 +26,12 @@ Fields of the MI355X path (after the reference's in the table):
                      otherwise), "mg", "fft" or "cap"
 * ``poisson_warm``   FP32 direct solves refine from the solution held in x
                      (the previous stage's streamfunction)
-* ``poisson_refine`` "last" (eulerpsi in FP32): only the last RK stage's
-                     direct FFT solve of each step refines in FP64, the
-                     others are plain FP32 solves (1.7e-6 from FP64 after
-                     1000 steps at 1024^2); "every": every stage refines
-                     (5.1e-7)
+* ``poisson_refine`` "last" (FP32): only the last RK stage's direct solve of
+                     each step refines in FP64, the others are plain FP32
+                     solves -- eulerpsi's FFT solves (1.7e-6 from FP64
+                     after 1000 steps at 1024^2, against 5.1e-7) and
+                     qgrsw's capacitance solves; "every": every stage
+                     refines.  Host-checked solves always refine
 * ``poisson_opts``   dict of multigrid schedule options, as picked by
                      ``fluids2d_amd.autotune``
 * ``tile``           LDS tile code of the fused transport stage (0 16x64,
