@@ -1859,6 +1859,15 @@ static void launch_up(const UpArgs<T>& a, int nu, bool fine, bool coarse,
   else launch_up_nu<T, 2>(a, fine, coarse, fc, s);
 }
 
+// tail workgroup size: opts.tail_threads (256, 512 or 1024; default 1024)
+template <class T>
+static void launch_tail(f2d_poisson* h, const TailArgs<T>& ta, size_t lds,
+                        hipStream_t s) {
+  int tb = h->opts.tail_threads;
+  if (tb != 256 && tb != 512) tb = 1024;
+  k_tail<T><<<1, tb, lds, s>>>(ta);
+}
+
 static int nb0(const LevelDev& F) {
   dim3 g = tiles(F, FC0);
   return (int)(g.x * g.y);
@@ -1961,15 +1970,6 @@ static void pcg_iteration(f2d_poisson* h, Pcg<T>& P, int k, T* x,
     a.singular = h->singular;
     k_pupdate<T><<<tiles(F, FC0), BS, 0, s>>>(a);
   }
-}
-
-// tail workgroup size: opts.tail_threads (256, 512 or 1024; default 1024)
-template <class T>
-static void launch_tail(f2d_poisson* h, const TailArgs<T>& ta, size_t lds,
-                        hipStream_t s) {
-  int tb = h->opts.tail_threads;
-  if (tb != 256 && tb != 512) tb = 1024;
-  k_tail<T><<<1, tb, lds, s>>>(ta);
 }
 
 template <class T>
