@@ -343,20 +343,6 @@ __global__ __launch_bounds__(256) void k_cap_symv_sum_b(const CapMember* __restr
   if (i < m.n) cap_symv_sum_at<float>(m.n, m.nt, m.part, m.bidx, m.rhs, i);
 }
 
-// plain FP32 solves: rhs = b at the unknowns; x = y at the unknowns
-__global__ __launch_bounds__(256) void k_cap_rhs_b(CapGeo g, const CapMember* __restrict__ t) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= (int64_t)g.NYP * g.NXP) return;
-  const CapMember& m = t[blockIdx.y];
-  m.rhs[k] = m.mw[k] ? m.b[k] : 0.f;
-}
-
-__global__ __launch_bounds__(256) void k_cap_out_b(int64_t n, const CapMember* __restrict__ t) {
-  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const CapMember& m = t[blockIdx.y];
-  if (k < n && m.mw[k]) m.x[k] = m.y[k];
-}
-
 // x64 = x0 + y at the unknowns (from as in k_cap_res_b)
 __global__ __launch_bounds__(256) void k_cap_acc_b(int64_t n, const CapMember* __restrict__ t,
                                                    int from) {
@@ -672,11 +658,15 @@ void cap_solve_f32(CapSolver* c, const float* b, int gmode, float* x, int nref, 
   float* rhs = (float*)c->rhs;
   float* y = (float*)c->y;
   if (nref == 0) {  // a plain FP32 solve, no FP64 refinement
-    k_cap_rhs<float, float><<<blocks(N), 256, 0, s>>>(c->g, b, gmode, c->mw, rhs);
-    fft_solve_f32_refine(c->fft, rhs, 0, y, 0, false, 0.0, false, nullptr, s);
+    // the first spectral solve reads b at W and leaves b_W in rhs; the
+    // second writes its result to x at W
+    FftMask m1, m2;
+    m1.in = c->mw;
+    m1.out = rhs;
+    m2.res = c->mw;
+    fft_solve_f32_masked(c->fft, b, gmode, y, m1, s);
     gemv<float>(c, s);
-    fft_solve_f32_refine(c->fft, rhs, 0, y, 0, false, 0.0, false, nullptr, s);
-    k_cap_out<float><<<blocks(N), 256, 0, s>>>(N, c->mw, y, x);
+    fft_solve_f32_masked(c->fft, rhs, 0, x, m2, s);
     if (measure && stats) measure_res<float, float>(c, b, gmode, x, 1, stats, s);
     c->solved = true;
     return;
@@ -718,6 +708,8 @@ struct CapBatch {
   std::vector<float*> x;
   CapMember* t = nullptr;      // device member table
   FftBatchPtrs* ft = nullptr;  // device: rhs -> y through each member's spectrum
+  FftBatchPtrs* fa = nullptr;  // plain solves: b (masked, kept in rhs) -> y
+  FftBatchPtrs* fb = nullptr;  // plain solves: rhs -> x at the unknowns
   int maxntl = 0, maxn = 0;
 };
 
@@ -732,7 +724,7 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
   CapBatch* bt = new CapBatch();
   try {
     std::vector<CapMember> t(nm);
-    std::vector<FftBatchPtrs> ft(nm);
+    std::vector<FftBatchPtrs> ft(nm), fa(nm), fb(nm);
     for (int i = 0; i < nm; i++) {
       CapSolver* m = c[i];
       F2D_REQUIRE(m && !m->f64, "batched capacitance solves are FP32 solvers");
@@ -744,14 +736,23 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
       bt->x.push_back(x[i]);
       t[i] = CapMember{b[i], x[i], m->mw, (float*)m->rhs, (float*)m->y, m->x64,
                        (const float*)m->M, m->bidx, m->tpart, m->n, m->nt, m->ntl, 0};
-      ft[i] = FftBatchPtrs{m->rhs, fft_spectrum32(m->fft), m->y};
+      ft[i] = FftBatchPtrs{m->rhs, fft_spectrum32(m->fft), m->y, FftMask()};
+      fa[i] = FftBatchPtrs{b[i], fft_spectrum32(m->fft), m->y, FftMask()};
+      fa[i].m.in = m->mw;
+      fa[i].m.out = m->rhs;
+      fb[i] = FftBatchPtrs{m->rhs, fft_spectrum32(m->fft), x[i], FftMask()};
+      fb[i].m.res = m->mw;
       if (m->ntl > bt->maxntl) bt->maxntl = m->ntl;
       if (m->n > bt->maxn) bt->maxn = m->n;
     }
     F2D_HIP(hipMalloc(&bt->t, sizeof(CapMember) * nm));
     F2D_HIP(hipMalloc(&bt->ft, sizeof(FftBatchPtrs) * nm));
+    F2D_HIP(hipMalloc(&bt->fa, sizeof(FftBatchPtrs) * nm));
+    F2D_HIP(hipMalloc(&bt->fb, sizeof(FftBatchPtrs) * nm));
     F2D_HIP(hipMemcpy(bt->t, t.data(), sizeof(CapMember) * nm, hipMemcpyHostToDevice));
     F2D_HIP(hipMemcpy(bt->ft, ft.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
+    F2D_HIP(hipMemcpy(bt->fa, fa.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
+    F2D_HIP(hipMemcpy(bt->fb, fb.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
   } catch (...) {
     cap_batch_destroy(bt);
     throw;
@@ -764,6 +765,8 @@ void cap_batch_destroy(CapBatch* bt) {
   (void)hipDeviceSynchronize();
   if (bt->t) (void)hipFree(bt->t);
   if (bt->ft) (void)hipFree(bt->ft);
+  if (bt->fa) (void)hipFree(bt->fa);
+  if (bt->fb) (void)hipFree(bt->fb);
   delete bt;
 }
 
@@ -778,13 +781,10 @@ void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s) {
   const CapGeo& g = bt->c[0]->g;
   const int64_t N = (int64_t)g.NYP * g.NXP;
   if (nref == 0) {  // plain FP32 solves (cap_solve_f32 with nref 0)
-    const dim3 gp((unsigned)blocks(N), (unsigned)nm);
-    k_cap_rhs_b<<<gp, 256, 0, s>>>(g, bt->t);
-    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->fa, s);
     k_cap_symv_b<<<dim3((unsigned)bt->maxntl, (unsigned)nm), 256, 0, s>>>(bt->t);
     k_cap_symv_sum_b<<<dim3(blocks(bt->maxn), (unsigned)nm), 256, 0, s>>>(bt->t);
-    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->ft, s);
-    k_cap_out_b<<<gp, 256, 0, s>>>(N, bt->t);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->fb, s);
     for (CapSolver* m : bt->c) m->solved = true;
     return;
   }

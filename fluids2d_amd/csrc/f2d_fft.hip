@@ -307,13 +307,15 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
                                                       const double* __restrict__ bpin,
                                                       int nbp,
                                                       const float* __restrict__ x32,
+                                                      FftMask fm = FftMask(),
                                                       const FftBatchPtrs* __restrict__ bt =
                                                           nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double sh[256];
-  if (bt) {  // a batch: this member's b and spectrum
+  if (bt) {  // a batch: this member's b, spectrum and masks
     b = reinterpret_cast<const TB*>(bt[blockIdx.y].b);
     S = reinterpret_cast<cplx<T>*>(bt[blockIdx.y].S);
+    fm = bt[blockIdx.y].m;
   }
   const int nx = a.nx, P = a.P, lg = a.lgx;
   cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem);
@@ -345,7 +347,11 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
       double v = 0.0;
       if (sg) {
         const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
-        const double g = gval(b, k, a.NXP, gmode);
+        double g = gval(b, k, a.NXP, gmode);
+        if (fm.in) {  // b at the masked unknowns only, kept for a later solve
+          if (!fm.in[k]) g = 0.0;
+          reinterpret_cast<TB*>(fm.out)[k] = TB(g);
+        }
         // sums of b over the problem's points (the singular even case)
         if (ext_unknown(a, y, n)) sum += g;
         v = g;
@@ -380,6 +386,12 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
       } else {
         va = gval(b, ka, a.NXP, gmode);
         vb = gval(b, ka + a.NXP, a.NXP, gmode);
+        if (fm.in) {
+          if (!fm.in[ka]) va = 0.0;
+          if (!fm.in[ka + a.NXP]) vb = 0.0;
+          reinterpret_cast<TB*>(fm.out)[ka] = TB(va);
+          reinterpret_cast<TB*>(fm.out)[ka + a.NXP] = TB(vb);
+        }
         sum += va + vb;
       }
       b0[w] = mk<T>(T(va), T(vb));
@@ -574,12 +586,14 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
                                                       const double* __restrict__ rs_src =
                                                           nullptr,
                                                       double* __restrict__ rs_dst = nullptr,
+                                                      FftMask fm = FftMask(),
                                                       const FftBatchPtrs* __restrict__ bt =
                                                           nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
-  if (bt) {  // a batch: this member's spectrum and solution
+  if (bt) {  // a batch: this member's spectrum, solution and mask
     S = reinterpret_cast<const cplx<T>*>(bt[blockIdx.y].S);
     x = reinterpret_cast<TX*>(bt[blockIdx.y].x);
+    fm = bt[blockIdx.y].m;
   }
   // the row sums of this call's b become the next call's (one per workgroup;
   // the forward row pass has the same grid)
@@ -635,6 +649,8 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
           const double nv = x64[k] + val;
           put_ext<double>(x64, y, n, a, nv);
           put_ext<TX>(x, y, n, a, TX(nv));
+        } else if (MODE == 0 && fm.res) {
+          if (fm.res[k]) x[k] = TX(val);  // the masked unknowns only
         } else {
           put_ext<TX>(x, y, n, a, TX(val));
           if (MODE == 1) put_ext<double>(x64, y, n, a, val);
@@ -658,6 +674,10 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
       put_images<double>(x64, ya + 1, n, a, nb);
       put_images<TX>(x, ya, n, a, TX(na));
       put_images<TX>(x, ya + 1, n, a, TX(nb));
+    } else if (MODE == 0 && fm.res) {
+      const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
+      if (fm.res[ka]) x[ka] = TX(va);  // the masked unknowns only
+      if (fm.res[ka + a.NXP]) x[ka + a.NXP] = TX(vb);
     } else {
       put_images<TX>(x, ya, n, a, TX(va));
       put_images<TX>(x, ya + 1, n, a, TX(vb));
@@ -1103,11 +1123,33 @@ void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStr
   const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
   const dim3 grr(gr, nm), grc(gc, nm);
   k_fft_rows_fwd<float, float><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
-      a, nullptr, 0, nullptr, f->twx32, nullptr, nullptr, nullptr, gr, nullptr, table);
+      a, nullptr, 0, nullptr, f->twx32, nullptr, nullptr, nullptr, gr, nullptr, FftMask(),
+      table);
   k_fft_cols<float><<<grc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(a, nullptr, f->twy32,
                                                                      f->lamx, f->lamy, table);
   k_fft_rows_inv<float, float, 0><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
-      a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, table);
+      a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, FftMask(), table);
+}
+
+void fft_solve_f32_masked(FftSolver* f, const float* b, int gmode, float* x, const FftMask& m,
+                          hipStream_t s) {
+  F2D_REQUIRE(f->S32, "FFT solver built without FP32 buffers");
+  F2D_REQUIRE(!m.in || m.out, "a masked right-hand side needs its output array");
+  FftArgs a = f->a;
+  a.P = f->P32;
+  a.Q = f->Q32;
+  const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
+  FftMask mf = m, mi = m;
+  mf.res = nullptr;
+  mi.in = nullptr;
+  mi.out = nullptr;
+  k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+      a, b, gmode, (float2*)f->S32, (const float2*)f->twx32, nullptr, nullptr, nullptr, gr,
+      nullptr, mf);
+  k_fft_cols<float><<<gc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(
+      a, (float2*)f->S32, (const float2*)f->twy32, f->lamx, f->lamy);
+  k_fft_rows_inv<float, float, 0><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+      a, (const float2*)f->S32, (const float2*)f->twx32, x, nullptr, nullptr, nullptr, mi);
 }
 
 // whether two plans solve the same operator (same args and eigenvalues)

@@ -59,12 +59,25 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
 // Batched FP32 spectral solves of same-size members (blockIdx.y = member):
 // each member's right-hand side b (FP32, padded, gmode 0), spectrum S and
 // solution x; `f` is one member's plan (fft_same_plan holds for all).
+// Masks of a spectral solve inside the capacitance solve (all optional):
+// the forward row pass reads b only where in[k] (0 elsewhere) and writes
+// that masked b to out (b's type); the inverse row pass (plain result)
+// writes x only where res[k], without periodic halo images.
+struct FftMask {
+  const int8_t* in = nullptr;
+  void* out = nullptr;
+  const int8_t* res = nullptr;
+};
 struct FftBatchPtrs {
   const void* b;
   void* S;
   void* x;
+  FftMask m;
 };
 void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s);
+// one plain FP32 spectral solve with masks (x = the result, gmode as b's)
+void fft_solve_f32_masked(FftSolver* f, const float* b, int gmode, float* x, const FftMask& m,
+                          hipStream_t s);
 bool fft_same_plan(const FftSolver* f, const FftSolver* g);
 float2* fft_spectrum32(FftSolver* f);
 
