@@ -672,6 +672,21 @@ void cap_solve_f32(CapSolver* c, const float* b, int gmode, float* x, int nref, 
     return;
   }
   const bool from_x = warm && c->solved;
+  if (from_x && nref == 1 && !(measure && stats)) {
+    // one refinement from the caller's x, fused into the spectral passes:
+    // the first forward row pass forms the FP64 residual of x at W (kept in
+    // rhs), the second inverse row pass adds the correction to x at W
+    FftMask m1, m2;
+    m1.in = c->mw;
+    m1.out = rhs;
+    m1.xr = x;
+    m2.res = c->mw;
+    fft_solve_f32_masked(c->fft, b, gmode, y, m1, s);
+    gemv<float>(c, s);
+    fft_solve_f32_masked(c->fft, rhs, 0, x, m2, s, true);
+    c->solved = true;
+    return;
+  }
   const int passes = from_x ? (nref > 0 ? nref : 1) : (nref > 0 ? nref : 1) + 1;
   for (int p = 0; p < passes; p++) {
     // residual of the current iterate: the caller's x (warm, first pass),
@@ -710,6 +725,7 @@ struct CapBatch {
   FftBatchPtrs* ft = nullptr;  // device: rhs -> y through each member's spectrum
   FftBatchPtrs* fa = nullptr;  // plain solves: b (masked, kept in rhs) -> y
   FftBatchPtrs* fb = nullptr;  // plain solves: rhs -> x at the unknowns
+  FftBatchPtrs* fr = nullptr;  // one warm refinement: residual of x (kept in rhs) -> y
   int maxntl = 0, maxn = 0;
 };
 
@@ -724,7 +740,7 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
   CapBatch* bt = new CapBatch();
   try {
     std::vector<CapMember> t(nm);
-    std::vector<FftBatchPtrs> ft(nm), fa(nm), fb(nm);
+    std::vector<FftBatchPtrs> ft(nm), fa(nm), fb(nm), fr(nm);
     for (int i = 0; i < nm; i++) {
       CapSolver* m = c[i];
       F2D_REQUIRE(m && !m->f64, "batched capacitance solves are FP32 solvers");
@@ -742,6 +758,8 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
       fa[i].m.out = m->rhs;
       fb[i] = FftBatchPtrs{m->rhs, fft_spectrum32(m->fft), x[i], FftMask()};
       fb[i].m.res = m->mw;
+      fr[i] = fa[i];
+      fr[i].m.xr = x[i];
       if (m->ntl > bt->maxntl) bt->maxntl = m->ntl;
       if (m->n > bt->maxn) bt->maxn = m->n;
     }
@@ -749,10 +767,12 @@ CapBatch* cap_batch_create(CapSolver* const* c, int nm, const float* const* b,
     F2D_HIP(hipMalloc(&bt->ft, sizeof(FftBatchPtrs) * nm));
     F2D_HIP(hipMalloc(&bt->fa, sizeof(FftBatchPtrs) * nm));
     F2D_HIP(hipMalloc(&bt->fb, sizeof(FftBatchPtrs) * nm));
+    F2D_HIP(hipMalloc(&bt->fr, sizeof(FftBatchPtrs) * nm));
     F2D_HIP(hipMemcpy(bt->t, t.data(), sizeof(CapMember) * nm, hipMemcpyHostToDevice));
     F2D_HIP(hipMemcpy(bt->ft, ft.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
     F2D_HIP(hipMemcpy(bt->fa, fa.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
     F2D_HIP(hipMemcpy(bt->fb, fb.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
+    F2D_HIP(hipMemcpy(bt->fr, fr.data(), sizeof(FftBatchPtrs) * nm, hipMemcpyHostToDevice));
   } catch (...) {
     cap_batch_destroy(bt);
     throw;
@@ -767,6 +787,7 @@ void cap_batch_destroy(CapBatch* bt) {
   if (bt->ft) (void)hipFree(bt->ft);
   if (bt->fa) (void)hipFree(bt->fa);
   if (bt->fb) (void)hipFree(bt->fb);
+  if (bt->fr) (void)hipFree(bt->fr);
   delete bt;
 }
 
@@ -795,6 +816,14 @@ void cap_batch_solve(CapBatch* bt, int nref, bool warm, hipStream_t s) {
     // some members warm, some cold: each member's own solve
     for (int i = 0; i < nm; i++)
       cap_solve_f32(bt->c[i], bt->b[i], 0, bt->x[i], nref, true, false, nullptr, s);
+    return;
+  }
+  if (from_x && nref == 1) {  // cap_solve_f32's fused warm refinement
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->fr, s);
+    k_cap_symv_b<<<dim3((unsigned)bt->maxntl, (unsigned)nm), 256, 0, s>>>(bt->t);
+    k_cap_symv_sum_b<<<dim3(blocks(bt->maxn), (unsigned)nm), 256, 0, s>>>(bt->t);
+    fft_solve_f32_batch(bt->c[0]->fft, nm, bt->fb, s, true);
+    for (CapSolver* m : bt->c) m->solved = true;
     return;
   }
   const int passes = from_x ? (nref > 0 ? nref : 1) : (nref > 0 ? nref : 1) + 1;

@@ -291,6 +291,26 @@ __device__ __forceinline__ void put_ext(TX* x, int y, int n, const FftArgs& a, T
   }
 }
 
+// b - A x at padded point (j, i) of the operator masked to m (x read at the
+// masked unknowns only, 0 elsewhere; periodic directions wrapped), with the
+// arithmetic of the capacitance solve's residual (f2d_capacitance.hip)
+__device__ __forceinline__ double xm_at(const FftArgs& a, const int8_t* m, const float* x,
+                                        int j, int i) {
+  if (!a.ex) i = i < 3 ? i + a.onx : (i >= a.onx + 3 ? i - a.onx : i);
+  if (!a.ey) j = j < 3 ? j + a.ony : (j >= a.ony + 3 ? j - a.ony : j);
+  const int64_t k = (int64_t)j * a.NXP + i;
+  return m[k] ? (double)x[k] : 0.0;
+}
+__device__ __forceinline__ double masked_res(const FftArgs& a, double g, const int8_t* m,
+                                             const float* x, int64_t k) {
+  if (!m[k]) return 0.0;
+  const int j = (int)(k / a.NXP), i = (int)(k - (int64_t)j * a.NXP);
+  const double ax = a.cx * (xm_at(a, m, x, j, i - 1) + xm_at(a, m, x, j, i + 1)) +
+                    a.cy * (xm_at(a, m, x, j - 1, i) + xm_at(a, m, x, j + 1, i)) -
+                    a.d * (double)x[k];
+  return g - ax;
+}
+
 // Rows forward.  Without x64/x32: the rows of b (or of c2v(b)); with bpart
 // non-NULL each workgroup also writes the FP64 sum of its values of b (the
 // mean of b for the singular operator).  With x64 (or x32): the rows of
@@ -348,8 +368,8 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
       if (sg) {
         const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
         double g = gval(b, k, a.NXP, gmode);
-        if (fm.in) {  // b at the masked unknowns only, kept for a later solve
-          if (!fm.in[k]) g = 0.0;
+        if (fm.in) {  // b (or the residual) at the masked unknowns only, kept
+          g = fm.xr ? masked_res(a, g, fm.in, fm.xr, k) : (fm.in[k] ? g : 0.0);
           reinterpret_cast<TB*>(fm.out)[k] = TB(g);
         }
         // sums of b over the problem's points (the singular even case)
@@ -387,8 +407,13 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
         va = gval(b, ka, a.NXP, gmode);
         vb = gval(b, ka + a.NXP, a.NXP, gmode);
         if (fm.in) {
-          if (!fm.in[ka]) va = 0.0;
-          if (!fm.in[ka + a.NXP]) vb = 0.0;
+          if (fm.xr) {
+            va = masked_res(a, va, fm.in, fm.xr, ka);
+            vb = masked_res(a, vb, fm.in, fm.xr, ka + a.NXP);
+          } else {
+            if (!fm.in[ka]) va = 0.0;
+            if (!fm.in[ka + a.NXP]) vb = 0.0;
+          }
           reinterpret_cast<TB*>(fm.out)[ka] = TB(va);
           reinterpret_cast<TB*>(fm.out)[ka + a.NXP] = TB(vb);
         }
@@ -641,7 +666,9 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
         if (!ext_unknown(a, y, n)) continue;
         const double val = h ? vb : va;
         const int64_t k = (int64_t)(y + 3) * a.NXP + n + 3;
-        if (MODE == 3) {
+        if (MODE == 3 && fm.res) {
+          if (fm.res[k]) x[k] = TX((double)x[k] + val);  // the masked unknowns only
+        } else if (MODE == 3) {
           const double nv = (double)x[k] + val;
           if (x64) put_ext<double>(x64, y, n, a, nv);
           put_ext<TX>(x, y, n, a, TX(nv));
@@ -658,7 +685,11 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
       }
       continue;
     }
-    if (MODE == 3) {
+    if (MODE == 3 && fm.res) {
+      const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
+      if (fm.res[ka]) x[ka] = TX((double)x[ka] + va);  // the masked unknowns only
+      if (fm.res[ka + a.NXP]) x[ka + a.NXP] = TX((double)x[ka + a.NXP] + vb);
+    } else if (MODE == 3) {
       const int64_t ka = (int64_t)(ya + 3) * a.NXP + n + 3;
       const double na = (double)x[ka] + va, nb = (double)x[ka + a.NXP] + vb;
       if (x64) {
@@ -1115,7 +1146,8 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
 // One FP32 spectral solve (rows forward, columns, rows inverse; x = the
 // result) for each of `nm` same-size members, `f` the plan of one of them
 // (its args, twiddles and eigenvalues), `table` (device) their arrays.
-void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s) {
+void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s,
+                         bool acc) {
   F2D_REQUIRE(f->S32, "FFT solver built without FP32 buffers");
   FftArgs a = f->a;
   a.P = f->P32;
@@ -1127,12 +1159,16 @@ void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStr
       table);
   k_fft_cols<float><<<grc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(a, nullptr, f->twy32,
                                                                      f->lamx, f->lamy, table);
-  k_fft_rows_inv<float, float, 0><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
-      a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, FftMask(), table);
+  if (acc)
+    k_fft_rows_inv<float, float, 3><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+        a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, FftMask(), table);
+  else
+    k_fft_rows_inv<float, float, 0><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+        a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, FftMask(), table);
 }
 
 void fft_solve_f32_masked(FftSolver* f, const float* b, int gmode, float* x, const FftMask& m,
-                          hipStream_t s) {
+                          hipStream_t s, bool acc) {
   F2D_REQUIRE(f->S32, "FFT solver built without FP32 buffers");
   F2D_REQUIRE(!m.in || m.out, "a masked right-hand side needs its output array");
   FftArgs a = f->a;
@@ -1143,13 +1179,19 @@ void fft_solve_f32_masked(FftSolver* f, const float* b, int gmode, float* x, con
   mf.res = nullptr;
   mi.in = nullptr;
   mi.out = nullptr;
+  mi.xr = nullptr;
+  F2D_REQUIRE(!acc || mi.res, "an accumulating masked solve needs its mask");
   k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
       a, b, gmode, (float2*)f->S32, (const float2*)f->twx32, nullptr, nullptr, nullptr, gr,
       nullptr, mf);
   k_fft_cols<float><<<gc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(
       a, (float2*)f->S32, (const float2*)f->twy32, f->lamx, f->lamy);
-  k_fft_rows_inv<float, float, 0><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
-      a, (const float2*)f->S32, (const float2*)f->twx32, x, nullptr, nullptr, nullptr, mi);
+  if (acc)
+    k_fft_rows_inv<float, float, 3><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+        a, (const float2*)f->S32, (const float2*)f->twx32, x, nullptr, nullptr, nullptr, mi);
+  else
+    k_fft_rows_inv<float, float, 0><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+        a, (const float2*)f->S32, (const float2*)f->twx32, x, nullptr, nullptr, nullptr, mi);
 }
 
 // whether two plans solve the same operator (same args and eigenvalues)

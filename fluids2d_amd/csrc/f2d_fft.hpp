@@ -60,12 +60,15 @@ void fft_solve_f32_refine(FftSolver* f, const float* b, int gmode, float* x,
 // each member's right-hand side b (FP32, padded, gmode 0), spectrum S and
 // solution x; `f` is one member's plan (fft_same_plan holds for all).
 // Masks of a spectral solve inside the capacitance solve (all optional):
-// the forward row pass reads b only where in[k] (0 elsewhere) and writes
-// that masked b to out (b's type); the inverse row pass (plain result)
-// writes x only where res[k], without periodic halo images.
+// the forward row pass reads b only where in[k] (0 elsewhere) -- with xr,
+// the FP64 residual b - A x of the operator masked to in (x float, read at
+// in only) -- and writes that right-hand side to out (b's type); the
+// inverse row pass writes (plain) or adds (accumulating) its result to x
+// only where res[k], without periodic halo images.
 struct FftMask {
   const int8_t* in = nullptr;
   void* out = nullptr;
+  const float* xr = nullptr;
   const int8_t* res = nullptr;
 };
 struct FftBatchPtrs {
@@ -74,10 +77,12 @@ struct FftBatchPtrs {
   void* x;
   FftMask m;
 };
-void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s);
+// acc: the inverse row pass adds its result to x in FP64 (x = (float)(x + e))
+void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStream_t s,
+                         bool acc = false);
 // one plain FP32 spectral solve with masks (x = the result, gmode as b's)
 void fft_solve_f32_masked(FftSolver* f, const float* b, int gmode, float* x, const FftMask& m,
-                          hipStream_t s);
+                          hipStream_t s, bool acc = false);
 bool fft_same_plan(const FftSolver* f, const FftSolver* g);
 float2* fft_spectrum32(FftSolver* f);
 

@@ -128,6 +128,16 @@ def test_capacitance_warm_fp32(dev, L):
         print(f"\nsolve {k}: passes={it} relres={rr:.2e} err={errs[-1]:.2e}")
         assert it == (2 if k == 0 else 1)
     assert max(errs) <= 1e-7
+    # the same warm refinement without statistics, fused into the spectral
+    # passes (residual in the forward row pass, accumulation in the inverse)
+    b = b0 + 0.05 * rng.standard_normal(m.shape) * mesh.mskv
+    x_ref = np.zeros(m.shape)
+    ref.solve(b, x_ref)
+    L.call("f2d_poisson_solve", h, L.ptr(torch.from_numpy(b).to(dev, torch.float32)), L.ptr(dx),
+           1e-12, 10, None, None, L.stream_ptr())
+    err = np.linalg.norm(dx.double().cpu().numpy() - x_ref) / np.linalg.norm(x_ref)
+    print(f"\nfused warm solve: err={err:.2e}")
+    assert err <= 1e-7
     # a plain FP32 solve (no refinement): FP32 accuracy of the solution
     L.call("f2d_poisson_set_refine", h, 0)
     b = b0 + 0.1 * rng.standard_normal(m.shape) * mesh.mskv
