@@ -251,11 +251,12 @@ def basin_mask(n, seed, ny=None):
     return m
 
 
-def ensemble_member(n, seed, precision="fp32", model="qgrsw", dtmax=0.05):
+def ensemble_member(n, seed, precision="fp32", model="qgrsw", dtmax=0.05, solver=None):
     """One synthetic member: `model` on an n x n closed basin (Lx = Ly = 1,
     f0 = 10, g = H = 1), h = H + seeded Gaussian bumps, balanced by the QG
     projection (geos_adj.py:37-49).  dtmax keeps f0 dt = 0.5: the CFL step of
-    model.py:84-87 sees only the slow advective velocity."""
+    model.py:84-87 sees only the slow advective velocity.  solver: the
+    Param.poisson_solver to ask for (None: the default, "auto")."""
     import fluids2d_amd as f2d
     import fluids2d_amd.param as fp
     from fluids2d_amd.rsw import qg_projection
@@ -267,6 +268,8 @@ def ensemble_member(n, seed, precision="fp32", model="qgrsw", dtmax=0.05):
     p.dtmax = dtmax
     p.tend = 1e9
     p.maxite = 10 ** 9
+    if solver is not None:
+        p.poisson_solver = solver
     m = f2d.Model(p)
     m.mesh.msk[...] = basin_mask(n, seed)
     m.mesh.finalize()
@@ -645,6 +648,40 @@ def rsw_bench(seed, steps, warmup=20):
     return out
 
 
+def masked_bench(n, seed, solver, steps, warmup=6):
+    """eulerpsi in an n x n closed smoothed-noise basin (the ensemble's
+    members' masks and vorticity bumps, dt from the CFL condition alone):
+    step times in FP64 and FP32 (Param.poisson_refine default) with the
+    Poisson solver `solver` ("cap": the capacitance solve; "mg": multigrid
+    CG), the precisions' timed windows interleaved, and the solver's
+    iteration counts."""
+    import torch
+    out = {"workload": f"eulerpsi {n}x{n} closed smoothed-noise basin (seed {seed}), "
+                       f"Poisson solver {solver!r}", "steps": steps, "windows": WINDOWS}
+    models = {}
+    for prec in ("fp64", "fp32"):
+        m = ensemble_member(n, seed, prec, model="eulerpsi", dtmax=9e99, solver=solver)
+        out[f"{prec}_method"] = m.mesh.poisson_vertices.method
+        m.integrator.begin(m.state)
+        warm(m, warmup)
+        models[prec] = m
+    windows = {prec: [] for prec in models}
+    for _ in range(WINDOWS):
+        for prec, m in models.items():
+            windows[prec].append(timed_window(m, steps))
+    for prec, ws in windows.items():
+        ms = sum(w[0] for w in ws) / len(ws)
+        st = ws[-1][2]
+        out[f"{prec}_ms_per_step"] = ms
+        out[f"{prec}_windows_ms_per_step"] = [w[0] for w in ws]
+        out[f"{prec}_cell_steps_per_s"] = n * n / (ms * 1e-3)
+        out[f"{prec}_solver"] = {k: st[k] for k in ("max_iters", "last_iters", "last_relres",
+                                                    "last_refinements", "failed")}
+    del models
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -674,6 +711,9 @@ def main():
                     help="time only the rsw island channel (for profiling its kernels)")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="time only the qgrsw ensemble (for profiling its kernels)")
+    ap.add_argument("--only-masked", choices=["cap", "mg"], default=None,
+                    help="time only eulerpsi in a closed smoothed-noise basin with this "
+                         "Poisson solver (--nx sets the size, --steps the window)")
     ap.add_argument("--no-tune", action="store_true",
                     help="ensemble: default stream count and solver schedules (no tuner "
                          "runs)")
@@ -701,6 +741,9 @@ def main():
 
     if a.only_rsw:
         print(json.dumps({"rsw": rsw_bench(a.seed, a.rsw_steps or a.steps, a.warmup)}))
+        return
+    if a.only_masked:
+        print(json.dumps({"masked": masked_bench(a.nx, 7, a.only_masked, a.steps)}))
         return
     if a.only_ensemble:
         print(json.dumps({"ensemble": ensemble_bench(
