@@ -34,6 +34,7 @@ EXPORTS = [
     "f2d_poisson_set_refine", "f2d_nonfinite",
     "f2d_record_begin", "f2d_record_end", "f2d_recording_size", "f2d_recording_destroy",
     "f2d_launch_batch_create", "f2d_launch_batch_run", "f2d_launch_batch_destroy",
+    "f2d_transport_stage_cfl", "f2d_cfl_partials", "f2d_perpgrad_part", "f2d_cfl_finish",
 ]
 
 
@@ -100,7 +101,7 @@ def sw_stage(integrator, stage, dt, prev=(), out=(None, None, None), y=(None, No
     return st
 
 
-ABI_VERSION = 7  # f2d_abi_version() of the library these bindings describe
+ABI_VERSION = 8  # f2d_abi_version() of the library these bindings describe
 
 
 _lib = None
@@ -132,6 +133,11 @@ def lib():
                                       vp, vp, vp, i32, i32, vp, vp]),
         "f2d_transport_stage_log": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
                                           vp, vp, vp, i32, i32, vp, vp, vp]),
+        "f2d_transport_stage_cfl": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, vp, vp,
+                                          vp, vp, vp, i32, i32, vp, vp, i64, d, d, vp, vp]),
+        "f2d_cfl_partials": (i32, [gp, ctypes.POINTER(i64)]),
+        "f2d_perpgrad_part": (i32, [i32, gp, vp, vp, vp, vp, vp, d, d, i32, vp, i64, vp]),
+        "f2d_cfl_finish": (i32, [vp, i64, vp, vp, d, d, vp]),
         "f2d_transport_stage_psi": (i32, [i32, gp, i32, vp, vp, vp, vp, vp, d, d, i32, vp,
                                           vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "f2d_rk_addto": (i32, [i32, vp, vp, vp, vp, i64, i32, i32, vp, vp]),

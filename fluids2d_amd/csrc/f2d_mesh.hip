@@ -122,7 +122,7 @@ using namespace f2d;
 
 extern "C" {
 
-int f2d_abi_version(void) { return 7; }
+int f2d_abi_version(void) { return 8; }
 
 const char* f2d_last_error(void) { return g_last_error.c_str(); }
 

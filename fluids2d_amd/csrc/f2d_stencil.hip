@@ -94,6 +94,63 @@ __global__ void k_innerproduct(T* __restrict__ ke, const T* __restrict__ U,
   ke[i] = ke[i] + f * Um;
 }
 
+__device__ __forceinline__ double nanmax(double a, double b) {
+  return (a > b || a != a) ? a : b;
+}
+
+// block max of v over the 256 threads (NaN-propagating, order-free):
+// butterflies within each 64-lane wave, then the 4 wave results
+__device__ __forceinline__ double block_nanmax256(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
+  __syncthreads();
+  return r;
+}
+
+// The CFL step of model.py:84-87 from k_perpgrad's block maxima part[0..nb)
+// of |Ux| and part[nb..2nb) of |Uy|: dt = min(cfl / (max|Ux| + max|Uy| +
+// 1e-99), dtmax).  The maxima are exact and order-free, so every caller
+// (k_dt_fin's one workgroup, or each transport workgroup) gets the same dt.
+// Called by all 256 threads of the workgroup.
+// this thread's share (entries t, t + 256, ...) of the maxima of the block
+// maxima part[0..nb) and part[nb..2nb); its loads are all issued before
+// any is used
+__device__ __forceinline__ void cfl_partial_max(const double* __restrict__ part, int nb,
+                                                double& ax, double& ay) {
+  constexpr int U = 8;
+  for (int b0 = 0; b0 < nb; b0 += 256 * U) {
+    double vx[U], vy[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int b = b0 + u * 256 + (int)threadIdx.x;
+      vx[u] = b < nb ? part[b] : 0.0;
+      vy[u] = b < nb ? part[nb + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      ax = nanmax(ax, vx[u]);
+      ay = nanmax(ay, vy[u]);
+    }
+  }
+}
+
+__device__ __forceinline__ double cfl_of(double ax, double ay, double cfl, double dtmax) {
+  const double v = cfl / ((ax + ay) + 1e-99);
+  return (dtmax < v) ? dtmax : v;  // Python min(v, dtmax)
+}
+
+__device__ __forceinline__ double cfl_step256(const double* __restrict__ part, int nb,
+                                              double cfl, double dtmax, double* sh) {
+  double ax = 0.0, ay = 0.0;
+  cfl_partial_max(part, nb, ax, ay);
+  ax = block_nanmax256(ax, sh);
+  ay = block_nanmax256(ay, sh);
+  return cfl_of(ax, ay, cfl, dtmax);
+}
+
 // ---------------------------------------------------------------------------
 // Fused transport stage.
 //
@@ -126,6 +183,14 @@ struct TransportArgs {
   int nhb;  // halo-ring blocks (block indices 0 .. nhb-1)
   double* dt_log;  // nullable: *dt copied there (the step's time step)
   int64_t nhalo;
+  // nullable: the stage's dt is the CFL step of these k_perpgrad block
+  // maxima (2 cnb entries), formed by every workgroup; workgroup 0 writes
+  // it to dt_w (and dt_log) for the later stages
+  const double* cpart;
+  int cnb;
+  double ccfl, cdtmax;
+  double* dt_w;
+  double dtv;  // the stage's dt, set by the kernel at its start
   // U = perpgrad(psi) formed on the fly instead of read (psi non-NULL):
   // masks, contravariant factors and the periodic halo fill of k_perpgrad
   const T* psi;
@@ -207,7 +272,7 @@ __device__ __forceinline__ void stage_update(const TransportArgs<T>& a,
   if (a.dso) a.dso[k] = ds;
   if (a.integ < 0) return;
   double c[3];
-  const int nc = rk_coefs(a.integ, a.stage, *a.dt, c);
+  const int nc = rk_coefs(a.integ, a.stage, a.dtv, c);
   T x0 = (nc == 1) ? ds : a.ds0[k];
   T x1 = (nc == 2) ? ds : (nc > 2 ? a.ds1[k] : T(0));
   T x2 = (nc == 3) ? ds : T(0);
@@ -256,6 +321,21 @@ __device__ __forceinline__ int tile_of(int b, int nt) {
   return (nt & 7) ? b : (b & 7) * (nt >> 3) + (b >> 3);
 }
 
+// the stage's dt into a.dtv at a transport kernel's start (all 256 threads):
+// read from a.dt, or formed from CFL partials (then workgroup 0 stores it
+// in dt_w); workgroup 0 also copies it to dt_log
+template <class T>
+__device__ __forceinline__ void stage_dt(TransportArgs<T>& a) {
+  __shared__ double shd[4];
+  if (a.cpart) {
+    a.dtv = cfl_step256(a.cpart, a.cnb, a.ccfl, a.cdtmax, shd);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.dt_w = a.dtv;
+  } else {
+    a.dtv = a.dt ? *a.dt : 0.0;
+  }
+  if (a.dt_log && blockIdx.x == 0 && threadIdx.x == 0) *a.dt_log = a.dtv;
+}
+
 // B: the batched instantiation (a = tab[blockIdx.y]: one member of a
 // recorded member batch, launch_rec)
 template <class T, int M, int TY, int TX, bool B = false>
@@ -268,7 +348,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a,
   __shared__ T sfy[(TY + 1) * TX];
   const int t = threadIdx.x;
   const int P = a.P;
-  if (a.dt_log && blockIdx.x == 0 && t == 0) *a.dt_log = *a.dt;
+  stage_dt(a);
 
   // the halo-ring blocks come first: they start with the tiles instead of
   // trailing them
@@ -316,7 +396,7 @@ __global__ __launch_bounds__(256) void k_transport(TransportArgs<T> a,
     }
     // the stage's earlier tendencies, as many as its coefficients need
     double cf[3] = {0.0, 0.0, 0.0};
-    const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, *a.dt, cf);
+    const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, a.dtv, cf);
     int8_t mv[NCL];
     T d0[NCL], d1[NCL];
 #pragma unroll
@@ -413,9 +493,10 @@ __global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a,
   __shared__ T sq[NQR * LW];
   __shared__ T sfx[TY * (TX + 1)];
   __shared__ T sfy[(TY + 1) * TX];
+  __shared__ double shd[8];
   const int t = threadIdx.x;
-  if (a.dt_log && blockIdx.x == 0 && t == 0) *a.dt_log = *a.dt;
   if ((int)blockIdx.x < a.nhb) {  // halo-ring blocks first, as in k_transport
+    stage_dt(a);
     transport_halo<T, M>(a, (int64_t)blockIdx.x * 256 + t);
     return;
   }
@@ -438,10 +519,18 @@ __global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a,
   const int hc = hcc < 3 ? hcc : TX + hcc;  // LDS column
   T qh = T(0);
   if (hq) qh = a.q[(uint32_t)(j0 - 3 + hr) * P + (uint32_t)(i0 - 3 + hc)];
+  // the stage's dt: read, or formed from the CFL partials, whose loads go
+  // out with the tile's; the 4 wave maxima meet at the tile's first barrier
+  double pax = 0.0, pay = 0.0;
+  if (a.cpart)
+    cfl_partial_max(a.cpart, a.cnb, pax, pay);
+  else
+    a.dtv = a.dt ? *a.dt : 0.0;
   int8_t ox[NR], oy[NR], mv[NR];
   T ux[NR], uy[NR], d0[NR], d1[NR];
   double cf[3] = {0.0, 0.0, 0.0};
-  const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, *a.dt, cf);
+  // (the number of terms does not depend on dt; the coefficients follow)
+  const int nc = a.integ < 0 ? 0 : rk_coefs(a.integ, a.stage, 0.0, cf);
 #pragma unroll
   for (int i = 0; i < NR; i++) {
     const uint32_t k = oc + (uint32_t)(RP * i) * P;
@@ -478,7 +567,25 @@ __global__ __launch_bounds__(256) void k_transport_full(TransportArgs<T> a,
     if (NQR % RP == 0 || rr < NQR) sq[rr * LW + c + 3] = qv[i];
   }
   if (hq) sq[hr * LW + hc] = qh;
+  if (a.cpart) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      pax = nanmax(pax, __shfl_xor(pax, o, 64));
+      pay = nanmax(pay, __shfl_xor(pay, o, 64));
+    }
+    if ((t & 63) == 0) {
+      shd[t >> 6] = pax;
+      shd[4 + (t >> 6)] = pay;
+    }
+  }
   __syncthreads();
+  if (a.cpart) {
+    a.dtv = cfl_of(nanmax(nanmax(shd[0], shd[1]), nanmax(shd[2], shd[3])),
+                   nanmax(nanmax(shd[4], shd[5]), nanmax(shd[6], shd[7])), a.ccfl, a.cdtmax);
+    if (blockIdx.x == 0 && t == 0) *a.dt_w = a.dtv;
+  }
+  if (a.dt_log && blockIdx.x == 0 && t == 0) *a.dt_log = a.dtv;
+  if (nc > 0) rk_coefs(a.integ, a.stage, a.dtv, cf);
 #pragma unroll
   for (int i = 0; i < NR; i++) {
     const int r = r0 + RP * i;
@@ -647,23 +754,7 @@ struct PerpArgs {
   double* partials;  // nullable: 2 * nblocks block maxima of |Ux|, |Uy|
 };
 
-__device__ __forceinline__ double nanmax(double a, double b) {
-  return (a > b || a != a) ? a : b;
-}
-
 constexpr int PG_PTS = 4;  // points per thread of k_perpgrad
-
-// block max of v over the 256 threads (NaN-propagating, order-free):
-// butterflies within each 64-lane wave, then the 4 wave results
-__device__ __forceinline__ double block_nanmax256(double v, double* sh) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const double r = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
-  __syncthreads();
-  return r;
-}
 
 template <class T>
 __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
@@ -941,7 +1032,8 @@ static void launch_perpgrad(int dtype, const f2d_grid* g, const void* psi, void*
     a.partials = part;
     k_perpgrad<T><<<nb, 256, 0, S(stream)>>>(a);
   });
-  if (part) k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, maxabs, dt, cfl, dtmax);
+  if (part && (maxabs || dt))
+    k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, maxabs, dt, cfl, dtmax);
 }
 
 extern "C" {
@@ -992,11 +1084,18 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
                                 const int8_t* ocx, const int8_t* ocy, const int8_t* msk,
                                 const void* ds_prev0, const void* ds_prev1, void* ds_out,
                                 int integrator, int stage, const double* dt, void* stream,
-                                double* dt_log = nullptr, bool from_u = false) {
+                                double* dt_log = nullptr, bool from_u = false,
+                                const double* cpart = nullptr, int64_t npart = 0,
+                                double ccfl = 0.0, double cdtmax = 0.0) {
   return guarded_rec([&] {
     check_grid(g);
     check_method(method);
     F2D_REQUIRE(!dt_log || dt, "dt_log needs dt");
+    if (cpart) {
+      F2D_REQUIRE(integrator >= 0, "a stage dt from CFL partials needs an integrator");
+      F2D_REQUIRE(npart == 2 * (int64_t)perpgrad_blocks((g->ny + 6) * (g->nx + 6)),
+                  "npart must be f2d_cfl_partials() of this grid");
+    }
     if (psi) {
       F2D_REQUIRE(mskx && msky, "mskx and msky are required with psi");
     } else {
@@ -1038,6 +1137,11 @@ static int transport_stage_impl(int dtype, const f2d_grid* g, int method,
       a.dso = (T*)ds_out;
       a.dt_log = dt_log;
       a.dt = dt;
+      a.cpart = cpart;
+      a.cnb = (int)(npart / 2);
+      a.ccfl = ccfl;
+      a.cdtmax = cdtmax;
+      a.dt_w = const_cast<double*>(dt);
       a.integ = integrator;
       a.stage = stage;
       a.ny = (int)g->ny;
@@ -1096,6 +1200,22 @@ int f2d_transport_stage_log(int dtype, const f2d_grid* g, int method, const void
   return transport_stage_impl(dtype, g, method, q_in, q_out, Ux, Uy, nullptr, nullptr,
                               nullptr, 0.0, 0.0, 0, ocx, ocy, msk, ds_prev0, ds_prev1,
                               ds_out, integrator, stage, dt, stream, dt_log);
+}
+
+int f2d_transport_stage_cfl(int dtype, const f2d_grid* g, int method, const void* q_in,
+                            void* q_out, const void* Ux, const void* Uy, const int8_t* ocx,
+                            const int8_t* ocy, const int8_t* msk, const void* ds_prev0,
+                            const void* ds_prev1, void* ds_out, int integrator, int stage,
+                            double* dt, const double* part, int64_t npart, double cfl,
+                            double dtmax, double* dt_log, void* stream) {
+  if (!part) {
+    set_error("f2d_transport_stage_cfl: part is NULL");
+    return -1;
+  }
+  return transport_stage_impl(dtype, g, method, q_in, q_out, Ux, Uy, nullptr, nullptr,
+                              nullptr, 0.0, 0.0, 0, ocx, ocy, msk, ds_prev0, ds_prev1,
+                              ds_out, integrator, stage, dt, stream, dt_log, false, part,
+                              npart, cfl, dtmax);
 }
 
 int f2d_rk_addto(int dtype, void* y, const void* x0, const void* x1,
@@ -1168,6 +1288,38 @@ int f2d_perpgrad_cfl_noalloc(int dtype, const f2d_grid* g, const void* psi, void
     double* part = noalloc_scratch();
     launch_perpgrad(dtype, g, psi, Ux, Uy, mskx, msky, sx, sy, fill, part, nb, maxabs,
                     dt, cfl, dtmax, stream);
+  });
+}
+
+int f2d_cfl_partials(const f2d_grid* g, int64_t* npart) {
+  return guarded([&] {
+    check_grid(g);
+    F2D_REQUIRE(npart != nullptr, "npart is NULL");
+    *npart = 2 * (int64_t)perpgrad_blocks((g->ny + 6) * (g->nx + 6));
+  });
+}
+
+int f2d_perpgrad_part(int dtype, const f2d_grid* g, const void* psi, void* Ux, void* Uy,
+                      const int8_t* mskx, const int8_t* msky, double sx, double sy, int fill,
+                      double* part, int64_t npart, void* stream) {
+  return guarded([&] {
+    check_grid(g);
+    const int nb = perpgrad_blocks((g->ny + 6) * (g->nx + 6));
+    F2D_REQUIRE(part != nullptr, "part is NULL");
+    F2D_REQUIRE(npart == 2 * (int64_t)nb, "npart must be f2d_cfl_partials() of this grid");
+    launch_perpgrad(dtype, g, psi, Ux, Uy, mskx, msky, sx, sy, fill, part, nb, nullptr,
+                    nullptr, 0.0, 0.0, stream);
+  });
+}
+
+int f2d_cfl_finish(const double* part, int64_t npart, double* maxabs, double* dt, double cfl,
+                   double dtmax, void* stream) {
+  return guarded([&] {
+    F2D_REQUIRE(part != nullptr, "part is NULL");
+    F2D_REQUIRE(npart > 0 && npart % 2 == 0 && npart / 2 < (1 << 30),
+                "npart must be f2d_cfl_partials() of the grid");
+    F2D_REQUIRE(maxabs || dt, "maxabs and dt are both NULL");
+    k_dt_fin<<<1, 256, 0, S(stream)>>>(part, (int)(npart / 2), maxabs, dt, cfl, dtmax);
   });
 }
 

@@ -224,7 +224,7 @@ class Ensemble:
     # -- stepping -------------------------------------------------------------------
     def _member_step(self, i, dst):
         integ = self.models[i].integrator
-        integ._select_dt()
+        integ._select_dt(defer=True)
         integ.advance(stats=False, dt_log=dst)
 
     def _step_all(self, col):
@@ -272,8 +272,11 @@ class Ensemble:
             return False  # a member's steps check on the host: eager only
         if any(getattr(m.integrator, "forcings", None) for m in self.models):
             return False  # host forcing code runs at every step: eager only
+        # (a member whose next CFL step is pending as block maxima captures
+        # its steps in that state; see RKIntegrator._select_dt)
         return self._eager >= 2 and all(
-            m.integrator._dtn_valid or not m.integrator._wants_dtn() for m in self.models)
+            (m.integrator._dtn_valid or not m.integrator._wants_dtn()) and
+            (m.integrator._dtn_part or not m.integrator._defers()) for m in self.models)
 
     def steps(self, nsteps):
         """Advance every member by nsteps steps; returns the (members, nsteps)

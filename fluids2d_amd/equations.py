@@ -33,6 +33,9 @@ class _Base:
 class EulerPsi(_Base):
     """Euler in vorticity-streamfunction form (equations.py:269-282)."""
     logs_dt = True  # stage(..., dt_log=) records the step's dt
+    # stage(..., cfl_out=) leaves the next step's CFL step as block maxima,
+    # stage(..., cfl_in=) finishes it inside the first transport
+    defers_cfl = True
     prognostic = ("omega",)
 
     def rhs(self, s, ds):
@@ -53,20 +56,25 @@ class EulerPsi(_Base):
         op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
                     cfl=p.cfl, dtmax=p.dtmax)
 
-    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None):
+    def stage(self, s, alt, ds_prev, ds_out, k, dt, dt_next, stats=True, dt_log=None,
+              cfl_in=None, cfl_out=None):
         """One RK stage: transport + update, then the diag.  U is stored
         only by the step's last stage (it feeds the CFL step and the state):
         a later stage forms U = perpgrad(psi) inside its transport kernel,
         bit for bit the same velocity, and the earlier stages skip the
         perpgrad pass.  dt_log (stage 0): the transport kernel also copies
-        the step's dt there."""
+        the step's dt there.  cfl_out (last stage, instead of dt_next): the
+        CFL step's block maxima are left in that array; cfl_in (stage 0):
+        the transport forms this step's dt from such maxima and stores it in
+        dt (the same value, without a launch of its own)."""
         m, p = self.mesh, self.param
         if k > 0:
             op.transport_stage_psi(m, p.compflux, s.omega, alt.omega, s.psi, True, True,
                                    ds_prev, ds_out, p.integrator, k, dt)
         else:
             op.transport_stage(m, p.compflux, s.omega, alt.omega, s.U, ds_prev, ds_out,
-                               p.integrator, k, dt, dt_log)
+                               p.integrator, k, dt, dt_log, cfl_part=cfl_in, cfl=p.cfl,
+                               dtmax=p.dtmax)
         s, alt = s._replace(omega=alt.omega), alt._replace(omega=s.omega)
         last = k == _lib.NSTAGES[p.integrator] - 1
         if p.precision != "fp64" and getattr(p, "poisson_refine", "every") == "last" and \
@@ -81,8 +89,11 @@ class EulerPsi(_Base):
             m.poisson_vertices.set_refine(None if (last or stats) else 0)
         m.poisson_vertices.solve_c2v(s.omega, s.psi, stats=stats)
         if last:
-            op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
-                        cfl=p.cfl, dtmax=p.dtmax)
+            if cfl_out is not None:
+                op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, cfl_part=cfl_out)
+            else:
+                op.perpgrad(m, s.psi, s.U, contravariant=True, fill=True, dt_dev=dt_next,
+                            cfl=p.cfl, dtmax=p.dtmax)
         return s, alt
 
     def materialize(self, s, names=None):

@@ -65,6 +65,14 @@ class RKIntegrator:
         self._dt_written = None  # (value, buffer) of the last fixed dt written
         self._h_dt = torch.zeros(1, dtype=torch.float64).pin_memory()
         self._dtn_valid = False
+        # Models with ``defers_cfl``: the last stage leaves the next step's
+        # CFL step as block maxima in _cfl (_dtn_part: d_dtn is not written
+        # yet); after the swap (_dt_part) the first stage of the step forms
+        # it from them and writes d_dt.  Any other reader of the dt buffers
+        # finishes it first (_finish_cfl).
+        self._cfl = None
+        self._dtn_part = False
+        self._dt_part = False
         self.__doc__ = (self.__doc__ or "") + f"\n{rkdocs[param.integrator]}"
         self.rhs = self._host_rhs
         self.diag = self._host_diag
@@ -131,7 +139,7 @@ class RKIntegrator:
         self.model.diag(self.dstate)
         self.model.materialize(self.dstate)
         download(self.dstate, s)
-        self._dtn_valid = False
+        self._dtn_valid = self._dtn_part = self._dt_part = False
 
     def step(self, state, time):
         """RKIntegrator.step (integrators.py:76-79): one step of time.dt on
@@ -146,7 +154,7 @@ class RKIntegrator:
     def begin(self, state):
         """Host state -> device twin."""
         upload(state, self.dstate)
-        self._dtn_valid = False
+        self._dtn_valid = self._dtn_part = self._dt_part = False
 
     def end(self, state, names=None):
         """Device twin -> host state."""
@@ -170,6 +178,9 @@ class RKIntegrator:
         if not self._dtn_valid:
             op.cfl_dt(self.model.cfl_velocity(self.dstate), p.cfl, p.dtmax, self.d_dtn)
             self._dtn_valid = True
+        elif self._dtn_part:
+            self._finish_cfl(self.d_dtn)
+        self._dtn_part = self._dt_part = False
         self.d_dt, self.d_dtn = self.d_dtn, self.d_dt
         self._h_dt.copy_(self.d_dt, non_blocking=True)
         torch.cuda.current_stream().synchronize()
@@ -188,6 +199,21 @@ class RKIntegrator:
     def _wants_dtn(self):
         p = self.param
         return not (p.dt > 0 or p.model == "rsw")
+
+    def _defers(self):
+        return self._wants_dtn() and getattr(self.model, "defers_cfl", False) \
+            and not self.forcings
+
+    def _finish_cfl(self, dt_dev):
+        """The CFL step pending in _cfl into dt_dev (one small launch)."""
+        p = self.param
+        op.cfl_finish(self._cfl, dt_dev, p.cfl, p.dtmax)
+
+    def _dt_ready(self):
+        """d_dt written, for a path that reads it directly."""
+        if self._dt_part:
+            self._finish_cfl(self.d_dt)
+            self._dt_part = False
 
     def nonfinite(self):
         """Names of the prognostic fields of the device state with a
@@ -213,12 +239,22 @@ class RKIntegrator:
         s, alt = self.dstate, self.dalt
         dtn = self.d_dtn if self._wants_dtn() else None
         if self.forcings:
+            self._dt_ready()
             if dt_log is not None:
                 self._log_dt(dt_log)
             for k in range(self.nstages):
                 self._forced_stage(s, k, dtn if k == self.nstages - 1 else None)
             self._dtn_valid = dtn is not None
+            self._dtn_part = False
             return
+        defer = self._defers()
+        if not defer:
+            self._dt_ready()
+        elif self._cfl is None:
+            if torch.cuda.is_current_stream_capturing():
+                raise RuntimeError("the CFL partials must be allocated before a capture")
+            self._cfl = torch.zeros(op.cfl_partials(self.mesh), dtype=torch.float64,
+                                    device=self.d_dt.device)
         logs = getattr(self.model, "logs_dt", False)
         if dt_log is not None and not logs:
             self._log_dt(dt_log)
@@ -227,11 +263,18 @@ class RKIntegrator:
             prev = [self.dscratch[j] for j in range(k)]
             last = k == self.nstages - 1
             kw = {"dt_log": dt_log} if (k == 0 and logs and dt_log is not None) else {}
+            if k == 0 and self._dt_part:
+                kw["cfl_in"] = self._cfl
+            if last and defer:
+                kw["cfl_out"] = self._cfl
             s, alt = self.model.stage(s, alt, [_one(x) for x in prev],
                                       _one(ds_out) if ds_out is not None else None,
-                                      k, self.d_dt, dtn if last else None, stats, **kw)
+                                      k, self.d_dt, None if (defer or not last) else dtn,
+                                      stats, **kw)
         self.dstate, self.dalt = s, alt
+        self._dt_part = False
         self._dtn_valid = dtn is not None
+        self._dtn_part = defer
 
     def staged(self, dt_log=None):
         """advance(stats=False, dt_log=dt_log) cut at each stage's elliptic
@@ -241,6 +284,7 @@ class RKIntegrator:
         stage).  Running pre, the solve, post, cfl for k = 0, 1, ... is the
         step.  The dt is the one in d_dt now (call after _select_dt)."""
         assert not self.forcings and hasattr(self.model, "stage_split")
+        self._dt_ready()
         if dt_log is not None and not getattr(self.model, "logs_dt", False):
             self._log_dt(dt_log)
             dt_log = None
@@ -255,16 +299,20 @@ class RKIntegrator:
                                                dtn if last else None,
                                                dt_log if k == 0 else None))
         self._dtn_valid = dtn is not None
+        self._dtn_part = False
         return plan
 
     def device_step(self, time):
         self.advance()
         time.pushforward()
 
-    def _select_dt(self):
+    def _select_dt(self, defer=False):
         """The dt of the next step into d_dt, on the device (Model.set_dt,
-        model.py:71-87)."""
+        model.py:71-87).  defer (for a following ``advance``): a CFL step
+        pending as block maxima stays so, and the step's first stage forms
+        it."""
         p = self.param
+        self._dt_part = False
         if p.dt > 0:
             self._write_dt(p.dt)
         elif p.model == "rsw":
@@ -274,7 +322,12 @@ class RKIntegrator:
             self._dt_written = None
             if not self._dtn_valid:
                 op.cfl_dt(self.model.cfl_velocity(self.dstate), p.cfl, p.dtmax, self.d_dtn)
+                self._dtn_part = False
+            elif self._dtn_part and not (defer and self._defers()):
+                self._finish_cfl(self.d_dtn)
+                self._dtn_part = False
             self.d_dt, self.d_dtn = self.d_dtn, self.d_dt
+            self._dt_part, self._dtn_part = self._dtn_part, False
 
     def device_steps(self, nsteps, stats=False, graph=None):
         """nsteps steps with dt kept on the device (no host round trip).
@@ -292,7 +345,8 @@ class RKIntegrator:
         n = 0
         while n < nsteps:
             if use and nsteps - n >= G and self._eager_steps >= 2 and \
-                    (self._dtn_valid or not self._wants_dtn()):
+                    (self._dtn_valid or not self._wants_dtn()) and \
+                    (self._dtn_part or not self._defers()):
                 g = self._graph()
                 g.replay()
                 for e, k in self._graph_solves:
@@ -301,7 +355,7 @@ class RKIntegrator:
                           _lib.stream_ptr())
                 n += G
                 continue
-            self._select_dt()
+            self._select_dt(defer=True)
             self.advance(stats=stats, dt_log=dts[n:n + 1])
             self._eager_steps += 1
             n += 1
@@ -315,8 +369,9 @@ class RKIntegrator:
         steps taken (0 to 2 eager steps; the capture itself advances
         nothing)."""
         n = 0
-        while self._eager_steps < 2 or (self._wants_dtn() and not self._dtn_valid):
-            self._select_dt()
+        while self._eager_steps < 2 or (self._wants_dtn() and not self._dtn_valid) or \
+                (self._defers() and not self._dtn_part):
+            self._select_dt(defer=True)
             self.advance(stats=False)
             self._eager_steps += 1
             n += 1
@@ -339,7 +394,7 @@ class RKIntegrator:
         return (self.mesh.version, id(self.model),
                 tuple(t.data_ptr() for t in leaves(self.dstate)),
                 tuple(t.data_ptr() for t in leaves(self.dalt)),
-                self.d_dt.data_ptr(), self.d_dtn.data_ptr())
+                self.d_dt.data_ptr(), self.d_dtn.data_ptr(), self._dtn_part)
 
     def _solvers(self):
         m = self.mesh
@@ -366,7 +421,7 @@ class RKIntegrator:
         g = torch.cuda.CUDAGraph()
         with _lib.capture(g):
             for i in range(G):
-                self._select_dt()
+                self._select_dt(defer=True)
                 self.advance(stats=False, dt_log=self._glog[i:i + 1])
         if self._graph_key() != key:
             raise RuntimeError("graph capture did not return the state buffers to "
@@ -408,10 +463,14 @@ class LFRAIntegrator(RKIntegrator):
                   _lib.ptr(xs[1]), _lib.ptr(xs[2]), y.numel(), len(terms), float(cs[0]),
                   float(cs[1]), float(cs[2]), _lib.ptr(self.d_dt), dtmask, _lib.stream_ptr())
 
+    def _defers(self):
+        return False  # the step's dt comes from f2d_cfl_dt
+
     def advance(self, stats=True, dt_log=None):
         p = self.param
         s = self.dstate
         dtn = self.d_dtn if self._wants_dtn() else None
+        self._dt_ready()
         if dt_log is not None:
             self._log_dt(dt_log)
         ds = self.dscratch[0]
@@ -437,6 +496,7 @@ class LFRAIntegrator(RKIntegrator):
         if dtn is not None:
             op.cfl_dt(self.model.cfl_velocity(s), p.cfl, p.dtmax, dtn)
         self._dtn_valid = dtn is not None
+        self._dtn_part = False
 
     def device_step(self, time):
         self._first = time.ite == 0

@@ -41,17 +41,23 @@ def transport_rhs(mesh, method, q, U, dq):
 
 
 def transport_stage(mesh, method, q_in, q_out, U, ds_prev, ds_out, integrator, stage,
-                    dt_dev, dt_log=None):
+                    dt_dev, dt_log=None, cfl_part=None, cfl=0.0, dtmax=0.0):
     """Fused rhs + fill + RK addto of one stage (see f2d_transport_stage);
     with dt_log (a one-element float64 device tensor) the kernel also
-    copies dt there (f2d_transport_stage_log)."""
+    copies dt there (f2d_transport_stage_log).  cfl_part: the block maxima
+    perpgrad(..., cfl_part=) left; the kernel forms the CFL step of
+    model.py:84-87 from them (with cfl, dtmax), uses it and stores it in
+    dt_dev (f2d_transport_stage_cfl)."""
     d0 = ds_prev[0] if len(ds_prev) > 0 else None
     d1 = ds_prev[1] if len(ds_prev) > 1 else None
     args = (_dt(q_in), mesh.grid_ref(), _lib.METHODS[method],
             ptr(q_in), ptr(q_out), ptr(U.x), ptr(U.y), ptr(mesh.d_oc.x), ptr(mesh.d_oc.y),
             ptr(mesh.d_msk), ptr(d0), ptr(d1), ptr(ds_out), _lib.INTEGRATORS[integrator],
             stage, ptr(dt_dev))
-    if dt_log is None:
+    if cfl_part is not None:
+        call("f2d_transport_stage_cfl", *args, ptr(cfl_part), cfl_part.numel(), float(cfl),
+             float(dtmax), ptr(dt_log), stream_ptr())
+    elif dt_log is None:
         call("f2d_transport_stage", *args, stream_ptr())
     else:
         call("f2d_transport_stage_log", *args, ptr(dt_log), stream_ptr())
@@ -97,15 +103,37 @@ def centerstovertices(mesh, omega, vomega, addto=False):
 
 
 def perpgrad(mesh, psi, U, contravariant=False, fill=False, dt_dev=None, cfl=0.0,
-             dtmax=0.0):
-    """operators.py:144-149 (+ fill, + the CFL step of model.py:84-87)."""
+             dtmax=0.0, cfl_part=None):
+    """operators.py:144-149 (+ fill, + the CFL step of model.py:84-87 into
+    dt_dev; or, with cfl_part, its block maxima left in that device array
+    of cfl_partials(mesh) entries, for cfl_finish or the next stage)."""
     if contravariant:
         sx, sy = 1 / mesh.dy ** 2, 1 / mesh.dx ** 2
     else:
         sx, sy = 1.0, 1.0
+    if cfl_part is not None:
+        assert dt_dev is None
+        call("f2d_perpgrad_part", _dt(psi), mesh.grid_ref(), ptr(psi), ptr(U.x), ptr(U.y),
+             ptr(mesh.d_mskx), ptr(mesh.d_msky), sx, sy, int(fill), ptr(cfl_part),
+             cfl_part.numel(), stream_ptr())
+        return
     call("f2d_perpgrad_ex", _dt(psi), mesh.grid_ref(), ptr(psi), ptr(U.x), ptr(U.y),
          ptr(mesh.d_mskx), ptr(mesh.d_msky), sx, sy, int(fill), None, ptr(dt_dev),
          float(cfl), float(dtmax), stream_ptr())
+
+
+def cfl_partials(mesh):
+    """Entries of the CFL block maxima that perpgrad(..., cfl_part=) writes."""
+    n = ctypes.c_int64()
+    call("f2d_cfl_partials", mesh.grid_ref(), ctypes.byref(n))
+    return int(n.value)
+
+
+def cfl_finish(part, dt_dev, cfl, dtmax):
+    """The CFL step of model.py:84-87 into dt_dev from perpgrad's block
+    maxima (bit for bit perpgrad(..., dt_dev=))."""
+    call("f2d_cfl_finish", ptr(part), part.numel(), None, ptr(dt_dev), float(cfl),
+         float(dtmax), stream_ptr())
 
 
 def fill(mesh, a):

@@ -155,6 +155,22 @@ int f2d_transport_stage_u(int dtype, const f2d_grid* g, int method, const void* 
                           const void* ds_prev1, void* ds_out, int integrator, int stage,
                           const double* dt, void* stream);
 
+/* f2d_transport_stage_log whose dt is the CFL step of model.py:84-87 formed
+ * from the block maxima `part` that f2d_perpgrad_part left (npart entries,
+ * f2d_cfl_partials of this grid) with `cfl` and `dtmax`: every workgroup
+ * forms it (the maxima are exact, so all get the same dt, bit for bit the
+ * one f2d_cfl_finish would write), uses it for this stage, and stores it
+ * in *dt for the step's later stages (and in *dt_log if not NULL).  The
+ * step loop of model.py:52-69 then needs no launch of its own for the dt:
+ * the last stage's perpgrad leaves the maxima, the next step's first stage
+ * finishes them. */
+int f2d_transport_stage_cfl(int dtype, const f2d_grid* g, int method, const void* q_in,
+                            void* q_out, const void* Ux, const void* Uy, const int8_t* ocx,
+                            const int8_t* ocy, const int8_t* msk, const void* ds_prev0,
+                            const void* ds_prev1, void* ds_out, int integrator, int stage,
+                            double* dt, const double* part, int64_t npart, double cfl,
+                            double dtmax, double* dt_log, void* stream);
+
 /* addto, integrators.py:154-196 (the sum of :174): over n entries
  * y += ((c0*x0 + c1*x1) + c2*x2), with the coefficients of `integrator`
  * stage `stage` evaluated from *dt as above. */
@@ -202,6 +218,24 @@ int f2d_perpgrad_cfl_noalloc(int dtype, const f2d_grid* g, const void* psi, void
                              void* Uy, const int8_t* mskx, const int8_t* msky,
                              double sx, double sy, int fill, double* maxabs,
                              double* dt, double cfl, double dtmax, void* stream);
+/* The CFL step of Model.set_dt (model.py:84-87) in two parts.
+ * f2d_cfl_partials: the number of float64 entries (*npart) of the block
+ * maxima of |Ux| and |Uy| that f2d_perpgrad_part writes for grid g.
+ * f2d_perpgrad_part: f2d_perpgrad_ex's gradient, with the block maxima
+ * left in the caller's device array `part` (npart entries) instead of
+ * being reduced.  f2d_cfl_finish: the reduction, dt = min(cfl / (max|Ux| +
+ * max|Uy| + 1e-99), dtmax) into *dt and the maxima into maxabs[2] (either
+ * may be NULL), bit for bit f2d_perpgrad_ex's; f2d_transport_stage_cfl
+ * forms the same dt inside the next stage instead. */
+int f2d_cfl_partials(const f2d_grid* g, int64_t* npart);
+/* perpgrad, operators.py:144-149, with the block maxima of the CFL step
+ * (model.py:84-87) left in part */
+int f2d_perpgrad_part(int dtype, const f2d_grid* g, const void* psi, void* Ux, void* Uy,
+                      const int8_t* mskx, const int8_t* msky, double sx, double sy, int fill,
+                      double* part, int64_t npart, void* stream);
+/* the CFL step of model.py:84-87 from those block maxima */
+int f2d_cfl_finish(const double* part, int64_t npart, double* maxabs, double* dt, double cfl,
+                   double dtmax, void* stream);
 /* fill_halo_array, meshes.py:135-143 (+ the y-periodic extension) */
 int f2d_fill_halo(int dtype, const f2d_grid* g, void* a, void* stream);
 /* Model.set_dt, model.py:71-87 (non-rsw branch):

@@ -52,7 +52,7 @@ def test_library_exports_every_symbol():
     L = _lib.lib()
     for n in names:
         getattr(L, n)
-    assert L.f2d_abi_version() == 7 == _lib.ABI_VERSION
+    assert L.f2d_abi_version() == 8 == _lib.ABI_VERSION
 
 
 def test_library_is_gfx950_code():
