@@ -35,6 +35,7 @@ EXPORTS = [
     "f2d_record_begin", "f2d_record_end", "f2d_recording_size", "f2d_recording_destroy",
     "f2d_launch_batch_create", "f2d_launch_batch_run", "f2d_launch_batch_destroy",
     "f2d_transport_stage_cfl", "f2d_cfl_partials", "f2d_perpgrad_part", "f2d_cfl_finish",
+    "f2d_cfl_dt_part",
 ]
 
 
@@ -150,6 +151,7 @@ def lib():
                                            d, d, vp]),
         "f2d_fill_halo": (i32, [i32, gp, vp, vp]),
         "f2d_cfl_dt": (i32, [i32, vp, vp, i64, d, d, vp, vp]),
+        "f2d_cfl_dt_part": (i32, [i32, vp, vp, i64, d, d, vp, i64, vp, vp]),
         "f2d_set_scalar": (i32, [vp, d, vp]),
         "f2d_copy_f64": (i32, [vp, vp, i32, vp]),
         "f2d_reserve_stream_scratch": (i32, [vp]),

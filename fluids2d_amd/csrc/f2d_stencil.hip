@@ -875,6 +875,46 @@ __global__ void k_fill(T* __restrict__ a, int ny, int nx, int xper, int yper) {
   a[k] = a[(int64_t)js * NX + is];
 }
 
+// f2d_cfl_dt with its block maxima in the caller's array (one per member of
+// a recorded member batch): the same maxima, the same dt
+struct CflArgs {
+  const void* Ux;
+  const void* Uy;
+  int64_t n;
+  double* part;  // 2 nb entries
+  int nb;
+  double cfl, dtmax;
+  double* dt;
+};
+
+template <class T, bool B = false>
+__global__ __launch_bounds__(256) void k_absmax_r(CflArgs a, const CflArgs* tab) {
+  if (B) a = tab[blockIdx.y];
+  __shared__ double sh[4];
+  const T* __restrict__ Ux = (const T*)a.Ux;
+  const T* __restrict__ Uy = (const T*)a.Uy;
+  double mx = 0.0, my = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < a.n;
+       k += (int64_t)a.nb * 256) {
+    mx = nanmax(mx, fabs((double)Ux[k]));
+    my = nanmax(my, fabs((double)Uy[k]));
+  }
+  mx = block_nanmax256(mx, sh);
+  my = block_nanmax256(my, sh);
+  if (threadIdx.x == 0) {
+    a.part[blockIdx.x] = mx;
+    a.part[a.nb + blockIdx.x] = my;
+  }
+}
+
+template <bool B = false>
+__global__ __launch_bounds__(256) void k_dt_fin_r(CflArgs a, const CflArgs* tab) {
+  if (B) a = tab[blockIdx.y];
+  __shared__ double sh[4];
+  const double dt = cfl_step256(a.part, a.nb, a.cfl, a.dtmax, sh);
+  if (threadIdx.x == 0) *a.dt = dt;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_absmax(const T* __restrict__ Ux,
                                                 const T* __restrict__ Uy,
@@ -1358,6 +1398,22 @@ int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
     F2D_DTYPE_DISPATCH(dtype, k_absmax<T><<<nb, 256, 0, S(stream)>>>(
                                   (const T*)Ux, (const T*)Uy, n, part));
     k_dt_fin<<<1, 256, 0, S(stream)>>>(part, nb, nullptr, dt, cfl, dtmax);
+  });
+}
+
+int f2d_cfl_dt_part(int dtype, const void* Ux, const void* Uy, int64_t n, double cfl,
+                    double dtmax, double* part, int64_t npart, double* dt, void* stream) {
+  return guarded_rec([&] {
+    F2D_REQUIRE(n > 0, "f2d_cfl_dt_part: empty");
+    F2D_REQUIRE(Ux && Uy && dt && part, "f2d_cfl_dt_part: NULL argument");
+    int nb = grid1d(n, 256);
+    if (nb > 1024) nb = 1024;
+    F2D_REQUIRE(npart >= 2 * (int64_t)nb, "f2d_cfl_dt_part: npart must be at least 2048 "
+                                          "(or 2 ceil(n / 256))");
+    CflArgs a{Ux, Uy, n, part, nb, cfl, dtmax, dt};
+    F2D_DTYPE_DISPATCH(dtype, launch_rec(k_absmax_r<T, false>, k_absmax_r<T, true>, dim3(nb),
+                                         dim3(256), 0, S(stream), a));
+    launch_rec(k_dt_fin_r<false>, k_dt_fin_r<true>, dim3(1), dim3(256), 0, S(stream), a);
   });
 }
 

@@ -199,9 +199,9 @@ class Ensemble:
             self._launch((g, k, dts), [lambda p=p, k=k: part(p, k) for p in plans])
             if k < nst:
                 solve.solve(nref=model.refine_for(k))
-        for p in plans:
-            if p[-1][2] is not None:
-                p[-1][2]()  # the member's CFL step
+        cfls = [p[-1][2] for p in plans if p[-1][2] is not None]
+        if cfls:  # the members' CFL steps, one launch per kernel
+            self._launch((g, "cfl", dts), cfls)
 
     def _launch(self, key, parts):
         """The members' parts as one launch per kernel: the batch recorded

@@ -141,10 +141,19 @@ def fill(mesh, a):
     call("f2d_fill_halo", _dt(a), mesh.grid_ref(), ptr(a), stream_ptr())
 
 
-def cfl_dt(U, cfl, dtmax, dt_dev):
-    """Model.set_dt (model.py:84-87) from device velocities."""
+def cfl_dt(U, cfl, dtmax, dt_dev, part=None):
+    """Model.set_dt (model.py:84-87) from device velocities.  part (a
+    float64 device array of CFL_PARTS entries): the block maxima go there,
+    and the call may be recorded into a member batch (f2d_cfl_dt_part)."""
+    if part is not None:
+        call("f2d_cfl_dt_part", _dt(U.x), ptr(U.x), ptr(U.y), U.x.numel(), float(cfl),
+             float(dtmax), ptr(part), part.numel(), ptr(dt_dev), stream_ptr())
+        return
     call("f2d_cfl_dt", _dt(U.x), ptr(U.x), ptr(U.y), U.x.numel(), float(cfl),
          float(dtmax), ptr(dt_dev), stream_ptr())
+
+
+CFL_PARTS = 2048  # entries of cfl_dt's part array (any grid)
 
 
 def set_scalar(dst, value):

@@ -19,6 +19,8 @@ the transport stage of f2d_stencil.hip), bit for bit the same results:
 import ctypes
 from collections import namedtuple
 
+import torch
+
 from . import _lib
 from . import operators as op
 from .equations import _Base
@@ -43,6 +45,7 @@ class _SW(_Base):
         super().__init__(param, mesh)
         self._ds_last = None
         self._pre = None
+        self._cflpart = None  # block maxima of stage_split's CFL step
         self._stats = True  # host-checked solves unless a stage says otherwise
 
     def _grid(self):
@@ -272,8 +275,17 @@ class QGRSW(_SW):
         f2d_record_begin can record."""
         cfl = None
         if dt_next is not None:
+            if self._cflpart is None:
+                if _lib.capturing():
+                    raise RuntimeError("stage_split: CFL array not allocated before a capture")
+                self._cflpart = torch.zeros(op.CFL_PARTS, dtype=torch.float64,
+                                            device=s.h.device)
+            part = self._cflpart
+
             def cfl():
-                op.cfl_dt(self.cfl_velocity(s), self.param.cfl, self.param.dtmax, dt_next)
+                # its own block-maxima array: recordable into a member batch
+                op.cfl_dt(self.cfl_velocity(s), self.param.cfl, self.param.dtmax, dt_next,
+                          part=part)
         return ((lambda: self._stage_pre(s, k, dt, dt_log)),
                 (lambda: self._stage_post(s, ds_prev, ds_out, k, dt)), cfl)
 

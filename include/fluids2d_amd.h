@@ -243,6 +243,12 @@ int f2d_fill_halo(int dtype, const f2d_grid* g, void* a, void* stream);
  * arrays; result in *dt (device float64). */
 int f2d_cfl_dt(int dtype, const void* Ux, const void* Uy, int64_t n,
                double cfl, double dtmax, double* dt, void* stream);
+/* f2d_cfl_dt (Model.set_dt, model.py:84-87) with its block maxima in the
+ * caller's device array `part` (npart >= 2048 entries, or 2 ceil(n / 256)),
+ * bit for bit the same dt.  Unlike f2d_cfl_dt it may be recorded into a
+ * member batch (f2d_record_begin): each member brings its own part. */
+int f2d_cfl_dt_part(int dtype, const void* Ux, const void* Uy, int64_t n, double cfl,
+                    double dtmax, double* part, int64_t npart, double* dt, void* stream);
 /* dt = value (device scalar): the fixed-dt and rsw branches of Model.set_dt,
  * model.py:71-82 */
 int f2d_set_scalar(double* dst, double value, void* stream);

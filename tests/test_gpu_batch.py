@@ -162,9 +162,14 @@ def test_unrecordable_group_steps_member_by_member(dev):
     d = [np.concatenate([e.steps(3).cpu().numpy(), e.steps(4).cpu().numpy()], axis=1)
          for e in (eb, eu)]
     # the parts with the pre-solve kernels (k < 3) fell back to member-by-member
-    # launches; the last stage's post-solve part (k = 3) was recorded
+    # launches; the last stage's post-solve part (k = 3) and the CFL steps
+    # were recorded
     assert eb.graph is not None and eb._lb
-    assert all((v is None) == (key[1] < 3) for key, v in eb._lb.items())
+    for key, v in eb._lb.items():
+        if key[1] == "cfl":
+            assert v is not None
+        else:
+            assert (v is None) == (key[1] < 3), key
     assert np.isfinite(d[0]).all()
     assert np.array_equal(d[0], d[1])
     for mb, mu in zip(*runs):

@@ -110,3 +110,41 @@ def test_deferred_cfl_in_ensemble(dev):
     assert np.array_equal(da, db) and np.all(da > 0)
     for a, b in zip(sa, sb):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_cfl_dt_part_recorded_batch(dev, dtype):
+    """f2d_cfl_dt_part gives f2d_cfl_dt's dt, and a recorded batch of three
+    members' calls (one launch per kernel) gives each member's own."""
+    from fluids2d_amd import _lib
+    from fluids2d_amd import operators as op
+    from fluids2d_amd.ensemble import LaunchBatch
+    from fluids2d_amd.states import Vector
+    rng = np.random.default_rng(8)
+    n = 262 * 262
+    Us, parts, dts, refs = [], [], [], []
+    for i in range(3):
+        ux = torch.from_numpy(rng.standard_normal(n) * (i + 1)).to(dev, dtype)
+        uy = torch.from_numpy(rng.standard_normal(n) * 0.5).to(dev, dtype)
+        Us.append(Vector(ux, uy))
+        parts.append(torch.zeros(op.CFL_PARTS, dtype=torch.float64, device=dev))
+        dts.append(torch.zeros(1, dtype=torch.float64, device=dev))
+        refs.append(torch.zeros(1, dtype=torch.float64, device=dev))
+    for U, r in zip(Us, refs):
+        op.cfl_dt(U, 0.9, 1e9, r)
+    for U, pt, d in zip(Us, parts, dts):
+        op.cfl_dt(U, 0.9, 1e9, d, part=pt)
+    torch.cuda.synchronize()
+    assert [d.item() for d in dts] == [r.item() for r in refs]
+    assert len({r.item() for r in refs}) == 3
+    for d in dts:
+        d.zero_()
+    lb = LaunchBatch([lambda U=U, pt=pt, d=d: op.cfl_dt(U, 0.9, 1e9, d, part=pt)
+                      for U, pt, d in zip(Us, parts, dts)])
+    torch.cuda.synchronize()
+    assert all(d.item() == 0.0 for d in dts)  # recording launched nothing
+    lb.run()
+    torch.cuda.synchronize()
+    assert [d.item() for d in dts] == [r.item() for r in refs]
+    with pytest.raises(_lib.F2DError, match="npart"):
+        op.cfl_dt(Us[0], 0.9, 1e9, dts[0], part=parts[0][:100])
