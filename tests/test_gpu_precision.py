@@ -10,7 +10,9 @@ the two workloads that run in that precision, each with its error bar:
 * a qgrsw member of the ensemble workload (closed smoothed-noise basin,
   f0 = 10, g = H = 1, Gaussian h bumps balanced by the QG projection,
   geos_adj.py:37-49) at 64^2 and 96x128 for 200 steps: relative L2 of u and
-  h <= 1e-5.
+  h <= 1e-5;
+* the Euler workload at its full 1024^2 size for 1000 steps against the
+  FP64 device path: relative L2 of the vorticity <= 1e-5.
 
 The oracle runs the reference's algorithm in FP64 (SuperLU solves).  The
 device runs FP32 arithmetic, stepped two ways: ``Model.run`` (host-checked
@@ -182,3 +184,17 @@ def test_qgrsw_basin_fp32_refined(dev, n, ny, seed, stepping):
     print(f"  after {nsteps} steps: h rel L2 {eh:.2e}, u rel L2 {eu:.2e}, "
           f"time rel {dt_err:.1e} (bar 1e-5)")
     assert eh <= 1e-5 and eu <= 1e-5
+
+
+def test_euler_periodic_1024_fp32_vs_fp64_1000_steps(dev):
+    """The headline workload at its full size (1024^2 doubly periodic
+    eulerpsi, k^-3 vorticity, seed 2024): FP32 + refinement against the FP64
+    device path (held to the oracle at the sizes the oracle runs) after 1000
+    steps at a common fixed dt, relative L2 of the vorticity <= 1e-5, and the
+    FP64 bulk diagnostics of the two runs close."""
+    import bench
+    err, dt, diag = bench.accuracy(1024, 2024, 1000)
+    print(f"\n1024^2 FP32 vs FP64 after 1000 steps (dt {dt:.4g}): rel L2 omega {err:.2e} "
+          f"(bar 1e-5); diagnostics {diag}")
+    assert err <= 1e-5
+    assert all(abs(v) <= 1e-6 for v in diag.values())
