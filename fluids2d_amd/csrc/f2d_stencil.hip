@@ -767,17 +767,37 @@ __global__ __launch_bounds__(256) void k_perpgrad(PerpArgs<T> a) {
   const int G = gridDim.x, b = blockIdx.x;
   const int64_t chunk = (b & 7) * (G >> 3) + (b >> 3);
   double mxv = 0.0, myv = 0.0;
+  // (j, i) of the first point by one division; each later point is 256
+  // further on, and NX > 256 (nx >= 251) keeps it within the next row
+  int j = 0, i = 0;
+  {
+    const int64_t k0 = chunk * PG_PTS * 256 + threadIdx.x;
+    if (n < ((int64_t)1 << 31)) {
+      j = (int)((uint32_t)k0 / (uint32_t)NX);
+      i = (int)((uint32_t)k0 - (uint32_t)j * (uint32_t)NX);
+    } else {
+      j = (int)(k0 / NX);
+      i = (int)(k0 - (int64_t)j * NX);
+    }
+  }
   for (int r = 0; r < PG_PTS; r++) {
     int64_t k = (chunk * PG_PTS + r) * 256 + threadIdx.x;
-    if (k >= n) break;
-    int j, i;
-    if (n < ((int64_t)1 << 31)) {
-      j = (int)((uint32_t)k / (uint32_t)NX);
-      i = (int)((uint32_t)k - (uint32_t)j * (uint32_t)NX);
-    } else {
-      j = (int)(k / NX);
-      i = (int)(k - (int64_t)j * NX);
+    if (r > 0) {
+      if (NX > 256) {
+        i += 256;
+        if (i >= NX) {
+          i -= NX;
+          j++;
+        }
+      } else if (n < ((int64_t)1 << 31)) {
+        j = (int)((uint32_t)k / (uint32_t)NX);
+        i = (int)((uint32_t)k - (uint32_t)j * (uint32_t)NX);
+      } else {
+        j = (int)(k / NX);
+        i = (int)(k - (int64_t)j * NX);
+      }
     }
+    if (k >= n) break;
     int js = j, is = i;
     if (a.fill) {
       if (a.xper) is = (i < 3) ? i + a.nx : (i >= a.nx + 3 ? i - a.nx : i);
