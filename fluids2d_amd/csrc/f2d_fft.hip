@@ -301,10 +301,11 @@ __device__ __forceinline__ double xm_at(const FftArgs& a, const int8_t* m, const
   const int64_t k = (int64_t)j * a.NXP + i;
   return m[k] ? (double)x[k] : 0.0;
 }
+// (j, i): the padded point k = j NXP + i, as the caller knows it
 __device__ __forceinline__ double masked_res(const FftArgs& a, double g, const int8_t* m,
-                                             const float* x, int64_t k) {
+                                             const float* x, int j, int i) {
+  const int64_t k = (int64_t)j * a.NXP + i;
   if (!m[k]) return 0.0;
-  const int j = (int)(k / a.NXP), i = (int)(k - (int64_t)j * a.NXP);
   const double ax = a.cx * (xm_at(a, m, x, j, i - 1) + xm_at(a, m, x, j, i + 1)) +
                     a.cy * (xm_at(a, m, x, j - 1, i) + xm_at(a, m, x, j + 1, i)) -
                     a.d * (double)x[k];
@@ -357,9 +358,10 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
     // an odd extension's boundary lines).  Component h of the complex row
     // pair holds row y0 + 2p + h.
     const int nh = a.ex ? a.onx : nx;
+    const int lg2 = a.ex ? lg : lg + 1;  // 2 nh = 2^lg2
     T* bs = reinterpret_cast<T*>(b0);
     for (int w = threadIdx.x; w < P * 2 * nh; w += blockDim.x) {
-      const int p = w / (2 * nh), r = w - p * (2 * nh);
+      const int p = w >> lg2, r = w & ((1 << lg2) - 1);
       const int h = r >= nh ? 1 : 0, n = r - h * nh;
       const int y = y0 + 2 * p + h;
       int yo, xo;
@@ -369,7 +371,7 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
         const int64_t k = (int64_t)(yo + 3) * a.NXP + xo + 3;
         double g = gval(b, k, a.NXP, gmode);
         if (fm.in) {  // b (or the residual) at the masked unknowns only, kept
-          g = fm.xr ? masked_res(a, g, fm.in, fm.xr, k) : (fm.in[k] ? g : 0.0);
+          g = fm.xr ? masked_res(a, g, fm.in, fm.xr, yo + 3, xo + 3) : (fm.in[k] ? g : 0.0);
           reinterpret_cast<TB*>(fm.out)[k] = TB(g);
         }
         // sums of b over the problem's points (the singular even case)
@@ -408,8 +410,8 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
         vb = gval(b, ka + a.NXP, a.NXP, gmode);
         if (fm.in) {
           if (fm.xr) {
-            va = masked_res(a, va, fm.in, fm.xr, ka);
-            vb = masked_res(a, vb, fm.in, fm.xr, ka + a.NXP);
+            va = masked_res(a, va, fm.in, fm.xr, y0 + 2 * p + 3, n + 3);
+            vb = masked_res(a, vb, fm.in, fm.xr, y0 + 2 * p + 4, n + 3);
           } else {
             if (!fm.in[ka]) va = 0.0;
             if (!fm.in[ka + a.NXP]) vb = 0.0;
@@ -431,8 +433,9 @@ __global__ __launch_bounds__(1024) void k_fft_rows_fwd(FftArgs a,
   // half spectra of the two rows: Xa = (C + D) / 2, Xb = -i (C - D) / 2 with
   // C = F(k), D = conj(F(nx - k))
   const int n2 = a.nk * 2 * P;
+  const int lgP = __ffs(P) - 1;  // P is a power of two
   for (int w = threadIdx.x; w < n2; w += blockDim.x) {
-    const int ab = w & 1, p = (w >> 1) % P, k = (w >> 1) / P;
+    const int ab = w & 1, p = (w >> 1) & (P - 1), k = (w >> 1) >> lgP;
     const cplx<T> C = r[p * nx + k];
     const cplx<T> D0 = r[p * nx + ((nx - k) & (nx - 1))];
     const cplx<T> D = mk<T>(D0.x, -D0.y);
@@ -631,8 +634,9 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
   const int y0 = row_block(blockIdx.x, gridDim.x) * 2 * P;
   // Xa -> b1, Xb -> b0 (k = 0 .. nx/2)
   const int n2 = a.nk * 2 * P;
+  const int lgP = __ffs(P) - 1;  // P is a power of two
   for (int w = threadIdx.x; w < n2; w += blockDim.x) {
-    const int ab = w & 1, p = (w >> 1) % P, k = (w >> 1) / P;
+    const int ab = w & 1, p = (w >> 1) & (P - 1), k = (w >> 1) >> lgP;
     const int64_t at = (int64_t)k * a.ny + y0 + 2 * p + ab;
     cplx<T> X;
     if (a.ex == 1) {  // odd x: the imaginary part stored
@@ -645,12 +649,12 @@ __global__ __launch_bounds__(1024) void k_fft_rows_inv(FftArgs a,
   __syncthreads();
   // conj(C) with C(k) = Xa(k) + i Xb(k) and C(nx - k) = conj(Xa(k)) +
   // i conj(Xb(k)), written over b1 (each thread reads its own k first)
-  for (int w = threadIdx.x; w < P * (h + 1); w += blockDim.x) {
-    const int p = w / (h + 1), k = w - p * (h + 1);
-    const cplx<T> Xa = b1[p * nx + k], Xb = b0[p * nx + k];
-    b1[p * nx + k] = mk<T>(Xa.x - Xb.y, -(Xa.y + Xb.x));
-    if (k > 0 && k < h) b1[p * nx + nx - k] = mk<T>(Xa.x + Xb.y, -(Xb.x - Xa.y));
-  }
+  for (int p = 0; p < P; p++)
+    for (int k = threadIdx.x; k <= h; k += blockDim.x) {
+      const cplx<T> Xa = b1[p * nx + k], Xb = b0[p * nx + k];
+      b1[p * nx + k] = mk<T>(Xa.x - Xb.y, -(Xa.y + Xb.x));
+      if (k > 0 && k < h) b1[p * nx + nx - k] = mk<T>(Xa.x + Xb.y, -(Xb.x - Xa.y));
+    }
   __syncthreads();
   const cplx<T>* r = fft_lds<T>(b1, b0, nx, lg, P, tl);
   // conj of the result: row a = Re, row b = -Im of r
