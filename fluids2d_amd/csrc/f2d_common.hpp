@@ -340,6 +340,16 @@ __device__ __forceinline__ T flux_by_order(int o, T U, T q_3, T q_2, T q_1,
 // RK stage coefficients evaluated from dt exactly as the reference's Python
 // expressions (integrators.py:82-124).  Returns the number of coefficients.
 // ---------------------------------------------------------------------------
+// 1.0 / c for the number c of wet vertices about a cell (verticestocenters,
+// operators.py:136-141), bit for bit the IEEE division: the quotients of
+// 1 .. 4 are the correctly rounded constants, 1 / 0 = +inf (numpy's
+// (1/0) * 0 = NaN follows), other counts divide
+__device__ __forceinline__ double inv_count(int c) {
+  if (c >= 1 && c <= 4)
+    return c == 4 ? 0.25 : (c == 2 ? 0.5 : (c == 1 ? 1.0 : 1.0 / 3.0));
+  return c == 0 ? __builtin_inf() : 1.0 / (double)c;
+}
+
 __host__ __device__ inline int rk_coefs(int integrator, int stage, double dt,
                                         double c[3]) {
   c[0] = c[1] = c[2] = 0.0;

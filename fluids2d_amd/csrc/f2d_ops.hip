@@ -155,7 +155,7 @@ __global__ void k_v2c_scaled(const T* __restrict__ vh, double scale,
     const int coef = (((int)mv[k] + (int)mv[k + NX]) + (int)mv[k + 1]) + (int)mv[k + NX + 1];
     const T sc = T(scale);
     const T s = (((vh[k] * sc) + (vh[k + NX] * sc)) + (vh[k + 1] * sc)) + (vh[k + NX + 1] * sc);
-    val = T((1.0 / (double)coef) * (double)s);
+    val = T(inv_count(coef) * (double)s);
   }
   h[k] = val * T(msk[k]);
 }

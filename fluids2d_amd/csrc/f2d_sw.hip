@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_qg_project(ProjArgs<T> a, const ProjArg
     const int coef = (((int)a.mv[q] + (int)a.mv[q + NX]) + (int)a.mv[q + 1]) + (int)a.mv[q + NX + 1];
     const T sc = T(a.scale);
     const T s = (((psi[q] * sc) + (psi[q + NX] * sc)) + (psi[q + 1] * sc)) + (psi[q + NX + 1] * sc);
-    dh = T((1.0 / (double)coef) * (double)s);
+    dh = T(inv_count(coef) * (double)s);
   } else {
     dh = a.ph[q];
   }
