@@ -160,11 +160,15 @@ __device__ __forceinline__ double bsum256(double v, double* sh) {
 
 // workgroup size of the FFT passes: 512 threads from 1024 points per
 // workgroup (a radix-4 pass has a butterfly for half of them), 1024 from
-// 4096, so that a CU holding one or two such workgroups (their LDS) still
-// has waves to switch between (measured: 256 threads at 1024 points made
-// the 1024^2 step 9% slower, 1024 threads at 2048 points the 2048^2 FP32
-// step 9% slower)
+// 4096 (FP64: from 2048), so that a CU holding one or two such workgroups
+// (their LDS) still has waves to switch between.  Measured: 256 threads at
+// 1024 points made the 1024^2 step 9% slower, 1024 threads at 2048 points
+// the 2048^2 FP32 step 9% slower; on the closed 1024^2 basin (2048-point
+// transforms) 1024 threads took the FP64 step from 0.680 to 0.636 ms and
+// the FP32 step from 0.425 to 0.449 ms, 256 threads made both slower.
+template <class T>
 static inline int fft_threads(int points) {
+  if (sizeof(T) == 8 && points >= 2048) return 1024;
   return points >= 4096 ? 1024 : (points >= 1024 ? 512 : 256);
 }
 
@@ -1069,10 +1073,10 @@ static void spectral(FftSolver* f, const TB* b, int gmode, TX* x, double* x64,
   // closed y: rows 0 .. ony-1 only (the rest mirror them, see k_fft_rows_fwd)
   const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
   const double* bpin = ((xres || xres32) && a.singular) ? rs_in : nullptr;
-  k_fft_rows_fwd<T, TB><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
+  k_fft_rows_fwd<T, TB><<<gr, fft_threads<T>(a.P * a.nx), lr, s>>>(a, b, gmode, S, twx, rs_out, xres, bpin, gr,
                                             xres32);
-  k_fft_cols<T><<<gc, fft_threads(a.Q * a.ny), lc, s>>>(a, S, twy, f->lamx, f->lamy);
-  k_fft_rows_inv<T, TX, MODE><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, S, twx, x, x64,
+  k_fft_cols<T><<<gc, fft_threads<T>(a.Q * a.ny), lc, s>>>(a, S, twy, f->lamx, f->lamy);
+  k_fft_rows_inv<T, TX, MODE><<<gr, fft_threads<T>(a.P * a.nx), lr, s>>>(a, S, twx, x, x64,
                                                   keep_sums ? f->rowsum[1] : nullptr,
                                                   keep_sums ? f->rowsum[0] : nullptr);
 }
@@ -1158,16 +1162,16 @@ void fft_solve_f32_batch(FftSolver* f, int nm, const FftBatchPtrs* table, hipStr
   a.Q = f->Q32;
   const int gr = (a.ey ? a.ony : a.ny) / (2 * a.P), gc = (a.ncu + a.Q - 1) / a.Q;
   const dim3 grr(gr, nm), grc(gc, nm);
-  k_fft_rows_fwd<float, float><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+  k_fft_rows_fwd<float, float><<<grr, fft_threads<float>(a.P * a.nx), f->lds_r32, s>>>(
       a, nullptr, 0, nullptr, f->twx32, nullptr, nullptr, nullptr, gr, nullptr, FftMask(),
       table);
-  k_fft_cols<float><<<grc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(a, nullptr, f->twy32,
+  k_fft_cols<float><<<grc, fft_threads<float>(a.Q * a.ny), f->lds_c32, s>>>(a, nullptr, f->twy32,
                                                                      f->lamx, f->lamy, table);
   if (acc)
-    k_fft_rows_inv<float, float, 3><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+    k_fft_rows_inv<float, float, 3><<<grr, fft_threads<float>(a.P * a.nx), f->lds_r32, s>>>(
         a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, FftMask(), table);
   else
-    k_fft_rows_inv<float, float, 0><<<grr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+    k_fft_rows_inv<float, float, 0><<<grr, fft_threads<float>(a.P * a.nx), f->lds_r32, s>>>(
         a, nullptr, f->twx32, nullptr, nullptr, nullptr, nullptr, FftMask(), table);
 }
 
@@ -1185,16 +1189,16 @@ void fft_solve_f32_masked(FftSolver* f, const float* b, int gmode, float* x, con
   mi.out = nullptr;
   mi.xr = nullptr;
   F2D_REQUIRE(!acc || mi.res, "an accumulating masked solve needs its mask");
-  k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+  k_fft_rows_fwd<float, float><<<gr, fft_threads<float>(a.P * a.nx), f->lds_r32, s>>>(
       a, b, gmode, (float2*)f->S32, (const float2*)f->twx32, nullptr, nullptr, nullptr, gr,
       nullptr, mf);
-  k_fft_cols<float><<<gc, fft_threads(a.Q * a.ny), f->lds_c32, s>>>(
+  k_fft_cols<float><<<gc, fft_threads<float>(a.Q * a.ny), f->lds_c32, s>>>(
       a, (float2*)f->S32, (const float2*)f->twy32, f->lamx, f->lamy);
   if (acc)
-    k_fft_rows_inv<float, float, 3><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+    k_fft_rows_inv<float, float, 3><<<gr, fft_threads<float>(a.P * a.nx), f->lds_r32, s>>>(
         a, (const float2*)f->S32, (const float2*)f->twx32, x, nullptr, nullptr, nullptr, mi);
   else
-    k_fft_rows_inv<float, float, 0><<<gr, fft_threads(a.P * a.nx), f->lds_r32, s>>>(
+    k_fft_rows_inv<float, float, 0><<<gr, fft_threads<float>(a.P * a.nx), f->lds_r32, s>>>(
         a, (const float2*)f->S32, (const float2*)f->twx32, x, nullptr, nullptr, nullptr, mi);
 }
 
@@ -1242,34 +1246,34 @@ double fft_bench(FftSolver* f, bool f64, int kind, int reps, hipStream_t s,
     if (f64) {
       double* buf = f->bench_buf;
       if (kind == 0)
-        k_fft_rows_fwd<double, double><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, buf, 0, f->S64, f->twx64,
+        k_fft_rows_fwd<double, double><<<gr, fft_threads<double>(a.P * a.nx), lr, s>>>(a, buf, 0, f->S64, f->twx64,
                                                            nullptr, nullptr, nullptr, gr,
                                                            nullptr);
       else if (kind == 2)
-        k_fft_cols<double><<<gc, fft_threads(a.Q * a.ny), lc, s>>>(a, f->S64, f->twy64, f->lamx, f->lamy);
+        k_fft_cols<double><<<gc, fft_threads<double>(a.Q * a.ny), lc, s>>>(a, f->S64, f->twy64, f->lamx, f->lamy);
       else
-        k_fft_rows_inv<double, double, 0><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S64, f->twx64, buf,
+        k_fft_rows_inv<double, double, 0><<<gr, fft_threads<double>(a.P * a.nx), lr, s>>>(a, f->S64, f->twx64, buf,
                                                               nullptr);
     } else {
       float* buf = reinterpret_cast<float*>(f->bench_buf);
       if (kind == 0 || kind == 1)
-        k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), lr, s>>>(
+        k_fft_rows_fwd<float, float><<<gr, fft_threads<float>(a.P * a.nx), lr, s>>>(
             a, buf, 0, f->S32, f->twx32, nullptr, kind == 1 ? f->x64 : nullptr,
             nullptr, gr, nullptr);
       else if (kind == 2)
-        k_fft_cols<float><<<gc, fft_threads(a.Q * a.ny), lc, s>>>(a, f->S32, f->twy32, f->lamx, f->lamy);
+        k_fft_cols<float><<<gc, fft_threads<float>(a.Q * a.ny), lc, s>>>(a, f->S32, f->twy32, f->lamx, f->lamy);
       else if (kind == 3)
-        k_fft_rows_inv<float, float, 0><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
+        k_fft_rows_inv<float, float, 0><<<gr, fft_threads<float>(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
                                                             nullptr);
       else if (kind == 4)
-        k_fft_rows_inv<float, float, 2><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
+        k_fft_rows_inv<float, float, 2><<<gr, fft_threads<float>(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
                                                             f->x64);
       else if (kind == 5)
-        k_fft_rows_fwd<float, float><<<gr, fft_threads(a.P * a.nx), lr, s>>>(
+        k_fft_rows_fwd<float, float><<<gr, fft_threads<float>(a.P * a.nx), lr, s>>>(
             a, buf, 0, f->S32, f->twx32, f->rowsum[1], nullptr,
             a.singular ? f->rowsum[0] : nullptr, gr, buf);
       else
-        k_fft_rows_inv<float, float, 3><<<gr, fft_threads(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
+        k_fft_rows_inv<float, float, 3><<<gr, fft_threads<float>(a.P * a.nx), lr, s>>>(a, f->S32, f->twx32, buf,
                                                             nullptr);
     }
   };
