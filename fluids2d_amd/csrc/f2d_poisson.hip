@@ -1617,7 +1617,7 @@ static void coarse_inverse(const TailLev& L, const std::vector<int8_t>& m,
 
 template <class T>
 static void build_pcg(f2d_poisson* h, Pcg<T>& P) {
-  const int tail_points = h->opts.tail_points > 0 ? h->opts.tail_points : 4096;
+  const int tail_points = h->opts.tail_points > 0 ? h->opts.tail_points : 1024;
   HostLevel l0;
   l0.d = h->base;
   l0.m = h->m0;
@@ -2245,12 +2245,12 @@ int f2d_poisson_create(const int8_t* unknowns_host, const f2d_grid* g,
     h->opts.mode = F2D_POISSON_F64;
     h->opts.nu = 2;
     h->opts.refine_inner = 4;
-    h->opts.tail_points = 4096;
+    h->opts.tail_points = 1024;
     if (opts) {
       h->opts = *opts;
       if (h->opts.nu <= 0) h->opts.nu = 2;
       if (h->opts.refine_inner <= 0) h->opts.refine_inner = 4;
-      if (h->opts.tail_points <= 0) h->opts.tail_points = 4096;
+      if (h->opts.tail_points <= 0) h->opts.tail_points = 1024;
     }
     F2D_REQUIRE(h->opts.nu == 1 || h->opts.nu == 2, "opts.nu must be 1 or 2");
     h->nu_coarse = h->opts.nu_coarse > 0 ? h->opts.nu_coarse : h->opts.nu;

@@ -431,7 +431,8 @@ typedef struct f2d_poisson_opts {
   int32_t nu;            /* RB-GS sweeps before and after (1 or 2)        */
   int32_t coarse_sweeps; /* symmetric sweeps on the coarsest level (0=auto) */
   int32_t refine_inner;  /* F32_REFINE: FP32 PCG iterations per refinement */
-  int32_t tail_points;   /* max points of the single-workgroup tail levels */
+  int32_t tail_points;   /* max points of the single-workgroup tail levels
+                            (0: 1024, levels from 32^2 down)             */
   int32_t tail_threads;  /* workgroup size of the tail kernel (256/512/1024) */
   int32_t nu_coarse;     /* sweeps on the multi-block coarse levels (0: nu) */
   int32_t nu_tail;       /* sweeps on the tail levels (0: nu)              */
