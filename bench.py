@@ -99,17 +99,23 @@ def calibrate(model, nsteps=3):
         model.time.pushforward()
 
 
-def warm(model, warmup):
+def warm(model, warmup, seconds=0.2):
     """`warmup` untimed steps, then the step graph of the buffer assignment
-    a timed window starts from is captured (if it is not yet) and replayed
-    once, untimed: a timed window replays graphs and captures none."""
+    a timed window starts from is captured (if it is not yet) and replayed,
+    untimed, for about `seconds` of GPU work: a timed window replays graphs
+    and captures none, and does not start on a GPU that has just been
+    idle."""
     import torch
     integ = model.integrator
     if warmup:
         integ.device_steps(warmup)
     integ.prepare_graph()
-    integ.device_steps(integ.graph_steps)
-    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while True:
+        integ.device_steps(8 * integ.graph_steps)
+        torch.cuda.synchronize()
+        if time.perf_counter() - t0 >= seconds:
+            break
 
 
 def timed_window(model, steps, solver="poisson_vertices"):
